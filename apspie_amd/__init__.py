@@ -1,0 +1,347 @@
+"""apspie_amd -- MI355X (gfx950) SpMSpV traversal and hash-SpGEMM library.
+
+Python binding over the C ABI in include/apspie.h (libapspie.so, built
+in-tree by apspie_amd/build.py).  Host arrays are numpy; device memory is
+owned by the library (or passed in as raw device addresses).  There is no
+CPU fallback: if the shared library is missing or fails to load, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapspie.so")
+
+SSSP_INF = np.float32(1.70141e38)
+
+RMAT_UNDIRECTED = 1
+RMAT_WEIGHTED = 2
+LOAD_UNDIRECTED = 1
+LOAD_SORT_COLUMNS = 2
+
+ERRORS = {
+    -1: "EINVAL",
+    -2: "ENOMEM",
+    -3: "EHIP",
+    -4: "EIO",
+    -5: "ELIMIT",
+    -6: "ENOCONV",
+    -7: "EBUSY",
+}
+
+
+class ApspieError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Csr(C.Structure):
+    _fields_ = [
+        ("m", C.c_int32),
+        ("n", C.c_int32),
+        ("nnz", C.c_int64),
+        ("row_ptr", C.c_void_p),
+        ("col_ind", C.c_void_p),
+        ("val", C.c_void_p),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("edges", C.c_int64),
+        ("vertices", C.c_int64),
+        ("improved", C.c_int64),
+        ("iterations", C.c_int32),
+        ("reserved", C.c_int32),
+        ("bytes_alg", C.c_double),
+        ("kernel_ms", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libapspie.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python apspie_amd/build.py` (no CPU fallback exists)")
+    L = C.CDLL(LIB_PATH)
+    P, vp, i32, i64, u32 = C.POINTER, C.c_void_p, C.c_int32, C.c_int64, C.c_uint32
+    sig = {
+        "apspie_ctx_create": [C.c_int, P(vp)],
+        "apspie_ctx_destroy": [vp],
+        "apspie_ctx_stream": [vp],
+        "apspie_ctx_synchronize": [vp],
+        "apspie_ctx_cu_count": [vp],
+        "apspie_ctx_set_profiling": [vp, C.c_int],
+        "apspie_ctx_profile_read": [vp, C.c_int, P(C.c_double), P(i64)],
+        "apspie_ctx_profile_reset": [vp],
+        "apspie_malloc": [vp, P(vp), C.c_size_t],
+        "apspie_free": [vp, vp],
+        "apspie_memcpy_h2d": [vp, vp, vp, C.c_size_t],
+        "apspie_memcpy_d2h": [vp, vp, vp, C.c_size_t],
+        "apspie_load_mtx": [C.c_char_p, u32, P(_Csr)],
+        "apspie_host_csr_free": [P(_Csr)],
+        "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
+        "apspie_csr_upload": [vp, P(_Csr), P(_Csr)],
+        "apspie_csr_free": [vp, P(_Csr)],
+        "apspie_bfs": [vp, P(_Csr), i32, vp, P(Stats)],
+        "apspie_sssp": [vp, P(_Csr), i32, vp, P(Stats)],
+        "apspie_sssp_step": [vp, P(_Csr), vp, vp, P(Stats)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.apspie_last_error.argtypes = [vp]
+    L.apspie_last_error.restype = C.c_char_p
+    L.apspie_ctx_stream.restype = vp
+    L.apspie_host_csr_free.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc: int, ctx=None, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().apspie_last_error(ctx).decode() if ctx else what
+        raise ApspieError(rc, f"{what}: {msg}" if ctx else msg)
+
+
+# ---------------------------------------------------------------------------
+# host graphs
+
+
+@dataclass
+class HostCsr:
+    """Host CSR: int32 row_ptr[m+1], int32 col_ind[nnz], optional float32 val."""
+
+    m: int
+    n: int
+    row_ptr: np.ndarray
+    col_ind: np.ndarray
+    val: Optional[np.ndarray] = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_ind.shape[0])
+
+    def degrees(self) -> np.ndarray:
+        return np.diff(self.row_ptr)
+
+    def argmax_degree(self) -> int:
+        return int(np.argmax(self.degrees())) if self.m else 0
+
+    def _as_c(self) -> _Csr:
+        self.row_ptr = np.ascontiguousarray(self.row_ptr, dtype=np.int32)
+        self.col_ind = np.ascontiguousarray(self.col_ind, dtype=np.int32)
+        if self.val is not None:
+            self.val = np.ascontiguousarray(self.val, dtype=np.float32)
+        s = _Csr()
+        s.m, s.n, s.nnz = self.m, self.n, self.nnz
+        s.row_ptr = self.row_ptr.ctypes.data
+        s.col_ind = self.col_ind.ctypes.data if self.nnz else None
+        s.val = self.val.ctypes.data if self.val is not None and self.nnz else None
+        return s
+
+
+def _host_from_c(s: _Csr) -> HostCsr:
+    m, n, nnz = s.m, s.n, s.nnz
+    rp = np.ctypeslib.as_array(C.cast(s.row_ptr, C.POINTER(C.c_int32)), shape=(m + 1,)).copy()
+    ci = np.ctypeslib.as_array(C.cast(s.col_ind, C.POINTER(C.c_int32)), shape=(max(nnz, 1),))[:nnz].copy()
+    val = None
+    if s.val:
+        val = np.ctypeslib.as_array(C.cast(s.val, C.POINTER(C.c_float)), shape=(max(nnz, 1),))[:nnz].copy()
+    lib().apspie_host_csr_free(C.byref(s))
+    return HostCsr(m, n, rp, ci, val)
+
+
+def load_mtx(path: str, undirected: bool = False, sort_columns: bool = False) -> HostCsr:
+    """readMtx + [makeSymmetric] + buildMatrix (src/util.cuh:355-643, src/matrix.cpp:55-111)."""
+    s = _Csr()
+    flags = (LOAD_UNDIRECTED if undirected else 0) | (LOAD_SORT_COLUMNS if sort_columns else 0)
+    _check(lib().apspie_load_mtx(os.fsencode(path), flags, C.byref(s)), None, f"load_mtx({path})")
+    return _host_from_c(s)
+
+
+def rmat(scale: int, edge_factor: int = 16, seed: int = 1, undirected: bool = False,
+         weight_seed: Optional[int] = None) -> HostCsr:
+    """Seeded RMAT(a=.57, b=.19, c=.19, d=.05) graph, deduplicated, sorted columns."""
+    s = _Csr()
+    flags = (RMAT_UNDIRECTED if undirected else 0) | (RMAT_WEIGHTED if weight_seed is not None else 0)
+    _check(lib().apspie_rmat(scale, edge_factor, seed, flags, weight_seed or 0, C.byref(s)), None, "rmat")
+    return _host_from_c(s)
+
+
+# ---------------------------------------------------------------------------
+# device side
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library; freed with free() or on GC."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().apspie_malloc(ctx.h, C.byref(p), max(self.nbytes, 1)), ctx.h, "malloc")
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().apspie_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), self.ctx.h, "h2d")
+        return self
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().apspie_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), self.ctx.h, "d2h")
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ctx.h:
+            lib().apspie_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCsr:
+    """CSR resident in HBM (library-owned)."""
+
+    def __init__(self, ctx: "Context", h: HostCsr):
+        self.ctx = ctx
+        self.m, self.n, self.nnz = h.m, h.n, h.nnz
+        self.weighted = h.val is not None
+        self._c = _Csr()
+        hc = h._as_c()
+        _check(lib().apspie_csr_upload(ctx.h, C.byref(hc), C.byref(self._c)), ctx.h, "csr_upload")
+
+    @property
+    def c(self) -> _Csr:
+        return self._c
+
+    def free(self) -> None:
+        if self._c.row_ptr and self.ctx.h:
+            lib().apspie_csr_free(self.ctx.h, C.byref(self._c))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """A device, a HIP stream and a scratch pool (apspie_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().apspie_ctx_create(device, C.byref(h)), None, "apspie_ctx_create")
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().apspie_ctx_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cu_count(self) -> int:
+        return lib().apspie_ctx_cu_count(self.h)
+
+    def synchronize(self) -> None:
+        _check(lib().apspie_ctx_synchronize(self.h), self.h, "synchronize")
+
+    # -- per-kernel-class event timing ---------------------------------------
+    KCLASS = {"expand": 0, "compact": 1, "other": 2}
+
+    def set_profiling(self, on: bool) -> None:
+        _check(lib().apspie_ctx_set_profiling(self.h, int(on)), self.h, "set_profiling")
+
+    def profile_reset(self) -> None:
+        _check(lib().apspie_ctx_profile_reset(self.h), self.h, "profile_reset")
+
+    def profile(self) -> dict:
+        """{kernel class: (device ms, launches)} since the last reset."""
+        out = {}
+        for name, k in self.KCLASS.items():
+            ms, n = C.c_double(), C.c_int64()
+            _check(lib().apspie_ctx_profile_read(self.h, k, C.byref(ms), C.byref(n)), self.h, "profile_read")
+            out[name] = (ms.value, n.value)
+        return out
+
+    def upload(self, h: HostCsr) -> DeviceCsr:
+        return DeviceCsr(self, h)
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    # -- plain path --------------------------------------------------------
+    def bfs_into(self, g: DeviceCsr, src: int, d_depth: DeviceBuffer) -> Stats:
+        st = Stats()
+        _check(lib().apspie_bfs(self.h, C.byref(g.c), src, d_depth.ptr, C.byref(st)), self.h, "bfs")
+        return st
+
+    def bfs(self, g: DeviceCsr, src: int):
+        """Returns (depth int32[m], Stats); -1 marks unreached vertices."""
+        d = self.alloc(4 * max(g.m, 1))
+        st = self.bfs_into(g, src, d)
+        out = d.download(np.int32, g.m)
+        d.free()
+        return out, st
+
+    def sssp_into(self, g: DeviceCsr, src: int, d_dist: DeviceBuffer) -> Stats:
+        st = Stats()
+        _check(lib().apspie_sssp(self.h, C.byref(g.c), src, d_dist.ptr, C.byref(st)), self.h, "sssp")
+        return st
+
+    def sssp(self, g: DeviceCsr, src: int):
+        """Returns (dist float32[m], Stats); SSSP_INF marks unreachable vertices."""
+        d = self.alloc(4 * max(g.m, 1))
+        st = self.sssp_into(g, src, d)
+        out = d.download(np.float32, g.m)
+        d.free()
+        return out, st
+
+    def sssp_step(self, g: DeviceCsr, x: np.ndarray):
+        """One min-plus SpMSpV step y = min-plus(A^T, x); returns (y, Stats)."""
+        dx = self.alloc(4 * max(g.m, 1)).upload(np.asarray(x, dtype=np.float32))
+        dy = self.alloc(4 * max(g.m, 1))
+        st = Stats()
+        _check(lib().apspie_sssp_step(self.h, C.byref(g.c), dx.ptr, dy.ptr, C.byref(st)), self.h, "sssp_step")
+        y = dy.download(np.float32, g.m)
+        dx.free()
+        dy.free()
+        return y, st
+
+
+def exported_symbols(header: str) -> Sequence[str]:
+    """Function names declared in a C header (for ABI completeness checks)."""
+    import re
+
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(apspie_[a-z0-9_]+)\s*\(", text)))

@@ -1,0 +1,101 @@
+// device.hpp -- wave64 building blocks shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace apspie {
+namespace dev {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ uint64_t ld_acquire_agent(const uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint64_t y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t x, int lane) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+    }
+    return x;
+}
+
+// Sum across the wave; every lane receives the total.
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t x) {
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
+    return x;
+}
+
+// Decoupled look-back over per-workgroup status words (Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016), done by
+// one full wave: 64 predecessors are inspected per step with a ballot.
+// Status word: bits 62-63 flag (0 none, 1 aggregate, 2 inclusive prefix),
+// bits 0-61 the value.  Values are sums that must stay below 2^62.
+// Called by all 64 lanes of one wave; returns the exclusive prefix of `bid`.
+__device__ __forceinline__ uint64_t lookback_exclusive(uint64_t *status, int bid, uint64_t agg,
+                                                       int lane) {
+    constexpr uint64_t FLAG_AGG = 1ull << 62, FLAG_INC = 2ull << 62, VMASK = (1ull << 62) - 1;
+    if (bid == 0) {
+        if (lane == 0) st_agent(&status[0], FLAG_INC | agg);
+        return 0;
+    }
+    if (lane == 0) st_agent(&status[bid], FLAG_AGG | agg);
+    uint64_t excl = 0;
+    int base = bid - 1;
+    for (;;) {
+        const int idx = base - lane;
+        uint64_t s;
+        for (;;) {
+            s = idx >= 0 ? ld_acquire_agent(&status[idx]) : FLAG_INC;
+            if (__all((s >> 62) != 0)) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t inc = __ballot((s >> 62) == 2);
+        if (inc) {
+            const int first = __ffsll((unsigned long long)inc) - 1;
+            excl += wave_sum(lane <= first ? (s & VMASK) : 0);
+            break;
+        }
+        excl += wave_sum(s & VMASK);
+        base -= WAVE;
+    }
+    if (lane == 0) st_agent(&status[bid], FLAG_INC | (excl + agg));
+    return excl;
+}
+
+// Bit i set iff byte i of the 8-byte word is non-zero.
+__device__ __forceinline__ uint32_t nonzero_bytes8(uint2 w) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        b |= ((w.x >> (8 * i)) & 0xffu) ? (1u << i) : 0u;
+        b |= ((w.y >> (8 * i)) & 0xffu) ? (1u << (i + 4)) : 0u;
+    }
+    return b;
+}
+
+}  // namespace dev
+}  // namespace apspie

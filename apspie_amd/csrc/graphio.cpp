@@ -1,0 +1,345 @@
+// graphio.cpp -- host-side graph construction: the MatrixMarket loader with
+// the reference's semantics and the seeded RMAT generator.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace apspie;
+
+namespace {
+
+int hw_threads() {
+    unsigned n = std::thread::hardware_concurrency();
+    if (const char *e = getenv("OMP_NUM_THREADS")) n = std::min<unsigned>(n ? n : 1, (unsigned)atoi(e));
+    return (int)std::max(1u, std::min(n ? n : 1u, 16u));
+}
+
+// Run f(lo, hi) over [0, n) split into contiguous ranges, one per thread.
+template <class F>
+void parallel_ranges(int64_t n, F &&f) {
+    int nt = hw_threads();
+    if (n < 65536) nt = 1;
+    if (nt == 1) {
+        f(int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) {
+        int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+        th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    for (auto &x : th) x.join();
+}
+
+void host_csr_alloc(apspie_host_csr *h, int32_t m, int32_t n, int64_t nnz, bool with_val) {
+    memset(h, 0, sizeof(*h));
+    h->m = m;
+    h->n = n;
+    h->nnz = nnz;
+    h->row_ptr = (int32_t *)malloc((size_t(m) + 1) * 4);
+    h->col_ind = (int32_t *)malloc(size_t(std::max<int64_t>(nnz, 1)) * 4);
+    h->val = with_val ? (float *)malloc(size_t(std::max<int64_t>(nnz, 1)) * 4) : nullptr;
+    if (!h->row_ptr || !h->col_ind || (with_val && !h->val)) {
+        apspie_host_csr_free(h);
+        throw Error(APSPIE_ENOMEM, "host CSR allocation failed");
+    }
+}
+
+// Sort each row's columns (and values) in place, rows split across threads.
+void sort_rows(apspie_host_csr *h) {
+    parallel_ranges(h->m, [&](int64_t lo, int64_t hi) {
+        std::vector<std::pair<int32_t, float>> tmp;
+        for (int64_t r = lo; r < hi; ++r) {
+            int32_t b = h->row_ptr[r], e = h->row_ptr[r + 1];
+            if (!h->val) {
+                std::sort(h->col_ind + b, h->col_ind + e);
+                continue;
+            }
+            tmp.clear();
+            for (int32_t p = b; p < e; ++p) tmp.emplace_back(h->col_ind[p], h->val[p]);
+            std::stable_sort(tmp.begin(), tmp.end(),
+                             [](const auto &x, const auto &y) { return x.first < y.first; });
+            for (int32_t p = b; p < e; ++p) {
+                h->col_ind[p] = tmp[p - b].first;
+                h->val[p] = tmp[p - b].second;
+            }
+        }
+    });
+}
+
+// MT19937 (Matsumoto & Nishimura 1998) producing 624 tempered words per
+// refill, so the twist and the tempering run as vectorisable loops; the
+// output stream equals std::mt19937 and the reference's genrand_int32
+// (src/mt19937ar.cpp:102-137) for the same init_genrand seed.
+class BlockMT {
+    uint32_t s_[624], out_[624];
+    int idx_ = 624;
+    static uint32_t mix(uint32_t a, uint32_t b, uint32_t c) {
+        const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+        return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
+    }
+    void refill() {
+        for (int i = 0; i < 227; ++i) s_[i] = mix(s_[i], s_[i + 1], s_[i + 397]);
+        for (int i = 227; i < 623; ++i) s_[i] = mix(s_[i], s_[i + 1], s_[i - 227]);
+        s_[623] = mix(s_[623], s_[0], s_[396]);
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = s_[i];
+            y ^= y >> 11;
+            y ^= (y << 7) & 0x9d2c5680u;
+            y ^= (y << 15) & 0xefc60000u;
+            y ^= y >> 18;
+            out_[i] = y;
+        }
+        idx_ = 0;
+    }
+
+   public:
+    explicit BlockMT(uint32_t seed) {
+        s_[0] = seed;
+        for (int i = 1; i < 624; ++i) s_[i] = 1812433253u * (s_[i - 1] ^ (s_[i - 1] >> 30)) + uint32_t(i);
+    }
+    inline uint32_t next() {
+        if (idx_ >= 624) refill();
+        return out_[idx_++];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MatrixMarket text scanner with scanf-like whitespace handling.
+struct Scanner {
+    std::vector<char> buf;
+    size_t pos = 0;
+    bool load(const char *path) {
+        FILE *f = fopen(path, "rb");
+        if (!f) return false;
+        fseek(f, 0, SEEK_END);
+        long n = ftell(f);
+        fseek(f, 0, SEEK_SET);
+        buf.resize(size_t(n) + 1);
+        size_t got = fread(buf.data(), 1, size_t(n), f);
+        fclose(f);
+        buf.resize(got);
+        buf.push_back('\0');
+        return true;
+    }
+    int peek() const { return pos + 1 < buf.size() ? (unsigned char)buf[pos] : EOF; }
+    int get() { return pos + 1 < buf.size() ? (unsigned char)buf[pos++] : EOF; }
+    bool read_int(int &out) {
+        const char *s = buf.data() + pos;
+        char *end = nullptr;
+        long v = strtol(s, &end, 10);
+        if (end == s) return false;
+        pos += size_t(end - s);
+        out = (int)v;
+        return true;
+    }
+    bool read_float(float &out) {
+        const char *s = buf.data() + pos;
+        char *end = nullptr;
+        float v = strtof(s, &end);
+        if (end == s) return false;
+        pos += size_t(end - s);
+        out = v;
+        return true;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void apspie_host_csr_free(apspie_host_csr *h) {
+    if (!h) return;
+    free(h->row_ptr);
+    free(h->col_ind);
+    free(h->val);
+    memset(h, 0, sizeof(*h));
+}
+
+// readEdge (src/util.cuh:355-367) skips leading '%' lines and reads "m n nnz";
+// readMtx (:370-430) reads nnz lines "a b [w]" into (col=a-1, row=b-1), weighted
+// iff the character right after the first line's second integer is a space;
+// makeSymmetric (:572-643) appends reversed non-loop entries, sorts by
+// (row, col) and drops exact repeats (self-loops stay); buildMatrix
+// (src/matrix.cpp:55-111) counting-sorts by row, stable.
+int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
+    if (!path || !out) return APSPIE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    return guarded(nullptr, [&] {
+        Scanner sc;
+        APSPIE_CHECK(sc.load(path), APSPIE_EIO, std::string("cannot open ") + path);
+        int c = sc.get(), old = 0;
+        while (c != EOF) {
+            if ((old == '\n' || old == 0) && c != '%') {
+                sc.pos--;
+                break;
+            }
+            old = c;
+            c = sc.get();
+        }
+        int m = 0, n = 0, nnz = 0;
+        APSPIE_CHECK(sc.read_int(m) && sc.read_int(n) && sc.read_int(nnz), APSPIE_EIO, "bad mtx header");
+        APSPIE_CHECK(m >= 0 && n >= 0 && nnz >= 0, APSPIE_EIO, "negative mtx size");
+        const bool undirected = flags & APSPIE_LOAD_UNDIRECTED;
+        const int64_t cap = undirected ? 2 * int64_t(nnz) : nnz;
+        std::vector<int32_t> row(cap), col(cap);
+        std::vector<float> val(cap);
+        int ch = 0;
+        bool weighted = false;
+        for (int j = 0; j < nnz; ++j) {
+            int a = 0, b = 0;
+            APSPIE_CHECK(sc.read_int(a) && sc.read_int(b), APSPIE_EIO, "not enough rows in mtx file");
+            if (j == 0) {
+                ch = sc.get();
+                weighted = ch == ' ';
+            }
+            float w = 1.0f;
+            if (weighted) APSPIE_CHECK(sc.read_float(w), APSPIE_EIO, "bad weight");
+            col[j] = a - 1;
+            row[j] = b - 1;
+            val[j] = w;
+        }
+        int64_t e = nnz;
+        if (undirected) {
+            int64_t k = nnz;
+            for (int64_t i = 0; i < nnz; ++i)
+                if (col[i] != row[i]) {
+                    row[k] = col[i];
+                    col[k] = row[i];
+                    val[k] = val[i];
+                    ++k;
+                }
+            e = k;
+            std::vector<int64_t> idx(e);
+            for (int64_t i = 0; i < e; ++i) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+                return row[x] != row[y] ? row[x] < row[y] : col[x] < col[y];
+            });
+            std::vector<int32_t> r2, c2;
+            std::vector<float> v2;
+            r2.reserve(e);
+            c2.reserve(e);
+            v2.reserve(e);
+            for (int64_t i = 0; i < e; ++i) {
+                const int64_t t = idx[i];
+                if (!r2.empty() && r2.back() == row[t] && c2.back() == col[t]) continue;
+                r2.push_back(row[t]);
+                c2.push_back(col[t]);
+                v2.push_back(val[t]);
+            }
+            e = (int64_t)r2.size();
+            row.swap(r2);
+            col.swap(c2);
+            val.swap(v2);
+        }
+        host_csr_alloc(out, m, n, e, weighted);
+        std::vector<int32_t> cur(size_t(m) + 1, 0);
+        for (int64_t i = 0; i < e; ++i) {
+            if (row[i] < 0 || row[i] >= m || col[i] < 0 || col[i] >= n) {
+                apspie_host_csr_free(out);
+                throw Error(APSPIE_EIO, "mtx index out of range");
+            }
+            cur[row[i] + 1]++;
+        }
+        for (int i = 0; i < m; ++i) cur[i + 1] += cur[i];
+        memcpy(out->row_ptr, cur.data(), (size_t(m) + 1) * 4);
+        for (int64_t i = 0; i < e; ++i) {
+            const int32_t d = cur[row[i]]++;
+            out->col_ind[d] = col[i];
+            if (weighted) out->val[d] = val[i];
+        }
+        if (flags & APSPIE_LOAD_SORT_COLUMNS) sort_rows(out);
+    });
+}
+
+int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint32_t weight_seed,
+                apspie_host_csr *out) {
+    if (!out) return APSPIE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    return guarded(nullptr, [&] {
+        APSPIE_CHECK(scale >= 1 && scale <= 30 && edge_factor >= 1, APSPIE_EINVAL, "bad rmat shape");
+        const int64_t m = int64_t(1) << scale;
+        const int64_t ne = m * edge_factor;
+        const bool und = flags & APSPIE_RMAT_UNDIRECTED;
+        APSPIE_CHECK((und ? 2 : 1) * ne < (int64_t(1) << 31) - 4096, APSPIE_ELIMIT, "rmat too large for int32 offsets");
+        // u = r / 2^32 < p  <=>  r < ceil(p * 2^32): exact integer thresholds
+        const uint64_t ta = (uint64_t)std::ceil(0.57 * 4294967296.0);
+        const uint64_t tb = (uint64_t)std::ceil(0.76 * 4294967296.0);
+        const uint64_t tc = (uint64_t)std::ceil(0.95 * 4294967296.0);
+        std::vector<uint32_t> es(ne), ed(ne);
+        {
+            BlockMT g(seed);
+            for (int64_t e = 0; e < ne; ++e) {
+                uint32_t s = 0, d = 0;
+                for (int l = scale - 1; l >= 0; --l) {
+                    // quadrant q: 0 = a (0,0), 1 = b (0,1), 2 = c (1,0), 3 = d (1,1)
+                    const uint64_t r = g.next();
+                    const uint32_t q = uint32_t(r >= ta) + uint32_t(r >= tb) + uint32_t(r >= tc);
+                    s |= (q >> 1) << l;
+                    d |= (q & 1u) << l;
+                }
+                es[e] = s;
+                ed[e] = d;
+            }
+        }
+        // bucket by source row (both directions when undirected); positions
+        // inside a row are claimed atomically and the rows sorted afterwards,
+        // so the result does not depend on thread timing
+        std::vector<int64_t> cnt(m + 1, 0);
+        parallel_ranges(ne, [&](int64_t lo, int64_t hi) {
+            for (int64_t e = lo; e < hi; ++e) {
+                if (es[e] == ed[e]) continue;
+                __atomic_fetch_add(&cnt[es[e] + 1], 1, __ATOMIC_RELAXED);
+                if (und) __atomic_fetch_add(&cnt[ed[e] + 1], 1, __ATOMIC_RELAXED);
+            }
+        });
+        for (int64_t i = 0; i < m; ++i) cnt[i + 1] += cnt[i];
+        std::vector<int32_t> bucket(std::max<int64_t>(cnt[m], 1));
+        {
+            std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+            parallel_ranges(ne, [&](int64_t lo, int64_t hi) {
+                for (int64_t e = lo; e < hi; ++e) {
+                    const uint32_t s = es[e], d = ed[e];
+                    if (s == d) continue;
+                    bucket[__atomic_fetch_add(&cur[s], 1, __ATOMIC_RELAXED)] = (int32_t)d;
+                    if (und) bucket[__atomic_fetch_add(&cur[d], 1, __ATOMIC_RELAXED)] = (int32_t)s;
+                }
+            });
+        }
+        std::vector<uint32_t>().swap(es);
+        std::vector<uint32_t>().swap(ed);
+        // sort + unique each row
+        std::vector<int32_t> len(m);
+        parallel_ranges(m, [&](int64_t lo, int64_t hi) {
+            for (int64_t r = lo; r < hi; ++r) {
+                int32_t *b = bucket.data() + cnt[r], *e = bucket.data() + cnt[r + 1];
+                std::sort(b, e);
+                len[r] = int32_t(std::unique(b, e) - b);
+            }
+        });
+        int64_t nnz = 0;
+        for (int64_t r = 0; r < m; ++r) nnz += len[r];
+        const bool weighted = flags & APSPIE_RMAT_WEIGHTED;
+        host_csr_alloc(out, (int32_t)m, (int32_t)m, nnz, weighted);
+        out->row_ptr[0] = 0;
+        for (int64_t r = 0; r < m; ++r) out->row_ptr[r + 1] = out->row_ptr[r] + len[r];
+        parallel_ranges(m, [&](int64_t lo, int64_t hi) {
+            for (int64_t r = lo; r < hi; ++r)
+                memcpy(out->col_ind + out->row_ptr[r], bucket.data() + cnt[r], size_t(len[r]) * 4);
+        });
+        if (weighted) {
+            BlockMT g(weight_seed);
+            for (int64_t p = 0; p < nnz; ++p) out->val[p] = float(1u + (g.next() >> 26));
+        }
+    });
+}
+
+}  // extern "C"

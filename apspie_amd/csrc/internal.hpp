@@ -1,0 +1,118 @@
+// internal.hpp -- context, error plumbing and scratch pool shared by the
+// library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/apspie.h"
+
+namespace apspie {
+
+// Error carried from deep inside an entry point to its C boundary.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &msg) : std::runtime_error(msg), code(c) {}
+};
+
+#define APSPIE_HIP(call)                                                                  \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::apspie::Error(APSPIE_EHIP, std::string(#call) + ": " +                \
+                                                   hipGetErrorString(e_) + " (" __FILE__ \
+                                                   ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define APSPIE_CHECK(cond, code, msg)                         \
+    do {                                                      \
+        if (!(cond)) throw ::apspie::Error((code), (msg));    \
+    } while (0)
+
+// Named slots of the grow-only scratch pool.  A slot keeps the largest size
+// asked of it; memory is released only when the context is destroyed.
+enum Slot : int {
+    S_FLAGS = 0,   // uint8 per vertex: "touched this round"
+    S_VISITED,     // uint32 bitmap: visited vertices (BFS)
+    S_QUEUE,       // int32 frontier vertex ids
+    S_QSTART,      // int32 row start of each frontier vertex
+    S_QOFF,        // int32 exclusive edge offset of each frontier vertex
+    S_QVAL,        // float per frontier vertex (SSSP distance snapshot)
+    S_TILES,       // int32 first frontier index of each edge tile
+    S_STATUS,      // uint64 look-back status, two parities
+    S_CTL,         // small control block
+    S_AUX0,
+    S_AUX1,
+    S_AUX2,
+    S_AUX3,
+    S_AUX4,
+    S_AUX5,
+    S_AUX6,
+    S_AUX7,
+    S_NSLOTS
+};
+
+}  // namespace apspie
+
+struct apspie_ctx {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    void *slot_ptr[apspie::S_NSLOTS] = {};
+    size_t slot_bytes[apspie::S_NSLOTS] = {};
+    void *pinned = nullptr;  // 4 KiB pinned host staging for counters
+    std::mutex mu;
+
+    // Per-kernel-class event timing (off unless enabled).
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_pool;                 // free events
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_open;  // (class, (start, stop))
+    double prof_ms[APSPIE_KCLASS_COUNT] = {};
+    int64_t prof_launches[APSPIE_KCLASS_COUNT] = {};
+
+    // Device buffer of at least `bytes` bytes for slot `s`.  Growing drops
+    // the old contents; callers initialise what they use.
+    void *slot(int s, size_t bytes);
+
+    // Bracket one launch of kernel class `cls` (no-ops unless profiling).
+    void prof_begin(int cls);
+    void prof_end();
+    // Fold finished brackets into the totals (stream must be idle).
+    void prof_collect();
+};
+
+namespace apspie {
+
+// Run `f` and turn exceptions into status codes at the C boundary.
+template <class F>
+int guarded(apspie_ctx *ctx, F &&f) {
+    try {
+        if (ctx) APSPIE_HIP(hipSetDevice(ctx->device));
+        f();
+        return APSPIE_OK;
+    } catch (const Error &e) {
+        if (ctx) ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->err = "host allocation failed";
+        return APSPIE_ENOMEM;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        return APSPIE_EINVAL;
+    }
+}
+
+void validate_csr(const apspie_csr *g, bool need_val);
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace apspie

@@ -1,0 +1,259 @@
+// runtime.cpp -- context lifetime, scratch pool and memory helpers.
+#include <cstring>
+#include <new>
+
+#include "internal.hpp"
+
+using namespace apspie;
+
+void *apspie_ctx::slot(int s, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (slot_bytes[s] >= bytes) return slot_ptr[s];
+    if (slot_ptr[s]) {
+        APSPIE_HIP(hipStreamSynchronize(stream));
+        APSPIE_HIP(hipFree(slot_ptr[s]));
+        slot_ptr[s] = nullptr;
+        slot_bytes[s] = 0;
+    }
+    // round up so a growing graph does not reallocate every call
+    size_t want = (bytes + (1u << 20) - 1) & ~size_t((1u << 20) - 1);
+    void *p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(APSPIE_ENOMEM, "scratch allocation of " + std::to_string(want) + " bytes failed");
+    }
+    slot_ptr[s] = p;
+    slot_bytes[s] = want;
+    return p;
+}
+
+void apspie_ctx::prof_begin(int cls) {
+    if (!profiling) return;
+    auto take = [&] {
+        hipEvent_t e;
+        if (!prof_pool.empty()) {
+            e = prof_pool.back();
+            prof_pool.pop_back();
+        } else {
+            APSPIE_HIP(hipEventCreate(&e));
+        }
+        return e;
+    };
+    hipEvent_t a = take(), b = take();
+    APSPIE_HIP(hipEventRecord(a, stream));
+    prof_open.push_back({cls, {a, b}});
+}
+
+void apspie_ctx::prof_end() {
+    if (!profiling || prof_open.empty()) return;
+    APSPIE_HIP(hipEventRecord(prof_open.back().second.second, stream));
+}
+
+void apspie_ctx::prof_collect() {
+    for (auto &o : prof_open) {
+        float ms = 0.f;
+        APSPIE_HIP(hipEventSynchronize(o.second.second));
+        APSPIE_HIP(hipEventElapsedTime(&ms, o.second.first, o.second.second));
+        prof_ms[o.first] += ms;
+        prof_launches[o.first] += 1;
+        prof_pool.push_back(o.second.first);
+        prof_pool.push_back(o.second.second);
+    }
+    prof_open.clear();
+}
+
+void apspie::validate_csr(const apspie_csr *g, bool need_val) {
+    APSPIE_CHECK(g != nullptr, APSPIE_EINVAL, "null csr");
+    APSPIE_CHECK(g->m >= 0 && g->n >= 0 && g->nnz >= 0, APSPIE_EINVAL, "negative csr size");
+    APSPIE_CHECK(g->nnz < (int64_t(1) << 31) - 4096, APSPIE_ELIMIT, "nnz must stay below 2^31");
+    APSPIE_CHECK(g->m == 0 || g->row_ptr != nullptr, APSPIE_EINVAL, "null row_ptr");
+    APSPIE_CHECK(g->nnz == 0 || g->col_ind != nullptr, APSPIE_EINVAL, "null col_ind");
+    APSPIE_CHECK((reinterpret_cast<uintptr_t>(g->row_ptr) & 15) == 0, APSPIE_EINVAL,
+                 "row_ptr must be 16-byte aligned");
+    APSPIE_CHECK(!need_val || g->nnz == 0 || g->val != nullptr, APSPIE_EINVAL, "weights required");
+}
+
+extern "C" {
+
+int apspie_ctx_create(int device, apspie_ctx **out) {
+    if (!out) return APSPIE_EINVAL;
+    *out = nullptr;
+    apspie_ctx *ctx = new (std::nothrow) apspie_ctx();
+    if (!ctx) return APSPIE_ENOMEM;
+    ctx->device = device;
+    int rc = guarded(ctx, [&] {
+        int n = 0;
+        APSPIE_HIP(hipGetDeviceCount(&n));
+        APSPIE_CHECK(device >= 0 && device < n, APSPIE_EINVAL, "no such device");
+        APSPIE_HIP(hipSetDevice(device));
+        APSPIE_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        APSPIE_HIP(hipEventCreate(&ctx->ev0));
+        APSPIE_HIP(hipEventCreate(&ctx->ev1));
+        APSPIE_HIP(hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device));
+        APSPIE_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+    });
+    if (rc != APSPIE_OK) {
+        fprintf(stderr, "apspie_ctx_create: %s\n", ctx->err.c_str());
+        apspie_ctx_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return APSPIE_OK;
+}
+
+int apspie_ctx_destroy(apspie_ctx *ctx) {
+    if (!ctx) return APSPIE_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int s = 0; s < S_NSLOTS; ++s)
+        if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto &o : ctx->prof_open) {
+        (void)hipEventDestroy(o.second.first);
+        (void)hipEventDestroy(o.second.second);
+    }
+    for (auto e : ctx->prof_pool) (void)hipEventDestroy(e);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return APSPIE_OK;
+}
+
+const char *apspie_last_error(const apspie_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void *apspie_ctx_stream(apspie_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int apspie_ctx_synchronize(apspie_ctx *ctx) {
+    return guarded(ctx, [&] { APSPIE_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int apspie_ctx_cu_count(const apspie_ctx *ctx) { return ctx ? ctx->cu_count : 0; }
+
+int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->profiling = enable != 0;
+    return APSPIE_OK;
+}
+
+int apspie_ctx_profile_read(apspie_ctx *ctx, int kclass, double *ms, int64_t *launches) {
+    if (!ctx || kclass < 0 || kclass >= APSPIE_KCLASS_COUNT) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        ctx->prof_collect();
+        if (ms) *ms = ctx->prof_ms[kclass];
+        if (launches) *launches = ctx->prof_launches[kclass];
+    });
+}
+
+int apspie_ctx_profile_reset(apspie_ctx *ctx) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        ctx->prof_collect();
+        for (int i = 0; i < APSPIE_KCLASS_COUNT; ++i) {
+            ctx->prof_ms[i] = 0;
+            ctx->prof_launches[i] = 0;
+        }
+    });
+}
+
+int apspie_malloc(apspie_ctx *ctx, void **ptr, size_t bytes) {
+    if (!ctx || !ptr) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        *ptr = nullptr;
+        if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(APSPIE_ENOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    });
+}
+
+int apspie_free(apspie_ctx *ctx, void *ptr) {
+    if (!ctx) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        if (ptr) {
+            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            APSPIE_HIP(hipFree(ptr));
+        }
+    });
+}
+
+int apspie_memcpy_h2d(apspie_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        if (!bytes) return;
+        APSPIE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int apspie_memcpy_d2h(apspie_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        if (!bytes) return;
+        APSPIE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owned *out) {
+    if (!ctx || !h || !out) return APSPIE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    return guarded(ctx, [&] {
+        APSPIE_CHECK(h->m >= 0 && h->nnz >= 0, APSPIE_EINVAL, "negative size");
+        APSPIE_CHECK(h->nnz < (int64_t(1) << 31) - 4096, APSPIE_ELIMIT, "nnz must stay below 2^31");
+        out->m = h->m;
+        out->n = h->n;
+        out->nnz = h->nnz;
+        auto alloc = [&](void **p, size_t b) {
+            if (hipMalloc(p, b ? b : 4) != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error(APSPIE_ENOMEM, "csr upload allocation failed");
+            }
+        };
+        try {
+            alloc((void **)&out->row_ptr, (size_t(h->m) + 1) * 4);
+            alloc((void **)&out->col_ind, size_t(h->nnz) * 4);
+            if (h->val) alloc((void **)&out->val, size_t(h->nnz) * 4);
+            APSPIE_HIP(hipMemcpyAsync(out->row_ptr, h->row_ptr, (size_t(h->m) + 1) * 4,
+                                      hipMemcpyHostToDevice, ctx->stream));
+            if (h->nnz)
+                APSPIE_HIP(hipMemcpyAsync(out->col_ind, h->col_ind, size_t(h->nnz) * 4,
+                                          hipMemcpyHostToDevice, ctx->stream));
+            if (h->val && h->nnz)
+                APSPIE_HIP(hipMemcpyAsync(out->val, h->val, size_t(h->nnz) * 4, hipMemcpyHostToDevice,
+                                          ctx->stream));
+            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            apspie_csr_free(ctx, out);
+            throw;
+        }
+    });
+}
+
+int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c) {
+    if (!ctx || !c) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        if (c->row_ptr) APSPIE_HIP(hipFree(c->row_ptr));
+        if (c->col_ind) APSPIE_HIP(hipFree(c->col_ind));
+        if (c->val) APSPIE_HIP(hipFree(c->val));
+        memset(c, 0, sizeof(*c));
+    });
+}
+
+apspie_csr apspie_csr_view(const apspie_csr_owned *c) {
+    apspie_csr v{};
+    if (!c) return v;
+    v.m = c->m;
+    v.n = c->n;
+    v.nnz = c->nnz;
+    v.row_ptr = c->row_ptr;
+    v.col_ind = c->col_ind;
+    v.val = c->val;
+    return v;
+}
+
+}  // extern "C"

@@ -1,0 +1,57 @@
+// traverse.hpp -- shapes and device-resident control state of the frontier
+// traversal kernels (traverse.hip).
+#pragma once
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace apspie {
+namespace trav {
+
+constexpr int WG = 256;           // threads per workgroup (4 waves of 64)
+constexpr int VPT = 8;            // vertices per thread in the compaction pass
+constexpr int CHUNK = WG * VPT;   // vertices per compaction workgroup
+constexpr int TILE = 2048;        // frontier edges per expansion tile
+constexpr int IPT = TILE / WG;    // edges per thread per tile
+constexpr uint32_t INF_BITS = 0x7effffeeu;  // bit pattern of 1.70141e38f
+
+enum { MODE_BFS = 0, MODE_SSSP = 1, MODE_STEP = 2 };
+
+// Frontier descriptor written by a compaction, read by the next expansion.
+struct Ctl {
+    int32_t nF;      // queued frontier vertices (out-degree > 0)
+    int32_t E;       // edges of the queued frontier
+    int32_t nTiles;  // ceil(E / TILE)
+    int32_t pad;
+};
+
+// Totals over one call.
+struct Acc {
+    unsigned long long edges;     // sum of frontier edge counts
+    unsigned long long vertices;  // vertices that entered a frontier
+    int32_t levels;               // rounds with a non-empty frontier
+    int32_t pad;
+};
+
+struct Control {
+    Ctl ctl[2];                 // by round parity
+    unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
+    Acc acc;
+};
+
+struct Bufs {
+    int nblk = 0;
+    unsigned expand_grid = 0;
+    uint8_t *flags = nullptr;
+    uint32_t *visited = nullptr;
+    int32_t *queue = nullptr, *qstart = nullptr, *qoff = nullptr, *tiles = nullptr;
+    float *qval = nullptr;
+    uint64_t *status = nullptr;
+    Control *C = nullptr;
+};
+
+Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz);
+
+}  // namespace trav
+}  // namespace apspie

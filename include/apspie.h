@@ -1,0 +1,172 @@
+/*
+ * apspie.h -- C ABI of the MI355X (gfx950) SpMSpV traversal and hash-SpGEMM
+ * library.
+ *
+ * Every function returns 0 on success and a negative APSPIE_E* code on
+ * failure; apspie_last_error() holds the message.  No function calls exit().
+ * Device pointers are plain HIP device addresses (hipMalloc / apspie_malloc /
+ * torch tensors' data_ptr()).  Work runs on the context's stream; the plain
+ * (blocking) entry points synchronise that stream before returning.
+ *
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the apspie checkout).  The reference has no FFI of its own: its hot path
+ * is header-only CUDA C++ templates called from test/<algo>/main.cu.
+ */
+#ifndef APSPIE_H
+#define APSPIE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+#define APSPIE_OK 0
+#define APSPIE_EINVAL (-1)   /* bad argument (null pointer, id out of range) */
+#define APSPIE_ENOMEM (-2)   /* device or host allocation failed */
+#define APSPIE_EHIP (-3)     /* HIP runtime error */
+#define APSPIE_EIO (-4)      /* file / parse error */
+#define APSPIE_ELIMIT (-5)   /* size beyond a documented limit */
+#define APSPIE_ENOCONV (-6)  /* iteration cap reached without convergence */
+#define APSPIE_EBUSY (-7)    /* serving: admission refused / queue full */
+
+/* Distance given to vertices SSSP does not reach: the reference's sentinel
+ * 1.70141e38 (src/spmspvSssp.cuh:26, :214). */
+#define APSPIE_SSSP_INF 1.70141e38f
+
+typedef struct apspie_ctx apspie_ctx;
+
+/* Device CSR (borrowed device pointers).  Replaces the raw
+ * (m, edge, d_csrRowPtr, d_csrColInd, d_csrVal) argument groups of
+ * src/spmspvBfs.cuh:96, src/sssp.cuh:28, src/spmspvMis.cuh:150 and the
+ * d_matrix fields of src/matrix.hpp:6-37.  col_ind values are < n;
+ * nnz < 2^31.  val may be NULL (all ones). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    const int32_t *row_ptr;
+    const int32_t *col_ind;
+    const float *val;
+} apspie_csr;
+
+/* Host CSR allocated by the library (apspie_host_csr_free releases it). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    int32_t *row_ptr;
+    int32_t *col_ind;
+    float *val; /* NULL when the input is unweighted and no weights were asked for */
+} apspie_host_csr;
+
+/* Device CSR allocated by the library (apspie_csr_free releases it). */
+typedef struct {
+    int32_t m, n;
+    int64_t nnz;
+    int32_t *row_ptr;
+    int32_t *col_ind;
+    float *val;
+} apspie_csr_owned;
+
+/* Per-call statistics.  For BFS: edges = expanded edges (the reference's
+ * "traversed edges", src/spmspvBfs.cuh:217), vertices = reached vertices.
+ * For SSSP: edges = relaxed edges over all rounds, vertices = sum of frontier
+ * sizes, improved = (vertex, round) pairs whose distance dropped.  bytes_alg
+ * is the algorithmic HBM traffic of the traversal, kernel_ms the device time
+ * between events recorded around it on the context stream. */
+typedef struct {
+    int64_t edges;
+    int64_t vertices;
+    int64_t improved;
+    int32_t iterations;
+    int32_t reserved;
+    double bytes_alg;
+    double kernel_ms;
+} apspie_stats;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces mgpu::CudaContext (passed to every reference entry point) and the
+ * per-call d_scratch / allocScratch of src/scratch.hpp:5-31,
+ * src/util.cuh:646-672: the context owns a stream and a grow-only scratch
+ * pool.  One context per host thread. */
+int apspie_ctx_create(int device, apspie_ctx **out);
+int apspie_ctx_destroy(apspie_ctx *ctx);
+const char *apspie_last_error(const apspie_ctx *ctx);
+/* The context's stream as a hipStream_t cast to void*. */
+void *apspie_ctx_stream(apspie_ctx *ctx);
+int apspie_ctx_synchronize(apspie_ctx *ctx);
+/* Compute-unit count of the context's device. */
+int apspie_ctx_cu_count(const apspie_ctx *ctx);
+
+/* Per-kernel timing, the counterpart of the reference's GpuTimer event pairs
+ * (src/util.cuh:318-352) and per-stage printf timings.  While enabled, entry
+ * points bracket every launch with HIP events on the context stream and
+ * accumulate device time per kernel class. */
+#define APSPIE_KCLASS_EXPAND 0   /* frontier edge expansion */
+#define APSPIE_KCLASS_COMPACT 1  /* frontier compaction + scan */
+#define APSPIE_KCLASS_OTHER 2    /* initialisation and everything else */
+#define APSPIE_KCLASS_COUNT 8
+int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable);
+/* Totals since the last reset: device milliseconds and launch count. */
+int apspie_ctx_profile_read(apspie_ctx *ctx, int kclass, double *ms, int64_t *launches);
+int apspie_ctx_profile_reset(apspie_ctx *ctx);
+
+/* Device memory helpers for callers without a framework allocator
+ * (the reference's cudaMalloc / cudaMemcpy calls in test/<algo>/main.cu). */
+int apspie_malloc(apspie_ctx *ctx, void **ptr, size_t bytes);
+int apspie_free(apspie_ctx *ctx, void *ptr);
+int apspie_memcpy_h2d(apspie_ctx *ctx, void *dst, const void *src, size_t bytes);
+int apspie_memcpy_d2h(apspie_ctx *ctx, void *dst, const void *src, size_t bytes);
+
+/* ---- graph loading -------------------------------------------------------- */
+#define APSPIE_LOAD_UNDIRECTED 1u   /* makeSymmetric (src/util.cuh:572-643) */
+#define APSPIE_LOAD_SORT_COLUMNS 2u /* sort each row's columns (not in the reference) */
+
+/* MatrixMarket reader: readEdge + readMtx + [makeSymmetric] + buildMatrix,
+ * src/util.cuh:355-430, :522-643, src/matrix.cpp:55-111.  Line "a b [w]"
+ * puts column a-1 into row b-1, as readMtx does. */
+int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out);
+void apspie_host_csr_free(apspie_host_csr *h);
+
+#define APSPIE_RMAT_UNDIRECTED 1u
+#define APSPIE_RMAT_WEIGHTED 2u
+/* Seeded RMAT(scale, edge_factor, a=.57, b=.19, c=.19, d=.05) with one
+ * MT19937 stream (seed), self-loops and duplicates removed, sorted columns;
+ * weights 1 + (genrand_int32() >> 26) from a second stream (weight_seed).
+ * Replaces the reference's downloaded datasets (dataset/large/). */
+int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint32_t weight_seed,
+                apspie_host_csr *out);
+
+/* Host CSR -> device (cudaMemcpy block of test/bfs/main.cu:104-120). */
+int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owned *out);
+int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c);
+/* View an owned CSR as a borrowed one. */
+apspie_csr apspie_csr_view(const apspie_csr_owned *c);
+
+/* ---- plain path ----------------------------------------------------------- */
+
+/* Level-synchronous BFS over Boolean OR-AND SpMSpV from `src`; d_depth[m]
+ * receives the level of each vertex, -1 if unreached.  Replaces
+ * spmspvBfs (src/spmspvBfs.cuh:96) and bfs (src/bfs.cuh:33), which take the
+ * level count from the CPU BFS; this runs until the frontier is empty. */
+int apspie_bfs(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *d_depth,
+               apspie_stats *stats);
+
+/* SSSP by synchronous min-plus Bellman-Ford over a changed-vertex frontier,
+ * to convergence; d_dist[m] receives distances, APSPIE_SSSP_INF when
+ * unreachable.  Weights must be finite and >= 0.  Replaces sssp
+ * (src/sssp.cuh:28). */
+int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist,
+                apspie_stats *stats);
+
+/* One min-plus SpMSpV step: frontier = {u : -1 < x[u] < 1e38},
+ * y[v] = min_u x[u] + A(u,v), APSPIE_SSSP_INF where nothing reaches.
+ * Replaces spmspvSssp (src/spmspvSssp.cuh:92). */
+int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, float *d_y,
+                     apspie_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

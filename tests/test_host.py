@@ -1,0 +1,80 @@
+"""Host-side product code against the oracle (no GPU compute): the
+MatrixMarket loader, the RMAT generator, and the C ABI's exported symbols."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import apspie_amd as A
+import oracle as O
+from conftest import MTX, ROOT
+
+NAMES = ["test_cc", "test_bc", "test_pr", "test_mesh", "test_mis", "test_spgemm", "chesapeake"]
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.lib()
+    names = A.exported_symbols(os.path.join(ROOT, "include", "apspie.h"))
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("und", [False, True])
+def test_loader_matches_oracle(name, und):
+    p = os.path.join(MTX, name + ".mtx")
+    got = A.load_mtx(p, undirected=und)
+    want = O.read_mtx(p, und)
+    assert (got.m, got.n) == (want.m, want.n)
+    assert np.array_equal(got.row_ptr, want.row_ptr)
+    assert np.array_equal(got.col_ind, want.col_ind)
+    assert got.val is None or np.array_equal(got.val, want.val)
+
+
+def test_loader_weighted_and_sorted(tmp_path):
+    f = tmp_path / "w.mtx"
+    f.write_text("%%MatrixMarket matrix coordinate real general\n% c\n4 4 5\n2 1 3.5\n1 2 2\n4 1 7\n3 1 1\n1 1 9\n")
+    g = A.load_mtx(str(f))
+    o = O.read_mtx(str(f), False)
+    assert np.array_equal(g.row_ptr, o.row_ptr) and np.array_equal(g.col_ind, o.col_ind)
+    assert np.array_equal(g.val, o.val)
+    s = A.load_mtx(str(f), sort_columns=True)
+    for r in range(s.m):
+        row = s.col_ind[s.row_ptr[r]:s.row_ptr[r + 1]]
+        assert np.all(np.diff(row) >= 0)
+        # values travel with their columns
+        for c, v in zip(row, s.val[s.row_ptr[r]:s.row_ptr[r + 1]]):
+            k = list(g.col_ind[g.row_ptr[r]:g.row_ptr[r + 1]]).index(c)
+            assert g.val[g.row_ptr[r] + k] == v
+
+
+def test_loader_errors(tmp_path):
+    with pytest.raises(A.ApspieError):
+        A.load_mtx(str(tmp_path / "missing.mtx"))
+    f = tmp_path / "short.mtx"
+    f.write_text("3 3 4\n1 2\n2 3\n")
+    with pytest.raises(A.ApspieError):
+        A.load_mtx(str(f))
+    f2 = tmp_path / "range.mtx"
+    f2.write_text("3 3 1\n5 1\n")
+    with pytest.raises(A.ApspieError):
+        A.load_mtx(str(f2))
+
+
+@pytest.mark.parametrize("scale,und", [(8, False), (8, True), (11, False), (12, True)])
+def test_rmat_matches_oracle(scale, und):
+    seed = 1000 + scale
+    got = A.rmat(scale, 16, seed, undirected=und, weight_seed=seed + 7)
+    want = O.rmat(scale, 16, seed, und, weight_seed=seed + 7)
+    assert np.array_equal(got.row_ptr, want.row_ptr)
+    assert np.array_equal(got.col_ind, want.col_ind)
+    assert np.array_equal(got.val, want.val)
+
+
+def test_rmat_limits():
+    with pytest.raises(A.ApspieError):
+        A.rmat(0, 16, 1)
+    with pytest.raises(A.ApspieError):
+        A.rmat(28, 16, 1)  # 2^32 edges: beyond int32 offsets
