@@ -89,6 +89,7 @@ def lib() -> C.CDLL:
         "apspie_ctx_set_profiling": [vp, C.c_int],
         "apspie_ctx_profile_read": [vp, C.c_int, P(C.c_double), P(i64)],
         "apspie_ctx_profile_reset": [vp],
+        "apspie_hbm_copy_bandwidth": [vp, C.c_size_t, C.c_int, P(C.c_double)],
         "apspie_malloc": [vp, P(vp), C.c_size_t],
         "apspie_free": [vp, vp],
         "apspie_memcpy_h2d": [vp, vp, vp, C.c_size_t],
@@ -101,6 +102,8 @@ def lib() -> C.CDLL:
         "apspie_bfs": [vp, P(_Csr), i32, vp, P(Stats)],
         "apspie_sssp": [vp, P(_Csr), i32, vp, P(Stats)],
         "apspie_sssp_step": [vp, P(_Csr), vp, vp, P(Stats)],
+        "apspie_mis": [vp, P(_Csr), vp, vp, P(Stats)],
+        "apspie_spmspv_or": [vp, P(_Csr), vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -293,6 +296,12 @@ class Context:
             out[name] = (ms.value, n.value)
         return out
 
+    def hbm_copy_bandwidth(self, nbytes: int = 1 << 30, iters: int = 5) -> float:
+        """Best device-copy rate in GB/s (read + write bytes)."""
+        g = C.c_double()
+        _check(lib().apspie_hbm_copy_bandwidth(self.h, nbytes, iters, C.byref(g)), self.h, "copy")
+        return g.value
+
     def upload(self, h: HostCsr) -> DeviceCsr:
         return DeviceCsr(self, h)
 
@@ -336,6 +345,27 @@ class Context:
         dx.free()
         dy.free()
         return y, st
+
+    def mis(self, g: DeviceCsr, prio: np.ndarray):
+        """Greedy-by-(prio, -id) MIS of a symmetric graph; returns (int32[m] 0/1, Stats)."""
+        dp = self.alloc(4 * max(g.m, 1)).upload(np.asarray(prio, dtype=np.uint32))
+        dm = self.alloc(4 * max(g.m, 1))
+        st = Stats()
+        _check(lib().apspie_mis(self.h, C.byref(g.c), dp.ptr, dm.ptr, C.byref(st)), self.h, "mis")
+        out = dm.download(np.int32, g.m)
+        dp.free()
+        dm.free()
+        return out, st
+
+    def spmspv_or(self, g: DeviceCsr, x: np.ndarray) -> np.ndarray:
+        """Boolean SpMSpV: out[v] = 1 iff some u with x[u] != 0 has v in its row."""
+        dx = self.alloc(4 * max(g.m, 1)).upload(np.asarray(x, dtype=np.int32))
+        dy = self.alloc(4 * max(g.m, 1))
+        _check(lib().apspie_spmspv_or(self.h, C.byref(g.c), dx.ptr, dy.ptr), self.h, "spmspv_or")
+        y = dy.download(np.int32, g.m)
+        dx.free()
+        dy.free()
+        return y
 
 
 def exported_symbols(header: str) -> Sequence[str]:

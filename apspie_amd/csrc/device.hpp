@@ -51,10 +51,14 @@ __device__ __forceinline__ uint32_t wave_sum32(uint32_t x) {
 
 // Decoupled look-back over per-workgroup status words (Merrill & Garland,
 // "Single-pass Parallel Prefix Scan with Decoupled Look-back", 2016), done by
-// one full wave: 64 predecessors are inspected per step with a ballot.
+// one full wave: each step inspects LB_DEPTH * 64 predecessors (lane l, slot
+// j looks at bid - 1 - (j * 64 + l)) and votes with ballots, so the
+// inclusive prefix front is found in few memory round trips.
 // Status word: bits 62-63 flag (0 none, 1 aggregate, 2 inclusive prefix),
 // bits 0-61 the value.  Values are sums that must stay below 2^62.
 // Called by all 64 lanes of one wave; returns the exclusive prefix of `bid`.
+constexpr int LB_DEPTH = 4;
+
 __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t *status, int bid, uint64_t agg,
                                                        int lane) {
     constexpr uint64_t FLAG_AGG = 1ull << 62, FLAG_INC = 2ull << 62, VMASK = (1ull << 62) - 1;
@@ -66,24 +70,53 @@ __device__ __forceinline__ uint64_t lookback_exclusive(uint64_t *status, int bid
     uint64_t excl = 0;
     int base = bid - 1;
     for (;;) {
-        const int idx = base - lane;
-        uint64_t s;
+        uint64_t s[LB_DEPTH];
         for (;;) {
-            s = idx >= 0 ? ld_acquire_agent(&status[idx]) : FLAG_INC;
-            if (__all((s >> 62) != 0)) break;
+            bool ready = true;
+#pragma unroll
+            for (int j = 0; j < LB_DEPTH; ++j) {
+                const int idx = base - (j * WAVE + lane);
+                s[j] = idx >= 0 ? ld_acquire_agent(&status[idx]) : FLAG_INC;
+                ready = ready && (s[j] >> 62) != 0;
+            }
+            if (__all(ready)) break;
             __builtin_amdgcn_s_sleep(1);
         }
-        const uint64_t inc = __ballot((s >> 62) == 2);
-        if (inc) {
-            const int first = __ffsll((unsigned long long)inc) - 1;
-            excl += wave_sum(lane <= first ? (s & VMASK) : 0);
-            break;
+        // nearest predecessor holding an inclusive prefix: slot j*, lane l*
+        int jstar = -1, lstar = 0;
+#pragma unroll
+        for (int j = LB_DEPTH - 1; j >= 0; --j) {
+            const uint64_t inc = __ballot((s[j] >> 62) == 2);
+            if (inc) {
+                jstar = j;
+                lstar = __ffsll((unsigned long long)inc) - 1;
+            }
         }
-        excl += wave_sum(s & VMASK);
-        base -= WAVE;
+        uint64_t part = 0;
+#pragma unroll
+        for (int j = 0; j < LB_DEPTH; ++j) {
+            const bool take = jstar < 0 || j < jstar || (j == jstar && lane <= lstar);
+            part += take ? (s[j] & VMASK) : 0;
+        }
+        excl += wave_sum(part);
+        if (jstar >= 0) break;
+        base -= LB_DEPTH * WAVE;
     }
     if (lane == 0) st_agent(&status[bid], FLAG_INC | (excl + agg));
     return excl;
+}
+
+// Bit i set iff byte i of the 16-byte word is non-zero.
+__device__ __forceinline__ uint32_t nonzero_bytes16(uint4 w) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        b |= ((w.x >> (8 * i)) & 0xffu) ? (1u << i) : 0u;
+        b |= ((w.y >> (8 * i)) & 0xffu) ? (1u << (i + 4)) : 0u;
+        b |= ((w.z >> (8 * i)) & 0xffu) ? (1u << (i + 8)) : 0u;
+        b |= ((w.w >> (8 * i)) & 0xffu) ? (1u << (i + 12)) : 0u;
+    }
+    return b;
 }
 
 // Bit i set iff byte i of the 8-byte word is non-zero.

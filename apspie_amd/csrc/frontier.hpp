@@ -1,0 +1,261 @@
+// frontier.hpp -- the two generic gfx950 kernels every frontier algorithm is
+// built from (BFS, SSSP, MIS, Boolean SpMSpV):
+//
+//   k_compact<Pick>  one coalesced pass over the vertex range in id order.
+//                    Pick decides which vertices enter the new frontier (and
+//                    may update per-vertex state on the way).  Entries with
+//                    out-degree > 0 are queued with (vertex, row start, edge
+//                    offset, 32-bit payload) through a single-pass decoupled
+//                    look-back scan, and the tile -> first-entry table for the
+//                    expansion is written.
+//   k_expand<Op>     the queued entries' edges, cut into equal tiles of TILE
+//                    edges; each tile finds its entries from the table, an
+//                    inclusive max-scan over entry start marks gives every edge
+//                    slot its owner, and Op runs on every edge in two phases
+//                    (load, then apply) so a thread issues all of its loads
+//                    before consuming any.
+//
+// Control state (frontier sizes, dynamic workgroup counters) lives in device
+// memory; a compaction at round L writes ctl[L & 1], the expansion of round
+// L + 1 reads it.  Compactions that follow an empty frontier exit at once
+// (unless told to always run), so the host can launch rounds speculatively.
+#pragma once
+
+#include "device.hpp"
+#include "traverse.hpp"
+
+namespace apspie {
+namespace trav {
+
+using namespace dev;
+
+struct Queue {
+    int32_t *vert;    // vertex id
+    int32_t *start;   // row_ptr[v]
+    int32_t *off;     // exclusive offset of the entry's first edge
+    uint32_t *val;    // payload (distance bits, priority, ...)
+    int32_t *tiles;   // first entry of each edge tile
+};
+
+// host helpers shared by the frontier algorithms (traverse.hip)
+Queue queue_of(const Bufs &b);
+unsigned init_grid(const Bufs &b, int m);
+void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *stats);
+
+// Compaction.  Wave w of workgroup b owns vertices [b*CHUNK + w*SEGV, +SEGV)
+// and walks them in ROUNDS rounds of 64 consecutive vertices; in-wave ranks
+// come from ballots (mbcnt) and wave scans, the workgroup offset from the
+// look-back.
+//
+// Pick interface:
+//   bool pick(int v, uint32_t &payload)       v may be >= m (padding lanes)
+//   void on_round(uint64_t ball, int v0, int lane)   ball: picked lanes of
+//                                                    vertices v0 .. v0+63
+//   void on_new(int v)                        for every picked vertex
+template <class Pick>
+__global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int nblk, int always,
+                                                 const int32_t *__restrict__ row_ptr, Queue q,
+                                                 uint64_t *status, Control *C) {
+    constexpr int NW = WGC / WAVE, SEGV = CHUNK / NW, ROUNDS = SEGV / WAVE;
+    static_assert(ROUNDS <= 32, "round bits must fit a 32-bit mask");
+    __shared__ int s_bid;
+    __shared__ uint64_t s_wsum[NW];
+    __shared__ uint32_t s_new[NW];
+    __shared__ uint64_t s_prefix;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int par = level & 1;
+    Ctl *next = &C->ctl[par];
+    if (!always && C->ctl[par ^ 1].nF == 0) {
+        if (blockIdx.x == 0 && tid == 0) *next = Ctl{0, 0, 0, 0};
+        return;
+    }
+    if (tid == 0) s_bid = (int)atomicAdd(&C->blk_counter[par], 1u);
+    __syncthreads();
+    const int bid = s_bid;
+    // state for the launch after this one (its previous user has finished)
+    if (tid == 0) status[(par ^ 1) * nblk + bid] = 0;
+    if (bid == 0 && tid == 0) {
+        C->blk_counter[par ^ 1] = 0;
+        C->nnew[par ^ 1] = 0;
+    }
+
+    const int vbase = bid * CHUNK + wid * SEGV;
+    uint32_t qb = 0, lane_cnt = 0, lane_es = 0, lane_new = 0;
+    int32_t rs[ROUNDS], dg[ROUNDS];
+    uint32_t vv[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int v = vbase + r * WAVE + lane;
+        uint32_t payload = 0;
+        const bool isnew = pk.pick(v, payload);
+        const uint64_t ball = __ballot(isnew);
+        rs[r] = 0;
+        dg[r] = 0;
+        vv[r] = payload;
+        if (ball) {
+            pk.on_round(ball, vbase + r * WAVE, lane);
+            const int a = v < m ? row_ptr[v] : 0;
+            int b = __shfl_down(a, 1, WAVE);
+            if (lane == WAVE - 1) b = v + 1 <= m ? row_ptr[v + 1] : 0;
+            if (isnew) {
+                const int d = b - a;
+                rs[r] = a;
+                dg[r] = d;
+                pk.on_new(v);
+                lane_new++;
+                if (d > 0) {
+                    qb |= 1u << r;
+                    lane_cnt++;
+                    lane_es += d;
+                }
+            }
+        }
+    }
+    const uint32_t wcnt = wave_sum32(lane_cnt), wes = wave_sum32(lane_es), wnew = wave_sum32(lane_new);
+    if (lane == 0) {
+        s_wsum[wid] = (uint64_t(wcnt) << 31) | wes;
+        s_new[wid] = wnew;
+    }
+    __syncthreads();
+    uint64_t wpre = 0, total = 0;
+    uint32_t bnew = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wid) wpre += s_wsum[w];
+        total += s_wsum[w];
+        bnew += s_new[w];
+    }
+    if (wid == 0) {
+        const uint64_t p = lookback_exclusive(status + par * nblk, bid, total, lane);
+        if (lane == 0) s_prefix = p;
+    }
+    __syncthreads();
+    const uint64_t base = s_prefix + wpre;
+    int pos = int(base >> 31);
+    int off = int(base & 0x7fffffffu);
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const bool qd = (qb >> r) & 1u;
+        const uint64_t qball = __ballot(qd);
+        if (!qball) continue;
+        const int rank =
+            __builtin_amdgcn_mbcnt_hi(uint32_t(qball >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qball), 0));
+        const uint32_t d = qd ? uint32_t(dg[r]) : 0u;
+        const uint32_t dincl = wave_incl_scan32(d, lane);
+        if (qd) {
+            const int p = pos + rank;
+            const int o = off + int(dincl - d);
+            q.vert[p] = vbase + r * WAVE + lane;
+            q.start[p] = rs[r];
+            q.off[p] = o;
+            q.val[p] = vv[r];
+            // tiles whose first edge falls inside this entry's edge range
+            for (int t = (o + TILE - 1) / TILE; t * TILE < o + int(d); ++t) q.tiles[t] = p;
+        }
+        pos += __popcll(qball);
+        off += int(__shfl(dincl, WAVE - 1, WAVE));
+    }
+    if (tid == 0) {
+        if (bnew) {
+            atomicAdd(&C->nnew[par], bnew);
+            atomicAdd(&C->acc.vertices, (unsigned long long)bnew);
+            atomicMax(&C->acc.levels, level + 1);
+        }
+        if (bid == nblk - 1) {
+            const uint64_t g = s_prefix + total;
+            Ctl c;
+            c.nF = int(g >> 31);
+            c.E = int(g & 0x7fffffffu);
+            c.nTiles = (c.E + TILE - 1) / TILE;
+            c.pad = 0;
+            *next = c;
+            atomicAdd(&C->acc.edges, (unsigned long long)c.E);
+        }
+    }
+}
+
+// Expansion over the entries queued by the previous compaction.
+//
+// Op interface:
+//   typedef Pre;                                   per-edge loaded state
+//   Pre load(int c, int p, uint32_t payload, int v)   issue loads for edge
+//                                                  (v -> c) at col_ind[p]
+//   void apply(int c, int p, uint32_t payload, int v, const Pre &)
+//   static constexpr bool kVert                    needs the entry's vertex id
+// Columns outside [0, m) are skipped.
+template <class Op>
+__global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Control *C, Queue q,
+                                               const int32_t *__restrict__ col_ind) {
+    constexpr int NW = WG / WAVE, SEG = TILE / NW;
+    __shared__ int32_t s_own[TILE];
+    __shared__ int32_t s_base[TILE + 1];  // start - (off - e0): col_ind index of slot 0
+    __shared__ uint32_t s_val[TILE + 1];
+    __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
+    __shared__ int32_t s_wmax[NW];
+    const Ctl cur = C->ctl[(level - 1) & 1];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
+        const int e0 = t * TILE;
+        const int f0 = q.tiles[t];
+        const int f1 = (t + 1 < cur.nTiles) ? q.tiles[t + 1] : cur.nF - 1;
+        const int cnt = min(f1 - f0 + 1, TILE + 1);  // every queued entry has >= 1 edge
+        for (int i = tid; i < TILE; i += WG) s_own[i] = 0;
+        __syncthreads();
+        for (int k = tid; k < cnt; k += WG) {
+            const int o = q.off[f0 + k] - e0;
+            s_base[k] = q.start[f0 + k] - o;
+            s_val[k] = q.val[f0 + k];
+            if (Op::kVert) s_vert[k] = q.vert[f0 + k];
+            if (o > 0 && o < TILE) s_own[o] = k;
+        }
+        __syncthreads();
+        int own[IPT];
+        int mx = 0;
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            own[r] = s_own[wid * SEG + r * WAVE + lane];
+            mx = max(mx, own[r]);
+        }
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
+        if (lane == 0) s_wmax[wid] = mx;
+        __syncthreads();
+        int carry = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w)
+            if (w < wid) carry = max(carry, s_wmax[w]);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            int xo = max(own[r], carry);
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const int y = __shfl_up(xo, d, WAVE);
+                if (lane >= d) xo = max(xo, y);
+            }
+            own[r] = xo;
+            carry = __shfl(xo, WAVE - 1, WAVE);
+        }
+        const int elim = min(TILE, cur.E - e0);
+        int pos[IPT], col[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            const int s = wid * SEG + r * WAVE + lane;
+            pos[r] = s < elim ? s_base[own[r]] + s : -1;
+        }
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
+        typename Op::Pre pre[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r)
+            if ((unsigned)col[r] < (unsigned)m)
+                pre[r] = op.load(col[r], pos[r], s_val[own[r]], Op::kVert ? s_vert[own[r]] : 0);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r)
+            if ((unsigned)col[r] < (unsigned)m)
+                op.apply(col[r], pos[r], s_val[own[r]], Op::kVert ? s_vert[own[r]] : 0, pre[r]);
+        __syncthreads();
+    }
+}
+
+}  // namespace trav
+}  // namespace apspie

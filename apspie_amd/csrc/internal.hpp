@@ -48,6 +48,7 @@ enum Slot : int {
     S_TILES,       // int32 first frontier index of each edge tile
     S_STATUS,      // uint64 look-back status, two parities
     S_CTL,         // small control block
+    S_PREV,        // uint32 per vertex: SSSP distance bits at the round start
     S_AUX0,
     S_AUX1,
     S_AUX2,

@@ -9,9 +9,10 @@
 namespace apspie {
 namespace trav {
 
-constexpr int WG = 256;           // threads per workgroup (4 waves of 64)
-constexpr int VPT = 8;            // vertices per thread in the compaction pass
-constexpr int CHUNK = WG * VPT;   // vertices per compaction workgroup
+constexpr int WG = 256;           // threads per expansion workgroup (4 waves of 64)
+constexpr int WGC = 512;          // threads per compaction workgroup (8 waves)
+constexpr int VPT = 16;           // vertices per thread in the compaction pass
+constexpr int CHUNK = WGC * VPT;  // vertices per compaction workgroup
 constexpr int TILE = 2048;        // frontier edges per expansion tile
 constexpr int IPT = TILE / WG;    // edges per thread per tile
 constexpr uint32_t INF_BITS = 0x7effffeeu;  // bit pattern of 1.70141e38f
@@ -37,6 +38,7 @@ struct Acc {
 struct Control {
     Ctl ctl[2];                 // by round parity
     unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
+    unsigned int nnew[2];         // vertices picked by the compaction, by round parity
     Acc acc;
 };
 
@@ -45,6 +47,7 @@ struct Bufs {
     unsigned expand_grid = 0;
     uint8_t *flags = nullptr;
     uint32_t *visited = nullptr;
+    uint32_t *prev = nullptr;  // SSSP: distance bits at the start of the round
     int32_t *queue = nullptr, *qstart = nullptr, *qoff = nullptr, *tiles = nullptr;
     float *qval = nullptr;
     uint64_t *status = nullptr;

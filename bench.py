@@ -88,11 +88,17 @@ def main():
     ap.add_argument("--no-bfs", action="store_true", help="skip the BFS s22 side measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scale", type=int, default=SSSP_SCALE)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="run only the 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
               "described in DESIGN.md and not run here", file=sys.stderr)
         sys.exit(2)
+    if args.copy_only:
+        with A.Context(0) as c:
+            print(json.dumps({"copy_gbps": round(c.hbm_copy_bandwidth(1 << 30, 6), 1)}))
+        return
 
     t = time.time()
     h = A.rmat(args.scale, 16, SSSP_SEED, undirected=False, weight_seed=SSSP_WSEED)
@@ -100,6 +106,8 @@ def main():
     log(f"generated RMAT s{args.scale} directed: m={h.m} nnz={h.nnz} src={src} ({time.time() - t:.1f}s)")
 
     ctx = A.Context(0)
+    copy_gbps = ctx.hbm_copy_bandwidth(1 << 30, 5)
+    log(f"device copy: {copy_gbps:.0f} GB/s (read + write)")
     g = ctx.upload(h)
     d = ctx.alloc(4 * h.m)
     run = lambda: ctx.sssp_into(g, src, d)  # noqa: E731
@@ -155,6 +163,7 @@ def main():
         "cold_ms": round(cold.kernel_ms, 4),
         "bytes_alg": st.bytes_alg,
         "traversal_frac": round(st.bytes_alg / (statistics.median(kms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "measured_copy_gbps": round(copy_gbps, 1),
         "roofline": roof,
     }
 

@@ -119,6 +119,11 @@ int apspie_free(apspie_ctx *ctx, void *ptr);
 int apspie_memcpy_h2d(apspie_ctx *ctx, void *dst, const void *src, size_t bytes);
 int apspie_memcpy_d2h(apspie_ctx *ctx, void *dst, const void *src, size_t bytes);
 
+/* STREAM-like device copy of `bytes` (read + write counted), `iters` times;
+ * *gbps = best rate.  Reports the measured HBM bandwidth beside the nominal
+ * peak (the reference has no counterpart). */
+int apspie_hbm_copy_bandwidth(apspie_ctx *ctx, size_t bytes, int iters, double *gbps);
+
 /* ---- graph loading -------------------------------------------------------- */
 #define APSPIE_LOAD_UNDIRECTED 1u   /* makeSymmetric (src/util.cuh:572-643) */
 #define APSPIE_LOAD_SORT_COLUMNS 2u /* sort each row's columns (not in the reference) */
@@ -165,6 +170,21 @@ int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist
  * Replaces spmspvSssp (src/spmspvSssp.cuh:92). */
 int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, float *d_y,
                      apspie_stats *stats);
+
+/* Maximal independent set of a symmetric graph with fixed priorities: the
+ * result is the greedy MIS that visits vertices by descending key
+ * (d_prio[v], -v) -- Luby's rule "join iff your key beats every undecided
+ * neighbour" iterated to its fixed point.  d_mis[m] receives 1 (in the set)
+ * or 0.  Replaces spmspvMis (src/spmspvMis.cuh:150), whose FP32 cuRAND
+ * priorities and `delta` threshold schedule give way to uint32 priorities
+ * (the schedule does not change the fixed point). */
+int apspie_mis(apspie_ctx *ctx, const apspie_csr *g, const uint32_t *d_prio, int32_t *d_mis,
+               apspie_stats *stats);
+
+/* One Boolean SpMSpV: d_out[v] = 1 iff v is in the row of some u with
+ * d_in[u] != 0, else 0.  Replaces spmspvCsr (src/spmspvMis.cuh:81), the
+ * reference's MIS validity check. */
+int apspie_spmspv_or(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_in, int32_t *d_out);
 
 #ifdef __cplusplus
 }
