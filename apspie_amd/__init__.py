@@ -104,6 +104,8 @@ def lib() -> C.CDLL:
         "apspie_sssp_step": [vp, P(_Csr), vp, vp, P(Stats)],
         "apspie_mis": [vp, P(_Csr), vp, vp, P(Stats)],
         "apspie_spmspv_or": [vp, P(_Csr), vp, vp],
+        "apspie_powerlaw_csr": [i32, i32, C.c_double, i32, u32, P(_Csr)],
+        "apspie_spgemm": [vp, P(_Csr), P(_Csr), P(_Csr), P(Stats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -185,6 +187,13 @@ def rmat(scale: int, edge_factor: int = 16, seed: int = 1, undirected: bool = Fa
     s = _Csr()
     flags = (RMAT_UNDIRECTED if undirected else 0) | (RMAT_WEIGHTED if weight_seed is not None else 0)
     _check(lib().apspie_rmat(scale, edge_factor, seed, flags, weight_seed or 0, C.byref(s)), None, "rmat")
+    return _host_from_c(s)
+
+
+def powerlaw(m: int, n: int, mean_len: float = 8.0, max_len: int = 4096, seed: int = 3) -> HostCsr:
+    """Seeded matrix with power-law row lengths, uniform sorted columns, U(-1,1) values."""
+    s = _Csr()
+    _check(lib().apspie_powerlaw_csr(m, n, mean_len, max_len, seed, C.byref(s)), None, "powerlaw")
     return _host_from_c(s)
 
 
@@ -356,6 +365,34 @@ class Context:
         dp.free()
         dm.free()
         return out, st
+
+    def spgemm_device(self, a: DeviceCsr, b: DeviceCsr):
+        """C = A * B left on the device: returns (_Csr owned by the library, Stats)."""
+        c = _Csr()
+        st = Stats()
+        _check(lib().apspie_spgemm(self.h, C.byref(a.c), C.byref(b.c), C.byref(c), C.byref(st)), self.h, "spgemm")
+        return c, st
+
+    def free_owned(self, c: _Csr) -> None:
+        lib().apspie_csr_free(self.h, C.byref(c))
+
+    def download_csr(self, c: _Csr) -> HostCsr:
+        m, nnz = c.m, c.nnz
+        rp = np.empty(m + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        va = np.empty(nnz, np.float32)
+        for host, dev in ((rp, c.row_ptr), (ci, c.col_ind), (va, c.val)):
+            if host.nbytes:
+                _check(lib().apspie_memcpy_d2h(self.h, host.ctypes.data, dev, host.nbytes), self.h, "d2h")
+        return HostCsr(m, c.n, rp, ci, va)
+
+    def spgemm(self, a: DeviceCsr, b: DeviceCsr):
+        """C = A * B; returns (HostCsr C, Stats)."""
+        c, st = self.spgemm_device(a, b)
+        try:
+            return self.download_csr(c), st
+        finally:
+            self.free_owned(c)
 
     def spmspv_or(self, g: DeviceCsr, x: np.ndarray) -> np.ndarray:
         """Boolean SpMSpV: out[v] = 1 iff some u with x[u] != 0 has v in its row."""

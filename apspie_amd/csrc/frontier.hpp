@@ -182,6 +182,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int 
 //                                                  (v -> c) at col_ind[p]
 //   void apply(int c, int p, uint32_t payload, int v, const Pre &)
 //   static constexpr bool kVert                    needs the entry's vertex id
+//   static constexpr bool kVal                     needs the entry's payload
 // Columns outside [0, m) are skipped.
 template <class Op>
 __global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Control *C, Queue q,
@@ -189,7 +190,7 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Co
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];  // start - (off - e0): col_ind index of slot 0
-    __shared__ uint32_t s_val[TILE + 1];
+    __shared__ uint32_t s_val[Op::kVal ? TILE + 1 : 1];
     __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
     __shared__ int32_t s_wmax[NW];
     const Ctl cur = C->ctl[(level - 1) & 1];
@@ -204,7 +205,7 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Co
         for (int k = tid; k < cnt; k += WG) {
             const int o = q.off[f0 + k] - e0;
             s_base[k] = q.start[f0 + k] - o;
-            s_val[k] = q.val[f0 + k];
+            if (Op::kVal) s_val[k] = q.val[f0 + k];
             if (Op::kVert) s_vert[k] = q.vert[f0 + k];
             if (o > 0 && o < TILE) s_own[o] = k;
         }
@@ -248,11 +249,11 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Co
 #pragma unroll
         for (int r = 0; r < IPT; ++r)
             if ((unsigned)col[r] < (unsigned)m)
-                pre[r] = op.load(col[r], pos[r], s_val[own[r]], Op::kVert ? s_vert[own[r]] : 0);
+                pre[r] = op.load(col[r], pos[r], Op::kVal ? s_val[own[r]] : 0u, Op::kVert ? s_vert[own[r]] : 0);
 #pragma unroll
         for (int r = 0; r < IPT; ++r)
             if ((unsigned)col[r] < (unsigned)m)
-                op.apply(col[r], pos[r], s_val[own[r]], Op::kVert ? s_vert[own[r]] : 0, pre[r]);
+                op.apply(col[r], pos[r], Op::kVal ? s_val[own[r]] : 0u, Op::kVert ? s_vert[own[r]] : 0, pre[r]);
         __syncthreads();
     }
 }

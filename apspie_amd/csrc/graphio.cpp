@@ -342,4 +342,66 @@ int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint3
     });
 }
 
+// Row lengths from a discrete power law P(len = k) ~ k^-alpha on
+// [1, max_len], alpha solved by bisection so the mean is mean_len; columns
+// uniform without replacement within a row (multiply-shift of one draw,
+// redrawn on repeats), sorted; values U(-1, 1) in CSR order.  One MT19937
+// stream, per row: length draw, column draws, then value draws.
+int apspie_powerlaw_csr(int32_t m, int32_t n, double mean_len, int32_t max_len, uint32_t seed,
+                        apspie_host_csr *out) {
+    if (!out) return APSPIE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    return guarded(nullptr, [&] {
+        APSPIE_CHECK(m >= 0 && n >= 1 && max_len >= 1 && max_len <= n, APSPIE_EINVAL, "bad power-law shape");
+        APSPIE_CHECK(mean_len >= 1.0 && mean_len <= max_len, APSPIE_EINVAL, "bad mean row length");
+        auto mean_of = [&](double a) {
+            double z = 0, s = 0;
+            for (int k = 1; k <= max_len; ++k) {
+                const double w = std::pow(double(k), -a);
+                z += w;
+                s += w * k;
+            }
+            return s / z;
+        };
+        double lo = -4.0, hi = 8.0;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            if (mean_of(mid) > mean_len) lo = mid;
+            else hi = mid;
+        }
+        const double alpha = 0.5 * (lo + hi);
+        std::vector<double> cdf(size_t(max_len) + 1, 0.0);
+        for (int k = 1; k <= max_len; ++k) cdf[k] = cdf[k - 1] + std::pow(double(k), -alpha);
+        for (int k = 1; k <= max_len; ++k) cdf[k] /= cdf[max_len];
+        BlockMT g(seed);
+        std::vector<int32_t> rp(size_t(m) + 1, 0), cols;
+        std::vector<float> vals;
+        std::vector<int32_t> mark(size_t(n), -1), row;
+        cols.reserve(size_t(double(m) * mean_len * 1.1) + 16);
+        vals.reserve(cols.capacity());
+        for (int32_t i = 0; i < m; ++i) {
+            const double u = (double(g.next()) + 0.5) / 4294967296.0;
+            const int len = int(std::lower_bound(cdf.begin() + 1, cdf.end(), u) - cdf.begin());
+            row.clear();
+            while ((int)row.size() < len) {
+                const int32_t c = int32_t((uint64_t(g.next()) * uint64_t(n)) >> 32);
+                if (mark[c] == i) continue;
+                mark[c] = i;
+                row.push_back(c);
+            }
+            std::sort(row.begin(), row.end());
+            for (int32_t c : row) {
+                cols.push_back(c);
+                vals.push_back(float((double(g.next()) + 0.5) / 4294967296.0 * 2.0 - 1.0));
+            }
+            APSPIE_CHECK(cols.size() < (size_t(1) << 31) - 4096, APSPIE_ELIMIT, "power-law matrix too large");
+            rp[i + 1] = int32_t(cols.size());
+        }
+        host_csr_alloc(out, m, n, int64_t(cols.size()), true);
+        memcpy(out->row_ptr, rp.data(), (size_t(m) + 1) * 4);
+        memcpy(out->col_ind, cols.data(), cols.size() * 4);
+        memcpy(out->val, vals.data(), vals.size() * 4);
+    });
+}
+
 }  // extern "C"

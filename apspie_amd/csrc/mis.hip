@@ -43,7 +43,7 @@ struct PickUndecided {
 };
 
 struct OpBlock {
-    static constexpr bool kVert = true;
+    static constexpr bool kVert = true, kVal = true;
     struct Pre {
         uint32_t p;
         uint8_t s;
@@ -74,7 +74,7 @@ struct PickJoin {
 };
 
 struct OpRemove {
-    static constexpr bool kVert = false;
+    static constexpr bool kVert = false, kVal = false;
     using Pre = uint8_t;
     uint8_t *state;
     __device__ Pre load(int c, int, uint32_t, int) const { return state[c]; }
@@ -92,7 +92,7 @@ struct PickNonzero {
 };
 
 struct OpSetOne {
-    static constexpr bool kVert = false;
+    static constexpr bool kVert = false, kVal = false;
     using Pre = int;
     int32_t *out;
     __device__ Pre load(int, int, uint32_t, int) const { return 0; }

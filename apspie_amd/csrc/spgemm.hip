@@ -1,0 +1,462 @@
+// spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned hash
+// tables (LDS for rows up to 4096 products, global memory beyond).
+//
+// Pipeline (one context stream):
+//   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
+//                edge-parallel over A with the frontier kernels.
+//   2. bins    rows by ub: <= 32, <= 256, <= 2048, <= 4096 (LDS tables of
+//              64 / 512 / 4096 / 8192 slots, one wave .. 512 threads per row)
+//              and > 4096 (global table of 2 * pow2ceil(ub) slots per row).
+//   3. symbolic: insert every product's column; nnz(C(i,:)) = distinct count.
+//   4. row_ptr(C) = exclusive scan (decoupled look-back).
+//   5. numeric: insert (column, a * b) with FP64 accumulation (products of
+//      FP32 inputs are exact in FP64; the sum is rounded to FP32 once), then
+//      write the row out with sorted columns.
+// The table uses an order-preserving home slot, home(c) = c * H / n, with
+// linear probing and no wrap-around (2H slots for at most H keys): a key's
+// slot lies in the cluster (run of occupied slots) holding its home, so all
+// keys of an earlier cluster are smaller and the row comes out sorted after
+// an insertion sort inside each cluster.  Correct for any column
+// distribution; fast when columns spread over [0, n).
+//
+// Replaces spgemm (src/spgemm.cuh:131-342, 2D-partitioned DCSC outer product
+// with per-partition global hash tables, src/spgemmKernel.cuh:164-738) and the
+// cuSPARSE csrgemm oracle path (src/spgemm.cuh:344-445).  Structural zeros
+// (products that cancel) are kept, as the reference's hash does
+// (src/spgemmKernel.cuh:194-198).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "frontier.hpp"
+#include "internal.hpp"
+
+namespace apspie {
+namespace sg {
+
+using namespace dev;
+using trav::Bufs;
+using trav::Control;
+using trav::WG;
+using trav::WGC;
+
+struct CsrD {
+    const int32_t *rp, *col;
+    const float *val;
+    int m, n;
+};
+
+constexpr int NBINS = 5;
+constexpr int BIN_UB[NBINS - 1] = {32, 256, 2048, 4096};
+
+// ub[i] += nnz(B(c,:)) for every edge (i -> c) of A
+struct OpUpper {
+    static constexpr bool kVert = true, kVal = false;
+    using Pre = int32_t;
+    const int32_t *brp;
+    int32_t *ub;
+    __device__ Pre load(int c, int, uint32_t, int) const { return brp[c + 1] - brp[c]; }
+    __device__ void apply(int, int, uint32_t, int v, Pre len) const {
+        if (len) atomicAdd(&ub[v], len);
+    }
+};
+
+struct PickRows {
+    int m;
+    __device__ bool pick(int v, uint32_t &) const { return v < m; }
+    __device__ void on_round(uint64_t, int, int) const {}
+    __device__ void on_new(int) const {}
+};
+
+__global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_init_ctl(int nblk, uint64_t *status, Control *C) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < 2 * nblk; i += gridDim.x * 256) status[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Control z{};
+        *C = z;
+    }
+}
+
+// rows -> bin lists; rows without products get nnz 0 directly
+__global__ __launch_bounds__(256) void k_bin(int m, const int32_t *ub, int32_t *nnzc, int32_t *lists,
+                                             int32_t *counts, unsigned long long *products) {
+    unsigned long long mine = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const int u = ub[i];
+        mine += (unsigned long long)u;
+        if (u == 0) {
+            nnzc[i] = 0;
+            continue;
+        }
+        int b = NBINS - 1;
+#pragma unroll
+        for (int k = NBINS - 2; k >= 0; --k)
+            if (u <= BIN_UB[k]) b = k;
+        const int pos = atomicAdd(&counts[b], 1);
+        lists[int64_t(b) * m + pos] = i;
+    }
+    mine = wave_sum(mine);
+    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(products, mine);
+}
+
+// exclusive scan of n int32 into out[0..n], out[n] = total (single pass)
+__global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, int n, uint64_t *status,
+                                              unsigned int *counter) {
+    constexpr int ITEMS = 16, TILE_N = 256 * ITEMS;
+    __shared__ int s_bid;
+    __shared__ uint64_t s_w[4];
+    __shared__ uint64_t s_pre;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) s_bid = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int bid = s_bid;
+    const int64_t b0 = int64_t(bid) * TILE_N + int64_t(tid) * ITEMS;
+    int32_t v[ITEMS];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        v[k] = b0 + k < n ? in[b0 + k] : 0;
+        sum += uint64_t(uint32_t(v[k]));
+    }
+    const uint64_t incl = wave_incl_scan(sum, lane);
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint64_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wid) wpre += s_w[w];
+        tot += s_w[w];
+    }
+    if (wid == 0) {
+        const uint64_t p = lookback_exclusive(status, bid, tot, lane);
+        if (lane == 0) s_pre = p;
+    }
+    __syncthreads();
+    uint64_t run = s_pre + wpre + incl - sum;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (b0 + k < n) out[b0 + k] = int32_t(run);
+        run += uint64_t(uint32_t(v[k]));
+        if (b0 + k == n - 1) out[n] = int32_t(run);
+    }
+    if (n == 0 && bid == 0 && tid == 0) out[0] = 0;
+}
+
+template <int T>
+__device__ __forceinline__ int block_excl_scan(int x, int *s_tmp, int &total) {
+    constexpr int NW = T / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int incl = (int)wave_incl_scan32((uint32_t)x, lane);
+    if (lane == WAVE - 1) s_tmp[wid] = incl;
+    __syncthreads();
+    int pre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wid) pre += s_tmp[w];
+        tot += s_tmp[w];
+    }
+    __syncthreads();
+    total = tot;
+    return pre + incl - x;
+}
+
+template <bool GLOBAL>
+__device__ __forceinline__ int ld_key(const int32_t *key, int s) {
+    if (GLOBAL) return __hip_atomic_load(key + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return key[s];
+}
+
+template <bool GLOBAL>
+__device__ __forceinline__ double ld_val(const double *val, int s) {
+    if (GLOBAL) return __hip_atomic_load(val + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return val[s];
+}
+
+// One row per workgroup iteration.  T threads, S table slots (LDS) or the
+// row's own slice of a global table.
+template <int T, int S, bool NUMERIC, bool GLOBAL>
+__global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
+                                            int32_t *nnzc, const int32_t *__restrict__ crp, int32_t *ccol,
+                                            float *cval, int32_t *gkeys, double *gvals,
+                                            const int64_t *__restrict__ goff, const int32_t *__restrict__ gslots) {
+    __shared__ int32_t s_key[GLOBAL ? 1 : S];
+    __shared__ double s_val[(NUMERIC && !GLOBAL) ? S : 1];
+    __shared__ int32_t s_bs[T];
+    __shared__ int32_t s_pre[T];
+    __shared__ float s_a[T];
+    __shared__ int32_t s_tmp[T / WAVE + 1];
+    __shared__ int32_t s_cnt;
+    const int tid = threadIdx.x;
+    for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
+        const int i = rows[ri];
+        int32_t *key = s_key;
+        double *val = s_val;
+        int slots = S;
+        if (GLOBAL) {
+            key = gkeys + goff[ri];
+            val = gvals + goff[ri];
+            slots = gslots[ri];
+        }
+        const int H = slots / 2;
+        for (int s = tid; s < slots; s += T) {
+            if (GLOBAL) {
+                __hip_atomic_store(key + s, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (NUMERIC) __hip_atomic_store(val + s, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                key[s] = -1;
+                if (NUMERIC) val[s] = 0.0;
+            }
+        }
+        if (tid == 0) s_cnt = 0;
+        __syncthreads();
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        for (int jb = ab; jb < ae; jb += T) {
+            const int j = jb + tid;
+            int L = 0, bs = 0;
+            float a = 0.f;
+            if (j < ae) {
+                const int k = A.col[j];
+                if ((unsigned)k < (unsigned)B.m) {
+                    a = A.val ? A.val[j] : 1.0f;
+                    bs = B.rp[k];
+                    L = B.rp[k + 1] - bs;
+                }
+            }
+            int tot;
+            const int pre = block_excl_scan<T>(L, s_tmp, tot);
+            s_bs[tid] = bs;
+            s_pre[tid] = pre;
+            s_a[tid] = a;
+            const int ne = min(T, ae - jb);
+            __syncthreads();
+            for (int t = tid; t < tot; t += T) {
+                int lo = 0, hi = ne - 1;  // last entry whose prefix <= t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int q = s_bs[lo] + (t - s_pre[lo]);
+                const int c = B.col[q];
+                if ((unsigned)c >= (unsigned)B.n) continue;
+                const double prod = NUMERIC ? double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f) : 0.0;
+                int s = int((uint64_t(uint32_t(c)) * uint64_t(H)) / uint64_t(uint32_t(B.n)));
+                for (;;) {
+                    const int old = atomicCAS(&key[s], -1, c);
+                    if (old == -1 || old == c) {
+                        if (NUMERIC) atomicAdd(&val[s], prod);
+                        else if (old == -1) atomicAdd(&s_cnt, 1);
+                        break;
+                    }
+                    ++s;  // < slots: at most H keys, homes < H
+                }
+            }
+            __syncthreads();
+        }
+        if (!NUMERIC) {
+            if (tid == 0) nnzc[i] = s_cnt;
+            __syncthreads();
+            continue;
+        }
+        // extraction: each thread owns a segment of slots and emits the
+        // clusters that start in it (a cluster running in from the left
+        // belongs to the previous segment)
+        const int seg = slots / T;
+        const int sb = tid * seg, se = sb + seg;
+        int p = sb;
+        if (p > 0 && ld_key<GLOBAL>(key, p - 1) != -1)
+            while (p < slots && ld_key<GLOBAL>(key, p) != -1) ++p;
+        int cnt = 0;
+        for (int q = p; q < se;) {
+            if (ld_key<GLOBAL>(key, q) == -1) {
+                ++q;
+                continue;
+            }
+            while (q < slots && ld_key<GLOBAL>(key, q) != -1) {
+                ++cnt;
+                ++q;
+            }
+        }
+        int tot;
+        int out = crp[i] + block_excl_scan<T>(cnt, s_tmp, tot);
+        for (int q = p; q < se;) {
+            if (ld_key<GLOBAL>(key, q) == -1) {
+                ++q;
+                continue;
+            }
+            const int o0 = out;
+            while (q < slots) {
+                const int c = ld_key<GLOBAL>(key, q);
+                if (c == -1) break;
+                // insertion into the sorted run [o0, out)
+                const float v = float(ld_val<GLOBAL>(val, q));
+                int w = out;
+                while (w > o0 && ccol[w - 1] > c) {
+                    ccol[w] = ccol[w - 1];
+                    cval[w] = cval[w - 1];
+                    --w;
+                }
+                ccol[w] = c;
+                cval[w] = v;
+                ++out;
+                ++q;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+struct Plan {
+    int counts[NBINS] = {};
+    int32_t *lists = nullptr;
+    std::vector<int32_t> grows;
+    int64_t gslots_total = 0;
+};
+
+template <bool NUMERIC>
+void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, int32_t *nnzc, const int32_t *crp,
+                 int32_t *ccol, float *cval, int32_t *gkeys, double *gvals, const int64_t *goff,
+                 const int32_t *gslots) {
+    const int m = A.m;
+    auto grid = [&](int n, unsigned per_cu) {
+        return std::max(1u, std::min((unsigned)n, (unsigned)ctx->cu_count * per_cu));
+    };
+    if (P.counts[0])
+        k_rows<64, 64, NUMERIC, false><<<grid(P.counts[0], 16), 64, 0, ctx->stream>>>(
+            P.lists + 0 * int64_t(m), P.counts[0], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[1])
+        k_rows<128, 512, NUMERIC, false><<<grid(P.counts[1], 8), 128, 0, ctx->stream>>>(
+            P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[2])
+        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[2], 4), 256, 0, ctx->stream>>>(
+            P.lists + 2 * int64_t(m), P.counts[2], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[3])
+        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[3], 1), 512, 0, ctx->stream>>>(
+            P.lists + 3 * int64_t(m), P.counts[3], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[4])
+        k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[4], 2), 1024, 0, ctx->stream>>>(
+            P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, gkeys, gvals, goff, gslots);
+    APSPIE_HIP(hipGetLastError());
+}
+
+}  // namespace sg
+}  // namespace apspie
+
+using namespace apspie;
+
+extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
+                             apspie_stats *stats) {
+    if (!ctx || !c) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    memset(c, 0, sizeof(*c));
+    return guarded(ctx, [&] {
+        using namespace sg;
+        validate_csr(a, false);
+        validate_csr(b, false);
+        APSPIE_CHECK(a->n == b->m, APSPIE_EINVAL, "inner dimensions differ");
+        const int m = a->m;
+        const CsrD A{a->row_ptr, a->col_ind, a->val, a->m, a->n};
+        const CsrD B{b->row_ptr, b->col_ind, b->val, b->m, b->n};
+        APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        // 1. product upper bounds, edge-parallel over A
+        const Bufs fb = trav::get_bufs(ctx, m, a->nnz);
+        int32_t *ub = (int32_t *)ctx->slot(S_AUX0, (size_t(m) + 1) * 4);
+        int32_t *nnzc = (int32_t *)ctx->slot(S_AUX1, (size_t(m) + 1) * 4);
+        int32_t *lists = (int32_t *)ctx->slot(S_AUX2, size_t(NBINS) * (size_t(m) + 1) * 4);
+        int32_t *counts = (int32_t *)ctx->slot(S_AUX3, 256);
+        unsigned long long *products = (unsigned long long *)(counts + 16);
+        const unsigned g256 = std::max(1u, std::min(ceil_div(std::max(m, 1), 256), 4096u));
+        k_zero_i32<<<g256, 256, 0, ctx->stream>>>(ub, m + 1);
+        k_zero_i32<<<1, 256, 0, ctx->stream>>>(counts, 64);
+        k_init_ctl<<<4, 256, 0, ctx->stream>>>(fb.nblk, fb.status, fb.C);
+        const apspie_csr ga{a->m, a->n, a->nnz, a->row_ptr, a->col_ind, a->val};
+        trav::k_compact<PickRows><<<fb.nblk, WGC, 0, ctx->stream>>>(PickRows{m}, 0, m, fb.nblk, 1, ga.row_ptr,
+                                                                      trav::queue_of(fb), fb.status, fb.C);
+        trav::k_expand<OpUpper><<<fb.expand_grid, WG, 0, ctx->stream>>>(OpUpper{b->row_ptr, ub}, 1, a->n, fb.C,
+                                                                          trav::queue_of(fb), a->col_ind);
+        // 2. bins
+        k_bin<<<g256, 256, 0, ctx->stream>>>(m, ub, nnzc, lists, counts, products);
+        APSPIE_HIP(hipGetLastError());
+        Plan P;
+        P.lists = lists;
+        unsigned long long prod_total = 0;
+        int hcounts[NBINS];
+        APSPIE_HIP(hipMemcpyAsync(hcounts, counts, sizeof(hcounts), hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipMemcpyAsync(&prod_total, products, 8, hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        for (int k = 0; k < NBINS; ++k) P.counts[k] = hcounts[k];
+        // global tables for the largest rows: 2 * pow2ceil(ub) slots each
+        int64_t *goff = nullptr;
+        int32_t *gslots = nullptr, *gkeys = nullptr;
+        double *gvals = nullptr;
+        if (P.counts[4]) {
+            const int ng = P.counts[4];
+            std::vector<int32_t> rows(ng), ubs(ng);
+            APSPIE_HIP(hipMemcpyAsync(rows.data(), lists + 4 * int64_t(m), size_t(ng) * 4, hipMemcpyDeviceToHost,
+                                      ctx->stream));
+            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<int32_t> hub(size_t(m) + 1);
+            APSPIE_HIP(hipMemcpyAsync(hub.data(), ub, size_t(m) * 4, hipMemcpyDeviceToHost, ctx->stream));
+            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            std::vector<int64_t> off(ng);
+            std::vector<int32_t> sl(ng);
+            int64_t tot = 0;
+            for (int r = 0; r < ng; ++r) {
+                int64_t h = 1;
+                while (h < hub[rows[r]]) h <<= 1;
+                APSPIE_CHECK(2 * h < (int64_t(1) << 31), APSPIE_ELIMIT, "row with too many products");
+                sl[r] = int32_t(2 * h);
+                off[r] = tot;
+                tot += 2 * h;
+            }
+            goff = (int64_t *)ctx->slot(S_AUX4, size_t(ng) * 8);
+            gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
+            gkeys = (int32_t *)ctx->slot(S_AUX6, size_t(tot) * 4);
+            gvals = (double *)ctx->slot(S_AUX7, size_t(tot) * 8);
+            APSPIE_HIP(hipMemcpyAsync(goff, off.data(), size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
+            APSPIE_HIP(hipMemcpyAsync(gslots, sl.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
+        }
+        // 3. symbolic
+        launch_bins<false>(ctx, P, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
+        // 4. row pointers
+        APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
+        c->m = m;
+        c->n = b->n;
+        {
+            const int nb = std::max(1, (int)ceil_div(m, 256 * 16));
+            uint64_t *st = (uint64_t *)ctx->slot(S_STATUS, size_t(nb) * 8 + 64);
+            unsigned *ctr = (unsigned *)(st + nb);
+            k_zero_i32<<<std::max(1u, std::min(ceil_div(nb * 2 + 2, 256), 1024u)), 256, 0, ctx->stream>>>(
+                (int32_t *)st, int64_t(nb) * 2 + 2);
+            k_scan<<<nb, 256, 0, ctx->stream>>>(nnzc, c->row_ptr, m, st, ctr);
+        }
+        int32_t total = 0;
+        APSPIE_HIP(hipMemcpyAsync(&total, c->row_ptr + m, 4, hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        c->nnz = total;
+        if (hipMalloc(&c->col_ind, size_t(std::max(total, 1)) * 4) != hipSuccess ||
+            hipMalloc(&c->val, size_t(std::max(total, 1)) * 4) != hipSuccess) {
+            (void)hipGetLastError();
+            apspie_csr_free(ctx, c);
+            throw Error(APSPIE_ENOMEM, "output allocation failed");
+        }
+        // 5. numeric
+        launch_bins<true>(ctx, P, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
+        APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        if (stats) {
+            float ms = 0.f;
+            APSPIE_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            memset(stats, 0, sizeof(*stats));
+            stats->edges = (int64_t)prod_total;  // products
+            stats->vertices = total;             // nnz(C)
+            stats->iterations = 1;
+            stats->kernel_ms = ms;
+            stats->bytes_alg = 8.0 * (double(m) + 1) + 16.0 * double(a->nnz) + 8.0 * double(prod_total) +
+                               8.0 * double(total);
+        }
+    });
+}

@@ -91,7 +91,7 @@ struct PickStep {
 };
 
 struct OpBfs {
-    static constexpr bool kVert = false;
+    static constexpr bool kVert = false, kVal = false;
     using Pre = uint32_t;
     const uint32_t *visited;
     uint8_t *flags;
@@ -102,7 +102,7 @@ struct OpBfs {
 };
 
 struct OpRelax {
-    static constexpr bool kVert = false;
+    static constexpr bool kVert = false, kVal = true;
     struct Pre {
         float w;
         uint32_t d;

@@ -143,6 +143,12 @@ void apspie_host_csr_free(apspie_host_csr *h);
 int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint32_t weight_seed,
                 apspie_host_csr *out);
 
+/* Seeded m x n sparse matrix with power-law row lengths (mean mean_len,
+ * at most max_len), uniform distinct sorted columns, FP32 values in (-1, 1):
+ * the SpGEMM input (the reference reads .mtx files, test/spgemm/main.cu). */
+int apspie_powerlaw_csr(int32_t m, int32_t n, double mean_len, int32_t max_len, uint32_t seed,
+                        apspie_host_csr *out);
+
 /* Host CSR -> device (cudaMemcpy block of test/bfs/main.cu:104-120). */
 int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owned *out);
 int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c);
@@ -185,6 +191,16 @@ int apspie_mis(apspie_ctx *ctx, const apspie_csr *g, const uint32_t *d_prio, int
  * d_in[u] != 0, else 0.  Replaces spmspvCsr (src/spmspvMis.cuh:81), the
  * reference's MIS validity check. */
 int apspie_spmspv_or(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_in, int32_t *d_out);
+
+/* C = A * B (CSR, FP32 values, FP64 accumulation), sorted columns, structural
+ * zeros kept.  C's arrays are allocated by the library (release with
+ * apspie_csr_free).  stats: edges = products, vertices = nnz(C).  Replaces
+ * spgemm(d_matrix *C, d_matrix *A, d_matrix *B, partSize, partNum, ctx)
+ * (src/spgemm.cuh:132), whose C fields are callee-allocated the same way
+ * (src/spgemmKernel.cuh:669-681), and its cuSPARSE twin
+ * spgemm<T>(C, A, B) (src/spgemm.cuh:345). */
+int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
+                  apspie_stats *stats);
 
 #ifdef __cplusplus
 }
