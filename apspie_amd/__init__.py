@@ -106,6 +106,8 @@ def lib() -> C.CDLL:
         "apspie_spmspv_or": [vp, P(_Csr), vp, vp],
         "apspie_powerlaw_csr": [i32, i32, C.c_double, i32, u32, P(_Csr)],
         "apspie_spgemm": [vp, P(_Csr), P(_Csr), P(_Csr), P(Stats)],
+        "apspie_strict_lower": [P(_Csr), P(_Csr)],
+        "apspie_tc": [vp, P(_Csr), P(i64), P(Stats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -187,6 +189,14 @@ def rmat(scale: int, edge_factor: int = 16, seed: int = 1, undirected: bool = Fa
     s = _Csr()
     flags = (RMAT_UNDIRECTED if undirected else 0) | (RMAT_WEIGHTED if weight_seed is not None else 0)
     _check(lib().apspie_rmat(scale, edge_factor, seed, flags, weight_seed or 0, C.byref(s)), None, "rmat")
+    return _host_from_c(s)
+
+
+def strict_lower(h: HostCsr) -> HostCsr:
+    """buildLower: rows keep their columns j < i, sorted."""
+    s = _Csr()
+    hc = h._as_c()
+    _check(lib().apspie_strict_lower(C.byref(hc), C.byref(s)), None, "strict_lower")
     return _host_from_c(s)
 
 
@@ -393,6 +403,13 @@ class Context:
             return self.download_csr(c), st
         finally:
             self.free_owned(c)
+
+    def tc(self, l: DeviceCsr):
+        """Triangles from the strict lower triangle L; returns (int count, Stats)."""
+        cnt = C.c_int64()
+        st = Stats()
+        _check(lib().apspie_tc(self.h, C.byref(l.c), C.byref(cnt), C.byref(st)), self.h, "tc")
+        return int(cnt.value), st
 
     def spmspv_or(self, g: DeviceCsr, x: np.ndarray) -> np.ndarray:
         """Boolean SpMSpV: out[v] = 1 iff some u with x[u] != 0 has v in its row."""

@@ -342,6 +342,32 @@ int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint3
     });
 }
 
+// buildLower (test/tc/main.cu:39-55): keep the entries strictly below the
+// diagonal (row i keeps columns j < i), columns sorted.
+int apspie_strict_lower(const apspie_host_csr *a, apspie_host_csr *l) {
+    if (!a || !l) return APSPIE_EINVAL;
+    memset(l, 0, sizeof(*l));
+    return guarded(nullptr, [&] {
+        APSPIE_CHECK(a->m >= 0 && a->row_ptr, APSPIE_EINVAL, "bad input csr");
+        std::vector<int32_t> rp(size_t(a->m) + 1, 0);
+        for (int32_t i = 0; i < a->m; ++i) {
+            int32_t c = 0;
+            for (int32_t p = a->row_ptr[i]; p < a->row_ptr[i + 1]; ++p) c += a->col_ind[p] < i;
+            rp[i + 1] = rp[i] + c;
+        }
+        host_csr_alloc(l, a->m, a->n, rp[a->m], false);
+        memcpy(l->row_ptr, rp.data(), (size_t(a->m) + 1) * 4);
+        parallel_ranges(a->m, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i) {
+                int32_t k = rp[i];
+                for (int32_t p = a->row_ptr[i]; p < a->row_ptr[i + 1]; ++p)
+                    if (a->col_ind[p] < i) l->col_ind[k++] = a->col_ind[p];
+                std::sort(l->col_ind + rp[i], l->col_ind + k);
+            }
+        });
+    });
+}
+
 // Row lengths from a discrete power law P(len = k) ~ k^-alpha on
 // [1, max_len], alpha solved by bisection so the mean is mean_len; columns
 // uniform without replacement within a row (multiply-shift of one draw,

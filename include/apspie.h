@@ -149,6 +149,10 @@ int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint3
 int apspie_powerlaw_csr(int32_t m, int32_t n, double mean_len, int32_t max_len, uint32_t seed,
                         apspie_host_csr *out);
 
+/* Strict lower triangle of a host CSR (row i keeps j < i), sorted columns:
+ * the L of triangle counting.  Replaces buildLower (test/tc/main.cu:39-55). */
+int apspie_strict_lower(const apspie_host_csr *a, apspie_host_csr *l);
+
 /* Host CSR -> device (cudaMemcpy block of test/bfs/main.cu:104-120). */
 int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owned *out);
 int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c);
@@ -201,6 +205,13 @@ int apspie_spmspv_or(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_in, 
  * spgemm<T>(C, A, B) (src/spgemm.cuh:345). */
 int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
                   apspie_stats *stats);
+
+/* Triangle count of an undirected graph from its strict lower triangle L
+ * (row i holds the sorted columns j < i): *count = sum over (i,k) in L of
+ * |L(i,:) & L(k,:)|, exact int64.  stats: edges = wedges checked,
+ * vertices = nnz(L).  Replaces the TC pipeline of test/tc/main.cu:148-261
+ * (cuSPARSE L*L^T, ewiseMultTc src/tc.cuh:168-199, float mgpu::Reduce). */
+int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, apspie_stats *stats);
 
 #ifdef __cplusplus
 }
