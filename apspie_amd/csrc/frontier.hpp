@@ -94,7 +94,9 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int 
         vv[r] = payload;
         if (ball) {
             pk.on_round(ball, vbase + r * WAVE, lane);
-            const int a = v < m ? row_ptr[v] : 0;
+            // row_ptr has m + 1 entries: lane v + 1 == m supplies the row end
+            // of vertex m - 1
+            const int a = v <= m ? row_ptr[v] : 0;
             int b = __shfl_down(a, 1, WAVE);
             if (lane == WAVE - 1) b = v + 1 <= m ? row_ptr[v + 1] : 0;
             if (isnew) {

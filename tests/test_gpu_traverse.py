@@ -85,6 +85,26 @@ def test_bfs_edge_cases(ctx):
         ctx.bfs(gp, -1)
 
 
+@pytest.mark.parametrize("m", [3, 64, 65, 100, 16384, 16385])
+def test_last_vertex_is_expanded(ctx, m):
+    """0 -> m-1 -> 1 -> ... : the last vertex is the only bridge (its row end
+    comes from row_ptr[m]); checks BFS, SSSP and their counters."""
+    src = np.array([0, m - 1] + list(range(1, m - 2)), dtype=np.int64)
+    dst = np.array([m - 1, 1] + list(range(2, m - 1)), dtype=np.int64)
+    order = np.argsort(src, kind="stable")
+    rp = np.zeros(m + 1, np.int32)
+    np.add.at(rp, src + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    h = A.HostCsr(m, m, rp, dst[order].astype(np.int32), np.full(len(src), 2.0, np.float32))
+    g = ctx.upload(h)
+    depth, st = ctx.bfs(g, 0)
+    want, sd, e_exp, vr = O.bfs(ocsr(h), 0)
+    assert np.array_equal(depth, want) and st.edges == e_exp and st.vertices == vr
+    assert depth[1] == 2
+    dist, _ = ctx.sssp(g, 0)
+    assert np.array_equal(dist, dist_from_oracle(O.sssp_dijkstra(ocsr(h), 0)))
+
+
 @pytest.mark.parametrize("scale,seed", [(8, 2008), (12, 2012), (14, 2014)])
 def test_sssp_rmat(ctx, scale, seed):
     h = A.rmat(scale, 16, seed, undirected=False, weight_seed=seed + 1000)
