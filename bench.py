@@ -129,6 +129,13 @@ def main():
     # 4 B per improved vertex (distance write)
     exp_bytes = 12.0 * st.edges + 12.0 * st.vertices + 4.0 * st.improved
     roof = kernel_roofline(ctx, run, max(3, args.steps // 2), exp_bytes)
+    traffic_file = os.path.join(ROOT, "profiles", "k_expand_sssp_rmat_s22_traffic.json")
+    if os.path.exists(traffic_file) and args.scale == SSSP_SCALE:
+        # HBM bytes per launch of the same kernel on the same workload, from
+        # rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel
+        tr = json.load(open(traffic_file))
+        roof["traffic"] = int(tr["traffic_bytes_per_launch"])
+        roof["traffic_source"] = os.path.relpath(traffic_file, ROOT)
     log(f"roofline(k_expand): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak)")
 
     out = {
