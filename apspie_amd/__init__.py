@@ -108,6 +108,8 @@ def lib() -> C.CDLL:
         "apspie_spgemm": [vp, P(_Csr), P(_Csr), P(_Csr), P(Stats)],
         "apspie_strict_lower": [P(_Csr), P(_Csr)],
         "apspie_tc": [vp, P(_Csr), P(i64), P(Stats)],
+        "apspie_bfs_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
+        "apspie_sssp_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -403,6 +405,32 @@ class Context:
             return self.download_csr(c), st
         finally:
             self.free_owned(c)
+
+    def bfs_batch(self, g: DeviceCsr, srcs):
+        """BFS from up to 256 sources; returns (int32[k, m] depths, Stats)."""
+        srcs = np.ascontiguousarray(srcs, dtype=np.int32)
+        k = len(srcs)
+        ds = self.alloc(4 * max(k, 1)).upload(srcs)
+        dd = self.alloc(4 * max(k * g.m, 1))
+        st = Stats()
+        _check(lib().apspie_bfs_batch(self.h, C.byref(g.c), ds.ptr, k, dd.ptr, C.byref(st)), self.h, "bfs_batch")
+        out = dd.download(np.int32, k * g.m).reshape(k, g.m)
+        ds.free()
+        dd.free()
+        return out, st
+
+    def sssp_batch(self, g: DeviceCsr, srcs):
+        """SSSP from up to 64 sources; returns (float32[k, m] distances, Stats)."""
+        srcs = np.ascontiguousarray(srcs, dtype=np.int32)
+        k = len(srcs)
+        ds = self.alloc(4 * max(k, 1)).upload(srcs)
+        dd = self.alloc(4 * max(k * g.m, 1))
+        st = Stats()
+        _check(lib().apspie_sssp_batch(self.h, C.byref(g.c), ds.ptr, k, dd.ptr, C.byref(st)), self.h, "sssp_batch")
+        out = dd.download(np.float32, k * g.m).reshape(k, g.m)
+        ds.free()
+        dd.free()
+        return out, st
 
     def tc(self, l: DeviceCsr):
         """Triangles from the strict lower triangle L; returns (int count, Stats)."""

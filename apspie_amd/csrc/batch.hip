@@ -1,0 +1,400 @@
+// batch.hip -- many traversals of one graph at once.
+//
+// Batched BFS (k <= 256 sources): bit-parallel multi-source BFS.  Per vertex
+// W = ceil(k/64) words of visited / next / frontier bits (vertex-major); the
+// union frontier goes through the generic compaction and expansion, and every
+// edge (u -> c) of it serves all sources at once:
+//     next[c] |= frontier[u] & ~visited[c]      (64-bit atomicOr per word)
+// The compaction turns next into the new frontier bits, marks them visited and
+// writes depth[s * m + c] = level for each new (s, c).
+//
+// Batched SSSP (k <= 64 sources): synchronous-round Bellman-Ford over the
+// union frontier with vertex-major distances dist[v * K + s], so the probes of
+// all sources for one edge touch one 256-byte row; the 64 lanes of a wave are
+// the 64 sources of an edge.  A vertex enters the next frontier with the mask
+// of sources whose distance it improved (atomicOr of a ballot); relaxations
+// read live distances, so results equal k separate runs (the fixed point is
+// the shortest-path distances) while rounds may be fewer.
+//
+// Rounds are device-driven (the round number lives in device memory), so a
+// chunk of rounds is captured once into a hipGraph and replayed until the
+// union frontier is empty.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "frontier.hpp"
+#include "internal.hpp"
+
+namespace apspie {
+namespace trav {
+namespace {
+
+template <int W>
+struct PickMs {
+    uint64_t *next, *vis, *fr;
+    int32_t *depth;
+    const int *lvl;
+    int m, k;
+    __device__ bool pick(int v, uint32_t &) const {
+        if (v >= m) return false;
+        uint64_t n[W];
+        bool any = false;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint64_t x = next[int64_t(v) * W + w];
+            if (x) {
+                next[int64_t(v) * W + w] = 0;
+                x &= ~vis[int64_t(v) * W + w];
+            }
+            n[w] = x;
+            any |= x != 0;
+        }
+        if (!any) return false;
+        const int level = *(volatile const int *)lvl;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            fr[int64_t(v) * W + w] = n[w];
+            if (n[w]) vis[int64_t(v) * W + w] |= n[w];
+            uint64_t b = n[w];
+            while (b) {
+                const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
+                if (s < k) depth[int64_t(s) * m + v] = level;
+                b &= b - 1;
+            }
+        }
+        return true;
+    }
+    __device__ void on_round(uint64_t, int, int) const {}
+    __device__ void on_new(int) const {}
+};
+
+template <int W>
+struct OpMs {
+    static constexpr bool kVert = true, kVal = false;
+    struct Pre {
+        uint64_t f[W], v[W];
+    };
+    const uint64_t *fr, *vis;
+    uint64_t *next;
+    __device__ Pre load(int c, int, uint32_t, int u) const {
+        Pre p;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            p.f[w] = fr[int64_t(u) * W + w];
+            p.v[w] = vis[int64_t(c) * W + w];
+        }
+        return p;
+    }
+    __device__ void apply(int c, int, uint32_t, int, const Pre &p) const {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint64_t x = p.f[w] & ~p.v[w];
+            if (x) atomicOr((unsigned long long *)&next[int64_t(c) * W + w], (unsigned long long)x);
+        }
+    }
+};
+
+template <int W>
+__global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *srcs, uint64_t *next, uint64_t *vis,
+                                                 int32_t *depth, int nblk, uint64_t *status, Control *C) {
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = gtid; i < int64_t(m) * W; i += stride) {
+        next[i] = 0;
+        vis[i] = 0;
+    }
+    for (int64_t i = gtid; i < int64_t(m) * k; i += stride) depth[i] = -1;
+    for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
+    if (gtid == 0) {
+        Control z{};
+        *C = z;
+    }
+}
+
+template <int W>
+__global__ void k_ms_seed(int k, const int32_t *srcs, uint64_t *next) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < k) atomicOr((unsigned long long *)&next[int64_t(srcs[s]) * W + s / 64], 1ull << (s % 64));
+}
+
+// ---------------------------------------------------------------------------
+// batched SSSP
+
+struct PickChanged {
+    uint64_t *chg, *fr;
+    int m;
+    __device__ bool pick(int v, uint32_t &) const {
+        if (v >= m) return false;
+        const uint64_t x = chg[v];
+        if (!x) return false;
+        chg[v] = 0;
+        fr[v] = x;
+        return true;
+    }
+    __device__ void on_round(uint64_t, int, int) const {}
+    __device__ void on_new(int) const {}
+};
+
+// Expansion with lanes = sources: each wave takes the 64 edge slots of one
+// round (owners from the shared tile staging), broadcasts them one at a time
+// and relaxes all active sources of the edge with coalesced row accesses.
+template <int K>
+__global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, Queue q,
+                                                       const int32_t *__restrict__ col_ind,
+                                                       const float *__restrict__ val, uint32_t *dist,
+                                                       const uint64_t *__restrict__ fr, uint64_t *chg) {
+    constexpr int NW = WG / WAVE, SEG = TILE / NW, U = 4;
+    __shared__ int32_t s_own[TILE];
+    __shared__ int32_t s_base[TILE + 1];
+    __shared__ uint32_t s_val[1];
+    __shared__ int32_t s_vert[TILE + 1];
+    __shared__ int32_t s_wmax[NW];
+    const int level = C->cur_level;
+    const Ctl cur = C->ctl[(level - 1) & 1];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
+        int own[IPT];
+        const int elim = tile_owners<false, true>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
+#pragma unroll 1
+        for (int r = 0; r < IPT; ++r) {
+            const int s = wid * SEG + r * WAVE + lane;
+            int c = -1, u = 0;
+            float w = 0.f;
+            uint64_t fu = 0;
+            if (s < elim) {
+                const int p = s_base[own[r]] + s;
+                c = col_ind[p];
+                w = val ? val[p] : 1.0f;
+                u = s_vert[own[r]];
+                if ((unsigned)c < (unsigned)m) fu = fr[u];
+                else c = -1;
+            }
+            const uint32_t fu_lo = uint32_t(fu), fu_hi = uint32_t(fu >> 32);
+#pragma unroll 1
+            for (int j0 = 0; j0 < WAVE; j0 += U) {
+                int cj[U], uj[U];
+                float wj[U];
+                bool act[U];
+                uint32_t du[U], dc[U];
+#pragma unroll
+                for (int x = 0; x < U; ++x) {
+                    cj[x] = __shfl(c, j0 + x, WAVE);
+                    uj[x] = __shfl(u, j0 + x, WAVE);
+                    wj[x] = __shfl(w, j0 + x, WAVE);
+                    const uint32_t lo = __shfl(fu_lo, j0 + x, WAVE), hi = __shfl(fu_hi, j0 + x, WAVE);
+                    const uint64_t f = (uint64_t(hi) << 32) | lo;
+                    act[x] = cj[x] >= 0 && lane < K && ((f >> lane) & 1ull);
+                }
+#pragma unroll
+                for (int x = 0; x < U; ++x)
+                    if (act[x]) {
+                        du[x] = dist[int64_t(uj[x]) * K + lane];
+                        dc[x] = dist[int64_t(cj[x]) * K + lane];
+                    }
+#pragma unroll
+                for (int x = 0; x < U; ++x) {
+                    bool imp = false;
+                    if (act[x]) {
+                        const uint32_t nb = __float_as_uint(__uint_as_float(du[x]) + wj[x]);
+                        if (nb < dc[x]) {
+                            __hip_atomic_fetch_min(&dist[int64_t(cj[x]) * K + lane], nb, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            imp = true;
+                        }
+                    }
+                    const uint64_t b = __ballot(imp);
+                    if (b && lane == 0) atomicOr((unsigned long long *)&chg[cj[x]], (unsigned long long)b);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_mss_init(int m, int k, const int32_t *srcs, uint32_t *dist, uint64_t *chg,
+                                                  int nblk, uint64_t *status, Control *C) {
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, stride = int64_t(gridDim.x) * 256;
+    for (int64_t i = gtid; i < int64_t(m) * K; i += stride) dist[i] = INF_BITS;
+    for (int64_t i = gtid; i < m; i += stride) chg[i] = 0;
+    for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
+    if (gtid == 0) {
+        Control z{};
+        *C = z;
+    }
+}
+
+template <int K>
+__global__ void k_mss_seed(int k, const int32_t *srcs, uint32_t *dist, uint64_t *chg) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < k) {
+        dist[int64_t(srcs[s]) * K + s] = 0u;
+        atomicOr((unsigned long long *)&chg[srcs[s]], 1ull << s);
+    }
+}
+
+// vertex-major [m][K] -> source-major [k][m]
+template <int K>
+__global__ __launch_bounds__(256) void k_mss_transpose(int m, int k, const uint32_t *src, uint32_t *dst) {
+    __shared__ uint32_t tile[64][K + 1];
+    const int v0 = blockIdx.x * 64;
+    for (int i = threadIdx.x; i < 64 * K; i += 256) {
+        const int vv = i / K, s = i % K;
+        if (v0 + vv < m) tile[vv][s] = src[int64_t(v0 + vv) * K + s];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * k; i += 256) {
+        const int s = i / 64, vv = i % 64;
+        if (v0 + vv < m) dst[int64_t(s) * m + v0 + vv] = tile[vv][s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// driver: device-driven rounds captured in chunks into a hipGraph
+
+struct Replay {
+    hipGraphExec_t exec = nullptr;
+    ~Replay() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+template <class F>
+int run_rounds(apspie_ctx *ctx, const Bufs &b, int max_rounds, F &&enqueue_round) {
+    // round 0 (seeding compaction) is enqueued by the caller; capture CH rounds
+    constexpr int CH = 4;
+    hipGraph_t graph = nullptr;
+    Replay rp;
+    APSPIE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < CH; ++i) enqueue_round();
+    APSPIE_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&rp.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    APSPIE_HIP(ie);
+    volatile int32_t *host_nf = (volatile int32_t *)ctx->pinned;
+    int rounds = 0;
+    for (;;) {
+        APSPIE_HIP(hipGraphLaunch(rp.exec, ctx->stream));
+        rounds += CH;
+        // the frontier written by the last compaction of the chunk
+        APSPIE_HIP(hipMemcpyAsync((void *)(host_nf + 1), &b.C->cur_level, 4, hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        const int lvl = host_nf[1];
+        APSPIE_HIP(hipMemcpyAsync((void *)host_nf, &b.C->ctl[(lvl - 1) & 1].nF, 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        if (*host_nf == 0) break;
+        APSPIE_CHECK(rounds <= max_rounds, APSPIE_ENOCONV, "batched traversal did not converge");
+    }
+    return rounds;
+}
+
+template <int W>
+void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, int32_t *depth,
+               apspie_stats *stats) {
+    const Bufs b = get_bufs(ctx, g->m, g->nnz);
+    const size_t words = size_t(g->m) * W;
+    uint64_t *next = (uint64_t *)ctx->slot(S_AUX0, words * 8);
+    uint64_t *vis = (uint64_t *)ctx->slot(S_AUX1, words * 8);
+    uint64_t *fr = (uint64_t *)ctx->slot(S_AUX2, words * 8);
+    APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * std::max(W, k), 256), 8192u));
+    k_ms_init<W><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, next, vis, depth, b.nblk, b.status, b.C);
+    k_ms_seed<W><<<ceil_div(k, 256), 256, 0, ctx->stream>>>(k, srcs, next);
+    const PickMs<W> pk{next, vis, fr, depth, &b.C->cur_level, g->m, k};
+    const OpMs<W> op{fr, vis, next};
+    auto compact = [&] {
+        k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
+                                                               b.status, b.C);
+    };
+    // seeding round (round 0) runs eagerly, the rest from the captured graph
+    k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
+                                                           b.status, b.C);
+    APSPIE_HIP(hipGetLastError());
+    run_rounds(ctx, b, g->m + 2, [&] {
+        k_expand<OpMs<W>><<<b.expand_grid, WG, 0, ctx->stream>>>(op, -1, g->m, b.C, queue_of(b), g->col_ind);
+        compact();
+    });
+    APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    fill_stats(ctx, b, MODE_BFS, g->m, stats);
+    if (stats) stats->bytes_alg = 0.0;
+}
+
+void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, float *out,
+                apspie_stats *stats) {
+    constexpr int K = 64;
+    const Bufs b = get_bufs(ctx, g->m, g->nnz);
+    uint32_t *dist = (uint32_t *)ctx->slot(S_AUX0, size_t(g->m) * K * 4);
+    uint64_t *chg = (uint64_t *)ctx->slot(S_AUX1, size_t(g->m) * 8);
+    uint64_t *fr = (uint64_t *)ctx->slot(S_AUX2, size_t(g->m) * 8);
+    APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * K, 256), 8192u));
+    k_mss_init<K><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, dist, chg, b.nblk, b.status, b.C);
+    k_mss_seed<K><<<1, 64, 0, ctx->stream>>>(k, srcs, dist, chg);
+    const PickChanged pk{chg, fr, g->m};
+    k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
+                                                             b.status, b.C);
+    APSPIE_HIP(hipGetLastError());
+    run_rounds(ctx, b, g->m * 2 + 4, [&] {
+        k_expand_ms_sssp<K><<<b.expand_grid, WG, 0, ctx->stream>>>(g->m, b.C, queue_of(b), g->col_ind, g->val,
+                                                                    dist, fr, chg);
+        k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
+                                                                 b.status, b.C);
+    });
+    k_mss_transpose<K><<<ceil_div(g->m, 64), 256, 0, ctx->stream>>>(g->m, k, dist, (uint32_t *)out);
+    APSPIE_HIP(hipGetLastError());
+    APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    fill_stats(ctx, b, MODE_SSSP, g->m, stats);
+    if (stats) {
+        stats->improved = 0;
+        stats->bytes_alg = 0.0;
+    }
+}
+
+void check_sources(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k) {
+    std::vector<int32_t> h(k);
+    APSPIE_HIP(hipMemcpyAsync(h.data(), d_srcs, size_t(k) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < k; ++s) APSPIE_CHECK(h[s] >= 0 && h[s] < g->m, APSPIE_EINVAL, "source out of range");
+}
+
+}  // namespace
+}  // namespace trav
+}  // namespace apspie
+
+using namespace apspie;
+
+extern "C" {
+
+int apspie_bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int32_t k, int32_t *d_depth,
+                     apspie_stats *stats) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        validate_csr(g, false);
+        APSPIE_CHECK(g->m == g->n, APSPIE_EINVAL, "traversal needs a square adjacency matrix");
+        APSPIE_CHECK(k >= 1 && k <= 256, APSPIE_ELIMIT, "batched BFS takes 1..256 sources");
+        APSPIE_CHECK(d_srcs && d_depth, APSPIE_EINVAL, "null sources or output");
+        trav::check_sources(ctx, g, d_srcs, k);
+        const int W = (k + 63) / 64;
+        if (W == 1) trav::bfs_batch<1>(ctx, g, d_srcs, k, d_depth, stats);
+        else if (W == 2) trav::bfs_batch<2>(ctx, g, d_srcs, k, d_depth, stats);
+        else trav::bfs_batch<4>(ctx, g, d_srcs, k, d_depth, stats);
+    });
+}
+
+int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int32_t k, float *d_dist,
+                      apspie_stats *stats) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        validate_csr(g, false);
+        APSPIE_CHECK(g->m == g->n, APSPIE_EINVAL, "traversal needs a square adjacency matrix");
+        APSPIE_CHECK(k >= 1 && k <= 64, APSPIE_ELIMIT, "batched SSSP takes 1..64 sources");
+        APSPIE_CHECK(d_srcs && d_dist, APSPIE_EINVAL, "null sources or output");
+        trav::check_sources(ctx, g, d_srcs, k);
+        trav::sssp_batch(ctx, g, d_srcs, k, d_dist, stats);
+    });
+}
+
+}  // extern "C"

@@ -52,8 +52,11 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
 //   void on_round(uint64_t ball, int v0, int lane)   ball: picked lanes of
 //                                                    vertices v0 .. v0+63
 //   void on_new(int v)                        for every picked vertex
+// A negative level_arg means "device-driven": the round number is read from
+// C->cur_level and the compaction advances it when it completes, so a
+// captured sequence of launches can be replayed unchanged (hipGraph).
 template <class Pick>
-__global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int nblk, int always,
+__global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, int nblk, int always,
                                                  const int32_t *__restrict__ row_ptr, Queue q,
                                                  uint64_t *status, Control *C) {
     constexpr int NW = WGC / WAVE, SEGV = CHUNK / NW, ROUNDS = SEGV / WAVE;
@@ -63,6 +66,9 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int 
     __shared__ uint32_t s_new[NW];
     __shared__ uint64_t s_prefix;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    // every block reads the round before it takes a dynamic id, so the last
+    // id's block may advance cur_level once it has finished
+    const int level = level_arg >= 0 ? level_arg : *(volatile int *)&C->cur_level;
     const int par = level & 1;
     Ctl *next = &C->ctl[par];
     if (!always && C->ctl[par ^ 1].nF == 0) {
@@ -172,6 +178,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int 
             c.pad = 0;
             *next = c;
             atomicAdd(&C->acc.edges, (unsigned long long)c.E);
+            if (level_arg < 0) C->cur_level = level + 1;
         }
     }
 }
@@ -186,8 +193,59 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level, int m, int 
 //   static constexpr bool kVert                    needs the entry's vertex id
 //   static constexpr bool kVal                     needs the entry's payload
 // Columns outside [0, m) are skipped.
+// Stages tile t of the queued entries' edges into LDS (entry start marks,
+// col_ind base, optional payload and vertex id) and gives this thread the
+// owner entry of its IPT edge slots (slot = wid*SEG + r*64 + lane) through an
+// inclusive max-scan.  Returns the number of valid slots in the tile.
+template <bool KVAL, bool KVERT>
+__device__ __forceinline__ int tile_owners(int t, const Ctl &cur, const Queue &q, int32_t *s_own, int32_t *s_base,
+                                           uint32_t *s_val, int32_t *s_vert, int32_t *s_wmax, int (&own)[IPT]) {
+    constexpr int NW = WG / WAVE, SEG = TILE / NW;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int e0 = t * TILE;
+    const int f0 = q.tiles[t];
+    const int f1 = (t + 1 < cur.nTiles) ? q.tiles[t + 1] : cur.nF - 1;
+    const int cnt = min(f1 - f0 + 1, TILE + 1);  // every queued entry has >= 1 edge
+    for (int i = tid; i < TILE; i += WG) s_own[i] = 0;
+    __syncthreads();
+    for (int k = tid; k < cnt; k += WG) {
+        const int o = q.off[f0 + k] - e0;
+        s_base[k] = q.start[f0 + k] - o;
+        if (KVAL) s_val[k] = q.val[f0 + k];
+        if (KVERT) s_vert[k] = q.vert[f0 + k];
+        if (o > 0 && o < TILE) s_own[o] = k;
+    }
+    __syncthreads();
+    int mx = 0;
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        own[r] = s_own[wid * SEG + r * WAVE + lane];
+        mx = max(mx, own[r]);
+    }
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
+    if (lane == 0) s_wmax[wid] = mx;
+    __syncthreads();
+    int carry = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        if (w < wid) carry = max(carry, s_wmax[w]);
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        int xo = max(own[r], carry);
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(xo, d, WAVE);
+            if (lane >= d) xo = max(xo, y);
+        }
+        own[r] = xo;
+        carry = __shfl(xo, WAVE - 1, WAVE);
+    }
+    return min(TILE, cur.E - e0);
+}
+
 template <class Op>
-__global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Control *C, Queue q,
+__global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, const Control *C, Queue q,
                                                const int32_t *__restrict__ col_ind) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     __shared__ int32_t s_own[TILE];
@@ -195,50 +253,12 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level, int m, const Co
     __shared__ uint32_t s_val[Op::kVal ? TILE + 1 : 1];
     __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
     __shared__ int32_t s_wmax[NW];
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;  // round of the next compaction
     const Ctl cur = C->ctl[(level - 1) & 1];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
-        const int e0 = t * TILE;
-        const int f0 = q.tiles[t];
-        const int f1 = (t + 1 < cur.nTiles) ? q.tiles[t + 1] : cur.nF - 1;
-        const int cnt = min(f1 - f0 + 1, TILE + 1);  // every queued entry has >= 1 edge
-        for (int i = tid; i < TILE; i += WG) s_own[i] = 0;
-        __syncthreads();
-        for (int k = tid; k < cnt; k += WG) {
-            const int o = q.off[f0 + k] - e0;
-            s_base[k] = q.start[f0 + k] - o;
-            if (Op::kVal) s_val[k] = q.val[f0 + k];
-            if (Op::kVert) s_vert[k] = q.vert[f0 + k];
-            if (o > 0 && o < TILE) s_own[o] = k;
-        }
-        __syncthreads();
         int own[IPT];
-        int mx = 0;
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-            own[r] = s_own[wid * SEG + r * WAVE + lane];
-            mx = max(mx, own[r]);
-        }
-#pragma unroll
-        for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
-        if (lane == 0) s_wmax[wid] = mx;
-        __syncthreads();
-        int carry = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w)
-            if (w < wid) carry = max(carry, s_wmax[w]);
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-            int xo = max(own[r], carry);
-#pragma unroll
-            for (int d = 1; d < WAVE; d <<= 1) {
-                const int y = __shfl_up(xo, d, WAVE);
-                if (lane >= d) xo = max(xo, y);
-            }
-            own[r] = xo;
-            carry = __shfl(xo, WAVE - 1, WAVE);
-        }
-        const int elim = min(TILE, cur.E - e0);
+        const int elim = tile_owners<Op::kVal, Op::kVert>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
         int pos[IPT], col[IPT];
 #pragma unroll
         for (int r = 0; r < IPT; ++r) {

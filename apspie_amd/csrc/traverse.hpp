@@ -40,6 +40,8 @@ struct Control {
     unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
     unsigned int nnew[2];         // vertices picked by the compaction, by round parity
     Acc acc;
+    int cur_level;  // device-driven rounds: the round the next compaction runs
+    int pad2;
 };
 
 struct Bufs {

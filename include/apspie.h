@@ -213,6 +213,20 @@ int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, aps
  * (cuSPARSE L*L^T, ewiseMultTc src/tc.cuh:168-199, float mgpu::Reduce). */
 int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, apspie_stats *stats);
 
+/* ---- batched path ----------------------------------------------------------
+ * No counterpart in the reference (it runs one source per call); each result
+ * row equals the plain entry point's output for that source. */
+
+/* BFS from k <= 256 sources at once (device array d_srcs[k]); d_depth[k*m]
+ * holds source s's depths in row s (-1 unreached). */
+int apspie_bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int32_t k, int32_t *d_depth,
+                     apspie_stats *stats);
+
+/* SSSP from k <= 64 sources at once; d_dist[k*m] holds source s's distances
+ * in row s (APSPIE_SSSP_INF unreachable). */
+int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int32_t k, float *d_dist,
+                      apspie_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif
