@@ -110,6 +110,13 @@ def lib() -> C.CDLL:
         "apspie_tc": [vp, P(_Csr), P(i64), P(Stats)],
         "apspie_bfs_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
         "apspie_sssp_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
+        "apspie_server_create": [C.c_int, P(_Csr), C.c_int, vp, P(vp)],
+        "apspie_server_submit": [vp, C.c_int, C.c_int, i32, vp, C.c_double, P(C.c_uint64)],
+        "apspie_server_wait": [vp, C.c_uint64, P(C.c_double)],
+        "apspie_server_drain": [vp],
+        "apspie_server_stats": [vp, vp],
+        "apspie_server_reset_stats": [vp],
+        "apspie_server_destroy": [vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -119,6 +126,10 @@ def lib() -> C.CDLL:
     L.apspie_last_error.restype = C.c_char_p
     L.apspie_ctx_stream.restype = vp
     L.apspie_host_csr_free.restype = None
+    L.apspie_now.argtypes = []
+    L.apspie_now.restype = C.c_double
+    L.apspie_server_last_error.argtypes = [vp]
+    L.apspie_server_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -448,6 +459,94 @@ class Context:
         dx.free()
         dy.free()
         return y
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("lanes", C.c_int),
+        ("max_bfs_batch", C.c_int),
+        ("max_sssp_batch", C.c_int),
+        ("reserved", C.c_int),
+        ("budget_bytes", C.c_size_t),
+    ]
+
+
+class ServerReport(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("rejected", C.c_int64),
+        ("batches", C.c_int64),
+        ("mean_batch", C.c_double),
+        ("p50_ms", C.c_double),
+        ("p99_ms", C.c_double),
+        ("mean_ms", C.c_double),
+        ("max_ms", C.c_double),
+        ("qps", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_BFS, REQ_SSSP = 0, 1
+
+
+def now() -> float:
+    """The serving loop's steady clock, in seconds."""
+    return lib().apspie_now()
+
+
+class Server:
+    """apspie_server: batched BFS/SSSP service over device-resident graphs."""
+
+    def __init__(self, graphs: Sequence[DeviceCsr], lanes: int = 4, budget_bytes: int = 32 << 30,
+                 max_bfs_batch: int = 256, max_sssp_batch: int = 64, device: int = 0):
+        L = lib()
+        self._graphs = list(graphs)  # keep the device CSRs alive
+        arr = (_Csr * len(graphs))(*[g.c for g in graphs])
+        opts = ServerOpts(lanes, max_bfs_batch, max_sssp_batch, 0, budget_bytes)
+        h = C.c_void_p()
+        _check(L.apspie_server_create(device, arr, len(graphs), C.byref(opts), C.byref(h)), None, "server_create")
+        self.h = h.value
+
+    def submit(self, kind: int, graph: int, src: int, out: Optional[np.ndarray] = None,
+               arrival: float = -1.0) -> int:
+        t = C.c_uint64()
+        ptr = out.ctypes.data if out is not None else None
+        rc = lib().apspie_server_submit(self.h, kind, graph, src, ptr, arrival, C.byref(t))
+        if rc != 0:
+            raise ApspieError(rc, "server_submit")
+        return t.value
+
+    def wait(self, ticket: int) -> float:
+        lat = C.c_double()
+        rc = lib().apspie_server_wait(self.h, ticket, C.byref(lat))
+        if rc != 0:
+            raise ApspieError(rc, lib().apspie_server_last_error(self.h).decode())
+        return lat.value
+
+    def drain(self) -> None:
+        lib().apspie_server_drain(self.h)
+
+    def report(self) -> dict:
+        r = ServerReport()
+        lib().apspie_server_stats(self.h, C.byref(r))
+        return r.as_dict()
+
+    def reset_stats(self) -> None:
+        lib().apspie_server_reset_stats(self.h)
+
+    def close(self) -> None:
+        if self.h:
+            lib().apspie_server_destroy(self.h)
+        self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def exported_symbols(header: str) -> Sequence[str]:

@@ -1,0 +1,300 @@
+// serve.cpp -- serving loop over a device-resident graph pool.
+//
+// Requests (BFS or SSSP from one source on one pooled graph) enter a FIFO.
+// `lanes` worker threads each own a context (HIP stream + scratch) and loop:
+// take the oldest pending request, gather every other pending request for the
+// same (graph, kind) up to the batch limit while the device-memory budget
+// admits them, run one batched traversal (apspie_bfs_batch /
+// apspie_sssp_batch), copy each requested row back and complete the tickets.
+// Requests that arrive while batches run are picked up by the next lane that
+// frees, and batches of different lanes overlap on the device (one stream
+// each), so the device is kept busy without waiting for a full batch.
+//
+// Latency = completion time - arrival time on one steady clock.  A request
+// whose footprint exceeds the whole budget is refused at submission.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace apspie;
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id;
+    int kind, graph;
+    int32_t src;
+    void *out;
+    double arrive, done_at = 0;
+    size_t footprint;
+    bool done = false;
+    int status = 0;
+};
+
+}  // namespace
+
+struct apspie_server {
+    apspie_server_opts opts{};
+    std::vector<apspie_csr> graphs;
+    int device = 0;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> pending;
+    std::map<uint64_t, std::unique_ptr<Request>> all;
+    uint64_t next_id = 1;
+    size_t used = 0;
+    bool stop = false;
+    int inflight = 0;
+    std::vector<std::thread> workers;
+    std::vector<double> lat_ms;
+    double first_arrive = 0, last_done = 0;
+    int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_reqs = 0;
+    std::string err;
+
+    size_t footprint(int kind, int g) const {
+        const size_t m = size_t(graphs[g].m);
+        // output row + share of the batch state (vertex-major masks or
+        // distances) + the host-visible copy staging
+        return kind == 0 ? m * 5 + 4096 : m * 13 + 4096;
+    }
+    void worker(int lane);
+};
+
+void apspie_server::worker(int lane) {
+    apspie_ctx *ctx = nullptr;
+    if (apspie_ctx_create(device, &ctx) != APSPIE_OK) {
+        std::lock_guard<std::mutex> lk(mu);
+        err = "lane " + std::to_string(lane) + ": context creation failed";
+        return;
+    }
+    void *d_srcs = nullptr, *d_out = nullptr;
+    size_t out_cap = 0;
+    apspie_malloc(ctx, &d_srcs, 256 * 4);
+    std::vector<Request *> batch;
+    std::vector<int32_t> srcs;
+    for (;;) {
+        batch.clear();
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+            if (stop && pending.empty()) break;
+            // oldest request decides (graph, kind); gather the same kind in
+            // arrival order while budget and batch limit allow
+            Request *head = pending.front();
+            const int limit = head->kind == 0 ? opts.max_bfs_batch : opts.max_sssp_batch;
+            for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < limit;) {
+                Request *r = *it;
+                if (r->kind == head->kind && r->graph == head->graph) {
+                    if (used + r->footprint > opts.budget_bytes) break;  // admit no more now
+                    used += r->footprint;
+                    batch.push_back(r);
+                    it = pending.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            if (batch.empty()) {
+                // budget full: wait for another lane to release memory
+                cv_work.wait(lk, [&] { return stop || inflight == 0 || used == 0; });
+                continue;
+            }
+            inflight++;
+        }
+        const int kind = batch[0]->kind, gi = batch[0]->graph;
+        const apspie_csr &g = graphs[gi];
+        const int k = (int)batch.size();
+        srcs.resize(k);
+        for (int i = 0; i < k; ++i) srcs[i] = batch[i]->src;
+        int rc = apspie_memcpy_h2d(ctx, d_srcs, srcs.data(), size_t(k) * 4);
+        const size_t need = size_t(k) * size_t(g.m) * 4;
+        if (rc == APSPIE_OK && need > out_cap) {
+            if (d_out) apspie_free(ctx, d_out);
+            d_out = nullptr;
+            out_cap = 0;
+            rc = apspie_malloc(ctx, &d_out, need);
+            if (rc == APSPIE_OK) out_cap = need;
+        }
+        if (rc == APSPIE_OK) {
+            apspie_stats st;
+            rc = kind == 0 ? apspie_bfs_batch(ctx, &g, (const int32_t *)d_srcs, k, (int32_t *)d_out, &st)
+                           : apspie_sssp_batch(ctx, &g, (const int32_t *)d_srcs, k, (float *)d_out, &st);
+        }
+        for (int i = 0; i < k && rc == APSPIE_OK; ++i)
+            if (batch[i]->out)
+                rc = apspie_memcpy_d2h(ctx, batch[i]->out, (const char *)d_out + size_t(i) * g.m * 4,
+                                       size_t(g.m) * 4);
+        const double t = now_s();
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (Request *r : batch) {
+                r->done = true;
+                r->done_at = t;
+                r->status = rc;
+                used -= r->footprint;
+                if (rc == APSPIE_OK) {
+                    completed++;
+                    lat_ms.push_back((t - r->arrive) * 1e3);
+                    last_done = std::max(last_done, t);
+                }
+            }
+            if (rc != APSPIE_OK) err = apspie_last_error(ctx);
+            batches++;
+            batched_reqs += k;
+            inflight--;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+    if (d_out) apspie_free(ctx, d_out);
+    if (d_srcs) apspie_free(ctx, d_srcs);
+    apspie_ctx_destroy(ctx);
+}
+
+extern "C" {
+
+double apspie_now(void) { return now_s(); }
+
+int apspie_server_create(int device, const apspie_csr *graphs, int ngraphs, const apspie_server_opts *opts,
+                         apspie_server **out) {
+    if (!out || !graphs || ngraphs < 1) return APSPIE_EINVAL;
+    *out = nullptr;
+    auto *s = new (std::nothrow) apspie_server();
+    if (!s) return APSPIE_ENOMEM;
+    s->device = device;
+    apspie_server_opts o{};
+    o.lanes = 4;
+    o.max_bfs_batch = 256;
+    o.max_sssp_batch = 64;
+    o.budget_bytes = size_t(32) << 30;
+    if (opts) {
+        if (opts->lanes > 0) o.lanes = opts->lanes;
+        if (opts->max_bfs_batch > 0) o.max_bfs_batch = std::min(opts->max_bfs_batch, 256);
+        if (opts->max_sssp_batch > 0) o.max_sssp_batch = std::min(opts->max_sssp_batch, 64);
+        if (opts->budget_bytes > 0) o.budget_bytes = opts->budget_bytes;
+    }
+    s->opts = o;
+    for (int i = 0; i < ngraphs; ++i) {
+        if (graphs[i].m != graphs[i].n || graphs[i].m <= 0) {
+            delete s;
+            return APSPIE_EINVAL;
+        }
+        s->graphs.push_back(graphs[i]);
+    }
+    for (int l = 0; l < o.lanes; ++l) s->workers.emplace_back([s, l] { s->worker(l); });
+    *out = s;
+    return APSPIE_OK;
+}
+
+int apspie_server_submit(apspie_server *s, int kind, int graph, int32_t src, void *host_out, double arrival,
+                         uint64_t *ticket) {
+    if (!s || !ticket || (kind != 0 && kind != 1) || graph < 0 || graph >= (int)s->graphs.size()) return APSPIE_EINVAL;
+    if (src < 0 || src >= s->graphs[graph].m) return APSPIE_EINVAL;
+    auto r = std::make_unique<Request>();
+    r->kind = kind;
+    r->graph = graph;
+    r->src = src;
+    r->out = host_out;
+    r->arrive = arrival >= 0 ? arrival : now_s();
+    r->footprint = s->footprint(kind, graph);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (r->footprint > s->opts.budget_bytes) {
+            s->rejected++;
+            return APSPIE_EBUSY;
+        }
+        r->id = s->next_id++;
+        *ticket = r->id;
+        if (s->submitted == 0 || r->arrive < s->first_arrive) s->first_arrive = r->arrive;
+        s->submitted++;
+        s->pending.push_back(r.get());
+        s->all[r->id] = std::move(r);
+    }
+    s->cv_work.notify_one();
+    return APSPIE_OK;
+}
+
+int apspie_server_wait(apspie_server *s, uint64_t ticket, double *latency_ms) {
+    if (!s) return APSPIE_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return APSPIE_EINVAL;
+    Request *r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->done; });
+    if (latency_ms) *latency_ms = (r->done_at - r->arrive) * 1e3;
+    const int rc = r->status;
+    s->all.erase(it);
+    return rc;
+}
+
+int apspie_server_drain(apspie_server *s) {
+    if (!s) return APSPIE_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->pending.empty() && s->inflight == 0; });
+    for (auto it = s->all.begin(); it != s->all.end();)
+        it = it->second->done ? s->all.erase(it) : std::next(it);
+    return APSPIE_OK;
+}
+
+int apspie_server_stats(apspie_server *s, apspie_server_report *st) {
+    if (!s || !st) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    memset(st, 0, sizeof(*st));
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->rejected = s->rejected;
+    st->batches = s->batches;
+    st->mean_batch = s->batches ? double(s->batched_reqs) / double(s->batches) : 0.0;
+    if (!s->lat_ms.empty()) {
+        std::vector<double> v = s->lat_ms;
+        std::sort(v.begin(), v.end());
+        auto q = [&](double p) { return v[std::min(v.size() - 1, size_t(p * double(v.size() - 1) + 0.5))]; };
+        st->p50_ms = q(0.50);
+        st->p99_ms = q(0.99);
+        st->max_ms = v.back();
+        double sum = 0;
+        for (double x : v) sum += x;
+        st->mean_ms = sum / double(v.size());
+        const double span = s->last_done - s->first_arrive;
+        st->qps = span > 0 ? double(s->completed) / span : 0.0;
+    }
+    return APSPIE_OK;
+}
+
+int apspie_server_reset_stats(apspie_server *s) {
+    if (!s) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->lat_ms.clear();
+    s->submitted = s->completed = s->rejected = s->batches = s->batched_reqs = 0;
+    s->first_arrive = s->last_done = 0;
+    return APSPIE_OK;
+}
+
+const char *apspie_server_last_error(apspie_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int apspie_server_destroy(apspie_server *s) {
+    if (!s) return APSPIE_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+    return APSPIE_OK;
+}
+
+}  // extern "C"

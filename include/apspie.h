@@ -227,6 +227,47 @@ int apspie_bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs
 int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int32_t k, float *d_dist,
                       apspie_stats *stats);
 
+/* ---- serving ---------------------------------------------------------------
+ * No counterpart in the reference.  A pool of device-resident graphs
+ * (borrowed: they must outlive the server); BFS / SSSP requests are queued and
+ * served in batches by `lanes` worker threads, each with its own stream, so
+ * batches overlap on the device and requests that arrive meanwhile join the
+ * next batch a lane forms.  Admission is by device-memory footprint against
+ * budget_bytes.  All entry points are thread-safe. */
+typedef struct apspie_server apspie_server;
+typedef struct {
+    int lanes;            /* concurrent batches (default 4) */
+    int max_bfs_batch;    /* <= 256 (default 256) */
+    int max_sssp_batch;   /* <= 64 (default 64) */
+    int reserved;
+    size_t budget_bytes;  /* device memory for in-flight requests (default 32 GiB) */
+} apspie_server_opts;
+typedef struct {
+    int64_t submitted, completed, rejected, batches;
+    double mean_batch;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double qps; /* completed / (last completion - first arrival) */
+} apspie_server_report;
+
+#define APSPIE_REQ_BFS 0
+#define APSPIE_REQ_SSSP 1
+/* Seconds on the server's steady clock (for arrival times). */
+double apspie_now(void);
+int apspie_server_create(int device, const apspie_csr *graphs, int ngraphs, const apspie_server_opts *opts,
+                         apspie_server **out);
+/* Queue one request.  host_out: m int32 depths (BFS) or float distances
+ * (SSSP), or NULL to discard.  arrival < 0 means now.  APSPIE_EBUSY when the
+ * request's footprint exceeds the whole budget. */
+int apspie_server_submit(apspie_server *s, int kind, int graph, int32_t src, void *host_out, double arrival,
+                         uint64_t *ticket);
+/* Block until the ticket completes; returns its status. */
+int apspie_server_wait(apspie_server *s, uint64_t ticket, double *latency_ms);
+int apspie_server_drain(apspie_server *s);
+int apspie_server_stats(apspie_server *s, apspie_server_report *st);
+int apspie_server_reset_stats(apspie_server *s);
+const char *apspie_server_last_error(apspie_server *s);
+int apspie_server_destroy(apspie_server *s);
+
 #ifdef __cplusplus
 }
 #endif

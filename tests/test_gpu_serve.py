@@ -1,0 +1,65 @@
+"""GPU tests of the serving loop: every served result equals a solo plain
+launch of the same request (alone and in a mixed stream), admission by
+memory footprint refuses what can never fit and queues what must wait, and
+the report counts what it served."""
+import numpy as np
+import pytest
+
+import apspie_amd as A
+from apspie_amd import serving
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pool(ctx):
+    hosts = serving.build_pool([10, 11, 12, 13])
+    return hosts, [ctx.upload(h) for h in hosts]
+
+
+def solo(ctx, graphs, r):
+    g = graphs[r.graph]
+    return ctx.bfs(g, r.src)[0] if r.kind == A.REQ_BFS else ctx.sssp(g, r.src)[0]
+
+
+def test_served_equals_solo_in_a_mix(ctx, pool):
+    hosts, graphs = pool
+    reqs = serving.make_stream([h.m for h in hosts], n=600, rate=50000.0, seed=7)
+    outs = {i: np.empty(hosts[r.graph].m, np.int32 if r.kind == A.REQ_BFS else np.float32)
+            for i, r in enumerate(reqs)}
+    with A.Server(graphs, lanes=3) as srv:
+        rep = serving.run_served(srv, reqs, open_loop=True, outs=outs)
+    assert rep["completed"] == len(reqs) and rep["rejected"] == 0
+    assert rep["batches"] < len(reqs)  # requests were batched together
+    assert rep["p50_ms"] <= rep["p99_ms"]
+    for i, r in enumerate(reqs):
+        assert np.array_equal(outs[i], solo(ctx, graphs, r)), i
+
+
+def test_alone_equals_mix(ctx, pool):
+    hosts, graphs = pool
+    r = serving.Req(A.REQ_SSSP, 3, 5, 0.0)
+    alone = np.empty(hosts[3].m, np.float32)
+    with A.Server(graphs, lanes=1) as srv:
+        srv.wait(srv.submit(r.kind, r.graph, r.src, alone))
+    assert np.array_equal(alone, solo(ctx, graphs, r))
+
+
+def test_admission_by_footprint(ctx, pool):
+    hosts, graphs = pool
+    m3 = hosts[3].m
+    # room for about two SSSP requests on graph 3 at a time
+    with A.Server(graphs, lanes=4, budget_bytes=2 * (13 * m3 + 4096) + 100) as srv:
+        outs = [np.empty(m3, np.float32) for _ in range(12)]
+        ts = [srv.submit(A.REQ_SSSP, 3, 7 * i, outs[i]) for i in range(12)]
+        for t in ts:
+            srv.wait(t)
+        rep = srv.report()
+        assert rep["completed"] == 12 and rep["batches"] >= 6
+        for i in range(12):
+            assert np.array_equal(outs[i], ctx.sssp(graphs[3], 7 * i)[0])
+    with A.Server(graphs, lanes=1, budget_bytes=1000) as srv:
+        with pytest.raises(A.ApspieError) as e:
+            srv.submit(A.REQ_BFS, 0, 0)
+        assert e.value.code == -7
+        assert srv.report()["rejected"] == 1
