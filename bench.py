@@ -80,6 +80,102 @@ def kernel_roofline(ctx, fn, steps, bytes_per_step_expand):
     }
 
 
+def batch_sources(h, k, seed):
+    """k sources drawn with mt19937 from vertices with degree > 0."""
+    from apspie_amd.serving import mt19937_stream
+
+    cand = np.nonzero(h.degrees() > 0)[0]
+    d = mt19937_stream(seed, k)
+    return cand[(d * np.uint64(len(cand))) >> np.uint64(32)].astype(np.int32)
+
+
+def bench_batched(ctx):
+    """256-source BFS and 64-source SSSP on RMAT s20 as one batched call each,
+    against the same sources through plain one-at-a-time launches."""
+    res = {}
+    for kind, k, und, seed in (("bfs", 256, True, 4020), ("sssp", 64, False, 4120)):
+        h = A.rmat(20, 16, seed, undirected=und, weight_seed=None if und else seed + 1_000_000)
+        g = ctx.upload(h)
+        srcs = batch_sources(h, k, seed)
+        dsrc = ctx.alloc(4 * k).upload(srcs)
+        dout = ctx.alloc(4 * k * h.m)
+        one = ctx.alloc(4 * h.m)
+        lib = A.lib()
+        import ctypes as C
+
+        def batched():
+            st = A.Stats()
+            fn = lib.apspie_bfs_batch if kind == "bfs" else lib.apspie_sssp_batch
+            rc = fn(ctx.h, C.byref(g.c), dsrc.ptr, k, dout.ptr, C.byref(st))
+            assert rc == 0
+
+        def plain():
+            for s in srcs:
+                (ctx.bfs_into if kind == "bfs" else ctx.sssp_into)(g, int(s), one)
+
+        batched()
+        plain()
+        ctx.synchronize()
+        tb = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            batched()
+            tb.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        plain()
+        tp = time.perf_counter() - t0
+        # spot check: first and last rows equal the plain results
+        rows = dout.download(np.int32 if kind == "bfs" else np.float32, k * h.m).reshape(k, h.m)
+        ok = True
+        for s in (0, k - 1):
+            (ctx.bfs_into if kind == "bfs" else ctx.sssp_into)(g, int(srcs[s]), one)
+            ok &= bool(np.array_equal(rows[s], one.download(rows.dtype, h.m)))
+        b = min(tb)
+        res[kind] = {
+            "workload": f"{kind}_batch_rmat_s20_{'undirected' if und else 'directed_w1_64'}_k{k}",
+            "k": k,
+            "batched_ms": round(b * 1e3, 3),
+            "plain_ms": round(tp * 1e3, 3),
+            "batched_qps": round(k / b, 1),
+            "plain_qps": round(k / tp, 1),
+            "speedup": round(tp / b, 2),
+            "rows_match_plain": ok,
+        }
+        log(f"batched {kind}: {res[kind]}")
+        for x in (dsrc, dout, one):
+            x.free()
+        del g
+    return res
+
+
+def bench_served(ctx, n):
+    """Served stream (16-graph pool, Poisson 2000 req/s, 50/50 BFS/SSSP)
+    against plain one-at-a-time launches of the same requests."""
+    from apspie_amd import serving
+
+    t = time.time()
+    hosts = serving.build_pool()
+    graphs = [ctx.upload(h) for h in hosts]
+    ms = [h.m for h in hosts]
+    del hosts
+    log(f"serving pool of {len(graphs)} graphs ready ({time.time() - t:.1f}s)")
+    reqs = serving.make_stream(ms, n=n, rate=2000.0)
+    res = {"pool_scales": serving.POOL_SCALES, "requests": n, "rate_per_s": 2000.0}
+    with A.Server(graphs, lanes=4) as srv:
+        serving.run_served(srv, reqs[:500], open_loop=False)  # warm-up: contexts, buffers
+        res["served_open_loop"] = serving.run_served(srv, reqs, open_loop=True)
+        log(f"served open-loop: {res['served_open_loop']}")
+        res["served_saturation"] = serving.run_served(srv, reqs, open_loop=False)
+        log(f"served saturation: {res['served_saturation']}")
+    npl = min(n, 2000)
+    res["plain_open_loop"] = serving.run_plain(ctx, graphs, reqs[:npl], open_loop=True)
+    res["plain_saturation"] = serving.run_plain(ctx, graphs, reqs[:npl], open_loop=False)
+    res["plain_requests"] = npl
+    log(f"plain: {res['plain_open_loop']} {res['plain_saturation']}")
+    res["saturation_qps_ratio"] = round(res["served_saturation"]["qps"] / res["plain_saturation"]["qps"], 2)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +184,9 @@ def main():
     ap.add_argument("--no-bfs", action="store_true", help="skip the BFS s22 side measurement")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--scale", type=int, default=SSSP_SCALE)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS/SSSP comparison")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
+    ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -202,6 +301,11 @@ def main():
             "roofline": roof_b,
         }
         log(f"bfs: {ms_b:.3f} ms/step, {out['bfs']['gteps_expanded']} GTEPS (expanded)")
+
+    if not args.no_batch:
+        out["batched"] = bench_batched(ctx)
+    if not args.no_serve:
+        out["served"] = bench_served(ctx, args.serve_requests)
 
     if not args.no_cpu:
         import oracle as O
