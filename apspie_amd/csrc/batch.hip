@@ -21,6 +21,7 @@
 // union frontier is empty.
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "frontier.hpp"
@@ -72,6 +73,7 @@ struct PickMs {
 template <int W>
 struct OpMs {
     static constexpr bool kVert = true, kVal = false;
+    static constexpr int kBatch = W >= 2 ? 2 : IPT;
     struct Pre {
         uint64_t f[W], v[W];
     };
@@ -135,15 +137,18 @@ struct PickChanged {
     __device__ void on_new(int) const {}
 };
 
-// Expansion with lanes = sources: each wave takes the 64 edge slots of one
-// round (owners from the shared tile staging), broadcasts them one at a time
-// and relaxes all active sources of the edge with coalesced row accesses.
+// Expansion with lanes = (edge, source): a wave step covers E = 64 / K edge
+// slots of one round (owners from the shared tile staging) with K lanes per
+// edge, one per source, so each edge's relaxations read and update one
+// coalesced K-float row; the ballot of improved lanes becomes the edge
+// target's changed-source mask.
 template <int K>
 __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, Queue q,
                                                        const int32_t *__restrict__ col_ind,
                                                        const float *__restrict__ val, uint32_t *dist,
                                                        const uint64_t *__restrict__ fr, uint64_t *chg) {
-    constexpr int NW = WG / WAVE, SEG = TILE / NW, U = 4;
+    constexpr int NW = WG / WAVE, SEG = TILE / NW, E = WAVE / K, U = 4;
+    static_assert(WAVE % K == 0, "K divides the wave");
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];
     __shared__ uint32_t s_val[1];
@@ -152,6 +157,7 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
     const int level = C->cur_level;
     const Ctl cur = C->ctl[(level - 1) & 1];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int sub = lane / K, src = lane % K;
     for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
         int own[IPT];
         const int elim = tile_owners<false, true>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
@@ -171,25 +177,26 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
             }
             const uint32_t fu_lo = uint32_t(fu), fu_hi = uint32_t(fu >> 32);
 #pragma unroll 1
-            for (int j0 = 0; j0 < WAVE; j0 += U) {
+            for (int j0 = 0; j0 < WAVE; j0 += E * U) {
                 int cj[U], uj[U];
                 float wj[U];
                 bool act[U];
                 uint32_t du[U], dc[U];
 #pragma unroll
                 for (int x = 0; x < U; ++x) {
-                    cj[x] = __shfl(c, j0 + x, WAVE);
-                    uj[x] = __shfl(u, j0 + x, WAVE);
-                    wj[x] = __shfl(w, j0 + x, WAVE);
-                    const uint32_t lo = __shfl(fu_lo, j0 + x, WAVE), hi = __shfl(fu_hi, j0 + x, WAVE);
+                    const int j = j0 + x * E + sub;
+                    cj[x] = __shfl(c, j, WAVE);
+                    uj[x] = __shfl(u, j, WAVE);
+                    wj[x] = __shfl(w, j, WAVE);
+                    const uint32_t lo = __shfl(fu_lo, j, WAVE), hi = __shfl(fu_hi, j, WAVE);
                     const uint64_t f = (uint64_t(hi) << 32) | lo;
-                    act[x] = cj[x] >= 0 && lane < K && ((f >> lane) & 1ull);
+                    act[x] = cj[x] >= 0 && ((f >> src) & 1ull);
                 }
 #pragma unroll
                 for (int x = 0; x < U; ++x)
                     if (act[x]) {
-                        du[x] = dist[int64_t(uj[x]) * K + lane];
-                        dc[x] = dist[int64_t(cj[x]) * K + lane];
+                        du[x] = dist[int64_t(uj[x]) * K + src];
+                        dc[x] = dist[int64_t(cj[x]) * K + src];
                     }
 #pragma unroll
                 for (int x = 0; x < U; ++x) {
@@ -197,13 +204,16 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
                     if (act[x]) {
                         const uint32_t nb = __float_as_uint(__uint_as_float(du[x]) + wj[x]);
                         if (nb < dc[x]) {
-                            __hip_atomic_fetch_min(&dist[int64_t(cj[x]) * K + lane], nb, __ATOMIC_RELAXED,
+                            __hip_atomic_fetch_min(&dist[int64_t(cj[x]) * K + src], nb, __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT);
                             imp = true;
                         }
                     }
                     const uint64_t b = __ballot(imp);
-                    if (b && lane == 0) atomicOr((unsigned long long *)&chg[cj[x]], (unsigned long long)b);
+                    if (b && src == 0) {
+                        const uint64_t mine = K == 64 ? b : (b >> (sub * K)) & ((1ull << (K % 64)) - 1);
+                        if (mine) atomicOr((unsigned long long *)&chg[cj[x]], (unsigned long long)mine);
+                    }
                 }
             }
         }
@@ -252,41 +262,55 @@ __global__ __launch_bounds__(256) void k_mss_transpose(int m, int k, const uint3
 // ---------------------------------------------------------------------------
 // driver: device-driven rounds captured in chunks into a hipGraph
 
-struct Replay {
-    hipGraphExec_t exec = nullptr;
-    ~Replay() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-    }
-};
-
+// Replays a captured chunk of CH rounds until the union frontier is empty.
+// The instantiated graph is cached in the context under `key`, which names
+// every address and size its kernel arguments hold (slot growth drops the
+// cache), so repeated calls on one graph skip capture and instantiation.
 template <class F>
-int run_rounds(apspie_ctx *ctx, const Bufs &b, int max_rounds, F &&enqueue_round) {
-    // round 0 (seeding compaction) is enqueued by the caller; capture CH rounds
+int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &key, int max_rounds, F &&enqueue_round) {
     constexpr int CH = 4;
-    hipGraph_t graph = nullptr;
-    Replay rp;
-    APSPIE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < CH; ++i) enqueue_round();
-    APSPIE_HIP(hipStreamEndCapture(ctx->stream, &graph));
-    const hipError_t ie = hipGraphInstantiate(&rp.exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    APSPIE_HIP(ie);
-    volatile int32_t *host_nf = (volatile int32_t *)ctx->pinned;
+    hipGraphExec_t exec = nullptr;
+    auto it = ctx->graphs.find(key);
+    if (it != ctx->graphs.end()) {
+        exec = it->second;
+    } else {
+        if (ctx->graphs.size() >= 64) ctx->drop_graphs();  // bound the cache
+        hipGraph_t graph = nullptr;
+        APSPIE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < CH; ++i) enqueue_round();
+        APSPIE_HIP(hipStreamEndCapture(ctx->stream, &graph));
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        APSPIE_HIP(ie);
+        ctx->graphs[key] = exec;
+    }
+    Control *hc = (Control *)ctx->pinned;
     int rounds = 0;
     for (;;) {
-        APSPIE_HIP(hipGraphLaunch(rp.exec, ctx->stream));
+        APSPIE_HIP(hipGraphLaunch(exec, ctx->stream));
         rounds += CH;
-        // the frontier written by the last compaction of the chunk
-        APSPIE_HIP(hipMemcpyAsync((void *)(host_nf + 1), &b.C->cur_level, 4, hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipMemcpyAsync(hc, b.C, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-        const int lvl = host_nf[1];
-        APSPIE_HIP(hipMemcpyAsync((void *)host_nf, &b.C->ctl[(lvl - 1) & 1].nF, 4, hipMemcpyDeviceToHost,
-                                  ctx->stream));
-        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-        if (*host_nf == 0) break;
+        // frontier written by the last compaction that ran (an exhausted
+        // traversal stops advancing cur_level, see k_compact)
+        if (hc->ctl[(hc->cur_level - 1) & 1].nF == 0) break;
         APSPIE_CHECK(rounds <= max_rounds, APSPIE_ENOCONV, "batched traversal did not converge");
     }
     return rounds;
+}
+
+std::string graph_key(const char *what, const apspie_csr *g, const Bufs &b, std::initializer_list<const void *> ptrs,
+                      int extra) {
+    char buf[96];
+    std::string k(what);
+    snprintf(buf, sizeof(buf), "|%p|%p|%p|%d|%lld|%d|%d|%p|%p", (const void *)g->row_ptr, (const void *)g->col_ind,
+             (const void *)g->val, g->m, (long long)g->nnz, b.nblk, extra, (void *)b.C, (void *)b.queue);
+    k += buf;
+    for (const void *p : ptrs) {
+        snprintf(buf, sizeof(buf), "|%p", p);
+        k += buf;
+    }
+    return k;
 }
 
 template <int W>
@@ -311,7 +335,8 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
     k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
                                                            b.status, b.C);
     APSPIE_HIP(hipGetLastError());
-    run_rounds(ctx, b, g->m + 2, [&] {
+    const std::string key = graph_key("msbfs", g, b, {next, vis, fr, depth, b.status, b.tiles}, W * 1000 + k);
+    run_rounds(ctx, b, key, g->m + 2, [&] {
         k_expand<OpMs<W>><<<b.expand_grid, WG, 0, ctx->stream>>>(op, -1, g->m, b.C, queue_of(b), g->col_ind);
         compact();
     });
@@ -320,9 +345,9 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
     if (stats) stats->bytes_alg = 0.0;
 }
 
+template <int K>
 void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, float *out,
                 apspie_stats *stats) {
-    constexpr int K = 64;
     const Bufs b = get_bufs(ctx, g->m, g->nnz);
     uint32_t *dist = (uint32_t *)ctx->slot(S_AUX0, size_t(g->m) * K * 4);
     uint64_t *chg = (uint64_t *)ctx->slot(S_AUX1, size_t(g->m) * 8);
@@ -335,7 +360,8 @@ void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k
     k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
                                                              b.status, b.C);
     APSPIE_HIP(hipGetLastError());
-    run_rounds(ctx, b, g->m * 2 + 4, [&] {
+    const std::string key = graph_key("mssssp", g, b, {dist, chg, fr, b.status, b.tiles}, K);
+    run_rounds(ctx, b, key, g->m * 2 + 4, [&] {
         k_expand_ms_sssp<K><<<b.expand_grid, WG, 0, ctx->stream>>>(g->m, b.C, queue_of(b), g->col_ind, g->val,
                                                                     dist, fr, chg);
         k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
@@ -393,7 +419,8 @@ int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_src
         APSPIE_CHECK(k >= 1 && k <= 64, APSPIE_ELIMIT, "batched SSSP takes 1..64 sources");
         APSPIE_CHECK(d_srcs && d_dist, APSPIE_EINVAL, "null sources or output");
         trav::check_sources(ctx, g, d_srcs, k);
-        trav::sssp_batch(ctx, g, d_srcs, k, d_dist, stats);
+        if (k <= 16) trav::sssp_batch<16>(ctx, g, d_srcs, k, d_dist, stats);
+        else trav::sssp_batch<64>(ctx, g, d_srcs, k, d_dist, stats);
     });
 }
 

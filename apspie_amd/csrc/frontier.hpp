@@ -21,6 +21,8 @@
 // (unless told to always run), so the host can launch rounds speculatively.
 #pragma once
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "traverse.hpp"
 
@@ -244,6 +246,18 @@ __device__ __forceinline__ int tile_owners(int t, const Ctl &cur, const Queue &q
     return min(TILE, cur.E - e0);
 }
 
+// Slots whose loads an op issues before applying any (Op::kBatch when the
+// op declares it, else all IPT); ops with a large Pre lower it to bound
+// register use.
+template <class T, class = void>
+struct op_batch {
+    static constexpr int value = IPT;
+};
+template <class T>
+struct op_batch<T, std::void_t<decltype(T::kBatch)>> {
+    static constexpr int value = T::kBatch;
+};
+
 template <class Op>
 __global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, const Control *C, Queue q,
                                                const int32_t *__restrict__ col_ind) {
@@ -267,15 +281,22 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, cons
         }
 #pragma unroll
         for (int r = 0; r < IPT; ++r) col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
-        typename Op::Pre pre[IPT];
+        constexpr int B = op_batch<Op>::value;
+        static_assert(IPT % B == 0, "batch divides IPT");
 #pragma unroll
-        for (int r = 0; r < IPT; ++r)
-            if ((unsigned)col[r] < (unsigned)m)
-                pre[r] = op.load(col[r], pos[r], Op::kVal ? s_val[own[r]] : 0u, Op::kVert ? s_vert[own[r]] : 0);
+        for (int r0 = 0; r0 < IPT; r0 += B) {
+            typename Op::Pre pre[B];
 #pragma unroll
-        for (int r = 0; r < IPT; ++r)
-            if ((unsigned)col[r] < (unsigned)m)
-                op.apply(col[r], pos[r], Op::kVal ? s_val[own[r]] : 0u, Op::kVert ? s_vert[own[r]] : 0, pre[r]);
+            for (int r = 0; r < B; ++r)
+                if ((unsigned)col[r0 + r] < (unsigned)m)
+                    pre[r] = op.load(col[r0 + r], pos[r0 + r], Op::kVal ? s_val[own[r0 + r]] : 0u,
+                                     Op::kVert ? s_vert[own[r0 + r]] : 0);
+#pragma unroll
+            for (int r = 0; r < B; ++r)
+                if ((unsigned)col[r0 + r] < (unsigned)m)
+                    op.apply(col[r0 + r], pos[r0 + r], Op::kVal ? s_val[own[r0 + r]] : 0u,
+                             Op::kVert ? s_vert[own[r0 + r]] : 0, pre[r]);
+        }
         __syncthreads();
     }
 }

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -79,6 +80,11 @@ struct apspie_ctx {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_open;  // (class, (start, stop))
     double prof_ms[APSPIE_KCLASS_COUNT] = {};
     int64_t prof_launches[APSPIE_KCLASS_COUNT] = {};
+
+    // Instantiated hipGraphs of captured round chunks, keyed by everything
+    // their kernel arguments depend on; dropped whenever a slot moves.
+    std::map<std::string, hipGraphExec_t> graphs;
+    void drop_graphs();
 
     // Device buffer of at least `bytes` bytes for slot `s`.  Growing drops
     // the old contents; callers initialise what they use.

@@ -6,11 +6,17 @@
 
 using namespace apspie;
 
+void apspie_ctx::drop_graphs() {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+}
+
 void *apspie_ctx::slot(int s, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (slot_bytes[s] >= bytes) return slot_ptr[s];
+    APSPIE_HIP(hipStreamSynchronize(stream));
+    drop_graphs();  // captured kernels may hold the old address
     if (slot_ptr[s]) {
-        APSPIE_HIP(hipStreamSynchronize(stream));
         APSPIE_HIP(hipFree(slot_ptr[s]));
         slot_ptr[s] = nullptr;
         slot_bytes[s] = 0;
@@ -105,6 +111,7 @@ int apspie_ctx_destroy(apspie_ctx *ctx) {
     if (!ctx) return APSPIE_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_graphs();
     for (int s = 0; s < S_NSLOTS; ++s)
         if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
