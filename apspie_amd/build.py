@@ -61,6 +61,26 @@ def build(verbose: bool = True) -> str:
     return LIB
 
 
+def build_ref_api_test(verbose: bool = True) -> str:
+    """Compile the C++ driver of the reference-named host API
+    (tests/cpp/test_ref_api.cpp); it links libapspie.so and the oracle."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_ref_api.cpp")
+    out_dir = os.path.join(root, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "test_ref_api")
+    cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-I" + os.path.join(root, "include"), src, "-o", out,
+           "-L" + HERE, "-lapspie", "-L" + os.path.join(root, "oracle", "_build"), "-loracle",
+           "-Wl,-rpath,$ORIGIN/../../../apspie_amd", "-Wl,-rpath,$ORIGIN/../../../oracle/_build"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 if __name__ == "__main__":
     build()
     sys.exit(0)

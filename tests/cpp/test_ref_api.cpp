@@ -1,0 +1,162 @@
+// Drives the reference-named C++ host API (apspie_amd/host/apspie_ref.hpp)
+// the way the reference's test drivers do (test/bfs, sssp, mis, spgemm, tc
+// main.cu) and checks every result against the CPU oracle.  Prints
+// "REF_API OK" and exits 0 when all checks pass.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../apspie_amd/host/apspie_ref.hpp"
+#include "../../oracle/oracle.h"
+
+static int failures = 0;
+#define EXPECT(cond, what)                                   \
+    do {                                                     \
+        if (!(cond)) {                                       \
+            printf("FAIL %s (%s:%d)\n", what, __FILE__, __LINE__); \
+            failures++;                                      \
+        }                                                    \
+    } while (0)
+
+template <class T>
+static T *dev_copy(const T *h, size_t n) {
+    T *d = nullptr;
+    if (hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    if (n) (void)hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice);
+    return d;
+}
+
+template <class T>
+static std::vector<T> host_copy(const T *d, size_t n) {
+    std::vector<T> h(n);
+    if (n) (void)hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost);
+    return h;
+}
+
+static orc_csr as_orc(const apspie_host_csr &h) {
+    orc_csr o;
+    o.m = h.m;
+    o.n = h.n;
+    o.nnz = h.nnz;
+    o.row_ptr = h.row_ptr;
+    o.col_ind = h.col_ind;
+    o.val = h.val;
+    return o;
+}
+
+int main(int argc, char **argv) {
+    const char *mtx = argc > 1 ? argv[1] : "tests/golden/mtx/chesapeake.mtx";
+    mgpu::ContextPtr context = mgpu::CreateCudaDevice(0);
+    try {
+        // BFS (test/bfs/main.cu): undirected chesapeake, every source
+        apspie_ref::HostGraph hg;
+        apspie_ref::readMtxGraph(mtx, true, &hg);
+        const apspie_host_csr &h = hg.h;
+        const int m = h.m, edge = int(h.nnz);
+        int *d_rp = dev_copy(h.row_ptr, size_t(m) + 1), *d_ci = dev_copy(h.col_ind, size_t(edge));
+        int *d_res = nullptr;
+        (void)hipMalloc(&d_res, size_t(m) * 4);
+        orc_csr o = as_orc(h);
+        std::vector<int> want(m);
+        for (int src = 0; src < m; ++src) {
+            bfs(src, edge, m, h.row_ptr, d_rp, d_ci, d_res, 1000, *context);
+            orc_bfs(&o, src, 1 << 30, want.data(), nullptr, nullptr);
+            EXPECT(host_copy(d_res, m) == want, "bfs");
+        }
+        // SSSP (test/sssp/main.cu) with integer weights
+        std::vector<float> w(edge);
+        for (int p = 0; p < edge; ++p) w[p] = float(1 + (p * 7919) % 64);
+        float *d_w = dev_copy(w.data(), size_t(edge)), *d_dist = nullptr;
+        (void)hipMalloc(&d_dist, size_t(m) * 4);
+        o.val = w.data();
+        std::vector<float> wd(m);
+        for (int src = 0; src < m; src += 5) {
+            sssp<float>(src, edge, m, d_w, d_rp, d_ci, d_dist, 1000, *context);
+            orc_sssp_dijkstra(&o, src, wd.data());
+            std::vector<float> got = host_copy(d_dist, m);
+            bool ok = true;
+            for (int v = 0; v < m; ++v) ok &= wd[v] == -1.0f ? got[v] == APSPIE_SSSP_INF : got[v] == wd[v];
+            EXPECT(ok, "sssp");
+        }
+        // MIS (test/mis/main.cu) with FP32 priorities, checked by spmspvCsr +
+        // the reference's XOR rule (src/util.cuh:277)
+        std::vector<float> prio(m);
+        for (int v = 0; v < m; ++v) prio[v] = float((v * 2654435761u) % 1000) / 1000.0f + 0.001f;
+        float *d_prio = dev_copy(prio.data(), size_t(m));
+        int *d_mis = nullptr, *d_has = nullptr;
+        (void)hipMalloc(&d_mis, size_t(m) * 4);
+        (void)hipMalloc(&d_has, size_t(m) * 4);
+        spmspvMis<float>(edge, m, h.row_ptr, d_rp, d_ci, d_prio, d_mis, 0.01f, *context);
+        spmspvCsr(d_mis, edge, m, d_rp, d_ci, d_has, *context);
+        std::vector<int> mis = host_copy(d_mis, m), has = host_copy(d_has, m);
+        std::vector<uint32_t> keys(m);
+        for (int v = 0; v < m; ++v) memcpy(&keys[v], &prio[v], 4);
+        std::vector<int> wm(m);
+        o.val = nullptr;
+        orc_mis_greedy(&o, keys.data(), wm.data());
+        EXPECT(mis == wm, "mis");
+        bool xr = true;
+        for (int v = 0; v < m; ++v) xr &= mis[v] + has[v] == 1;
+        EXPECT(xr, "mis xor check");
+        // SpGEMM (test/spgemm/main.cu): E = A * D^T with D = A^T built from
+        // swapped COO arrays, i.e. E = A * A
+        std::vector<int> cr, cc;
+        std::vector<float> cv;
+        for (int r = 0; r < m; ++r)
+            for (int p = h.row_ptr[r]; p < h.row_ptr[r + 1]; ++p) {
+                cr.push_back(r);
+                cc.push_back(h.col_ind[p]);
+                cv.push_back(0.25f + float(p % 5));
+            }
+        d_matrix A, D, E;
+        matrix_new(&A, m, m);
+        matrix_new(&D, m, m);
+        matrix_new(&E, m, m);
+        buildMatrix<float>(&A, edge, cr.data(), cc.data(), cv.data());
+        buildMatrix<float>(&D, edge, cc.data(), cr.data(), cv.data());
+        spgemm(&E, &A, &D, 0, 0, *context);
+        copy_matrix_device(&E);
+        orc_csr oa;
+        oa.m = m;
+        oa.n = m;
+        oa.nnz = edge;
+        oa.row_ptr = A.h_cscColPtr;
+        oa.col_ind = A.h_cscRowInd;
+        oa.val = A.h_cscVal;
+        orc_csr oc;
+        int64_t prods = 0;
+        orc_spgemm(&oa, &oa, &oc, &prods);
+        bool sg = E.nnz == oc.nnz;
+        for (int i = 0; sg && i <= m; ++i) sg &= E.h_cscColPtr[i] == oc.row_ptr[i];
+        for (int p = 0; sg && p < E.nnz; ++p)
+            sg &= E.h_cscRowInd[p] == oc.col_ind[p] &&
+                  std::fabs(E.h_cscVal[p] - oc.val[p]) <= 1e-5f * std::fabs(oc.val[p]);
+        EXPECT(sg, "spgemm");
+        orc_csr_free(&oc);
+        // TC (test/tc/main.cu) on the strict lower triangle
+        apspie_host_csr l{};
+        apspie_ref::check(apspie_strict_lower(&h, &l), nullptr, "lower");
+        int *d_lrp = dev_copy(l.row_ptr, size_t(m) + 1), *d_lci = dev_copy(l.col_ind, size_t(l.nnz));
+        orc_csr ol = as_orc(l);
+        const long long t = tc(m, int(l.nnz), d_lrp, d_lci, *context);
+        EXPECT(t == orc_tc(&ol, nullptr), "tc");
+        apspie_host_csr_free(&l);
+        matrix_delete(&A);
+        matrix_delete(&D);
+        matrix_delete(&E);
+        for (void *p : {(void *)d_rp, (void *)d_ci, (void *)d_res, (void *)d_w, (void *)d_dist, (void *)d_prio,
+                        (void *)d_mis, (void *)d_has, (void *)d_lrp, (void *)d_lci})
+            (void)hipFree(p);
+    } catch (const std::exception &e) {
+        printf("FAIL exception: %s\n", e.what());
+        return 2;
+    }
+    if (failures) {
+        printf("%d failures\n", failures);
+        return 1;
+    }
+    printf("REF_API OK\n");
+    return 0;
+}
