@@ -202,6 +202,21 @@ def test_mis_greedy_equals_luby_and_is_maximal(seed):
     assert np.all(got + has == 1)
 
 
+@pytest.mark.parametrize("und", [False, True])
+def test_spmspv_or_brute_force(und):
+    a = O.rmat(8, 8, 31, und)
+    rng = np.random.RandomState(5)
+    for frac in (0.0, 0.02, 0.5, 1.0):
+        x = (rng.rand(a.m) < frac).astype(np.int32) * rng.randint(-3, 4, a.m)
+        want = np.zeros(a.m, np.int32)
+        for u in range(a.m):
+            if x[u] != 0:
+                want[a.col_ind[a.row_ptr[u]:a.row_ptr[u + 1]]] = 1
+        D = dense(a, drop_loops=False)  # y = A^T x over (OR, AND)
+        assert np.array_equal(((D.T @ (x != 0).astype(np.int64)) > 0).astype(np.int32), want)
+        assert np.array_equal(O.spmspv_or(a, x), want)
+
+
 # --- SSSP --------------------------------------------------------------------
 
 def py_bellman_ford(a, src):
