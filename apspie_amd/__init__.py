@@ -53,6 +53,35 @@ class _Csr(C.Structure):
     ]
 
 
+class _DCsr(C.Structure):
+    """apspie_csr / apspie_csr_owned: device CSR plus the optional CSC."""
+
+    _fields_ = [
+        ("m", C.c_int32),
+        ("n", C.c_int32),
+        ("nnz", C.c_int64),
+        ("row_ptr", C.c_void_p),
+        ("col_ind", C.c_void_p),
+        ("val", C.c_void_p),
+        ("csc_col_ptr", C.c_void_p),
+        ("csc_row_ind", C.c_void_p),
+        ("csc_val", C.c_void_p),
+    ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [
+        ("bfs_alpha", C.c_int32),
+        ("bfs_beta", C.c_int32),
+        ("sssp_pull_ratio", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+class Round(C.Structure):
+    _fields_ = [("frontier", C.c_int32), ("pull", C.c_int32), ("edges", C.c_int64)]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("edges", C.c_int64),
@@ -97,20 +126,25 @@ def lib() -> C.CDLL:
         "apspie_load_mtx": [C.c_char_p, u32, P(_Csr)],
         "apspie_host_csr_free": [P(_Csr)],
         "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
-        "apspie_csr_upload": [vp, P(_Csr), P(_Csr)],
-        "apspie_csr_free": [vp, P(_Csr)],
-        "apspie_bfs": [vp, P(_Csr), i32, vp, P(Stats)],
-        "apspie_sssp": [vp, P(_Csr), i32, vp, P(Stats)],
-        "apspie_sssp_step": [vp, P(_Csr), vp, vp, P(Stats)],
-        "apspie_mis": [vp, P(_Csr), vp, vp, P(Stats)],
-        "apspie_spmspv_or": [vp, P(_Csr), vp, vp],
+        "apspie_ctx_set_tuning": [vp, P(Tuning)],
+        "apspie_ctx_rounds": [vp, P(Round), C.c_int, P(C.c_int)],
+        "apspie_ctx_get_tuning": [vp, P(Tuning)],
+        "apspie_host_csr_transpose": [P(_Csr), P(_Csr)],
+        "apspie_csr_upload": [vp, P(_Csr), P(_DCsr)],
+        "apspie_csr_attach_csc": [vp, P(_DCsr), P(_Csr)],
+        "apspie_csr_free": [vp, P(_DCsr)],
+        "apspie_bfs": [vp, P(_DCsr), i32, vp, P(Stats)],
+        "apspie_sssp": [vp, P(_DCsr), i32, vp, P(Stats)],
+        "apspie_sssp_step": [vp, P(_DCsr), vp, vp, P(Stats)],
+        "apspie_mis": [vp, P(_DCsr), vp, vp, P(Stats)],
+        "apspie_spmspv_or": [vp, P(_DCsr), vp, vp],
         "apspie_powerlaw_csr": [i32, i32, C.c_double, i32, u32, P(_Csr)],
-        "apspie_spgemm": [vp, P(_Csr), P(_Csr), P(_Csr), P(Stats)],
+        "apspie_spgemm": [vp, P(_DCsr), P(_DCsr), P(_DCsr), P(Stats)],
         "apspie_strict_lower": [P(_Csr), P(_Csr)],
-        "apspie_tc": [vp, P(_Csr), P(i64), P(Stats)],
-        "apspie_bfs_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
-        "apspie_sssp_batch": [vp, P(_Csr), vp, i32, vp, P(Stats)],
-        "apspie_server_create": [C.c_int, P(_Csr), C.c_int, vp, P(vp)],
+        "apspie_tc": [vp, P(_DCsr), P(i64), P(Stats)],
+        "apspie_bfs_batch": [vp, P(_DCsr), vp, i32, vp, P(Stats)],
+        "apspie_sssp_batch": [vp, P(_DCsr), vp, i32, vp, P(Stats)],
+        "apspie_server_create": [C.c_int, P(_DCsr), C.c_int, vp, P(vp)],
         "apspie_server_submit": [vp, C.c_int, C.c_int, i32, vp, C.c_double, P(C.c_uint64)],
         "apspie_server_wait": [vp, C.c_uint64, P(C.c_double)],
         "apspie_server_drain": [vp],
@@ -205,6 +239,14 @@ def rmat(scale: int, edge_factor: int = 16, seed: int = 1, undirected: bool = Fa
     return _host_from_c(s)
 
 
+def transpose(h: HostCsr) -> HostCsr:
+    """A^T by rows (A's CSC), entries of each row in increasing source-row order."""
+    s = _Csr()
+    hc = h._as_c()
+    _check(lib().apspie_host_csr_transpose(C.byref(hc), C.byref(s)), None, "transpose")
+    return _host_from_c(s)
+
+
 def strict_lower(h: HostCsr) -> HostCsr:
     """buildLower: rows keep their columns j < i, sorted."""
     s = _Csr()
@@ -259,18 +301,36 @@ class DeviceBuffer:
 
 
 class DeviceCsr:
-    """CSR resident in HBM (library-owned)."""
+    """CSR resident in HBM (library-owned), optionally with its CSC.
 
-    def __init__(self, ctx: "Context", h: HostCsr):
+    csc: None (CSR only), "symmetric" (the CSR is its own transpose), or the
+    host transpose (transpose(h)) to upload beside it.
+    """
+
+    def __init__(self, ctx: "Context", h: HostCsr, csc=None):
         self.ctx = ctx
         self.m, self.n, self.nnz = h.m, h.n, h.nnz
         self.weighted = h.val is not None
-        self._c = _Csr()
+        self._c = _DCsr()
         hc = h._as_c()
         _check(lib().apspie_csr_upload(ctx.h, C.byref(hc), C.byref(self._c)), ctx.h, "csr_upload")
+        if csc is not None:
+            if isinstance(csc, str):
+                assert csc == "symmetric", csc
+                rc = lib().apspie_csr_attach_csc(ctx.h, C.byref(self._c), None)
+            else:
+                tc_ = csc._as_c()
+                rc = lib().apspie_csr_attach_csc(ctx.h, C.byref(self._c), C.byref(tc_))
+            if rc != 0:
+                self.free()
+                _check(rc, ctx.h, "csr_attach_csc")
 
     @property
-    def c(self) -> _Csr:
+    def has_csc(self) -> bool:
+        return bool(self._c.csc_col_ptr)
+
+    @property
+    def c(self) -> _DCsr:
         return self._c
 
     def free(self) -> None:
@@ -311,7 +371,7 @@ class Context:
         _check(lib().apspie_ctx_synchronize(self.h), self.h, "synchronize")
 
     # -- per-kernel-class event timing ---------------------------------------
-    KCLASS = {"expand": 0, "compact": 1, "other": 2}
+    KCLASS = {"expand": 0, "compact": 1, "other": 2, "pull": 3}
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().apspie_ctx_set_profiling(self.h, int(on)), self.h, "set_profiling")
@@ -334,8 +394,26 @@ class Context:
         _check(lib().apspie_hbm_copy_bandwidth(self.h, nbytes, iters, C.byref(g)), self.h, "copy")
         return g.value
 
-    def upload(self, h: HostCsr) -> DeviceCsr:
-        return DeviceCsr(self, h)
+    def upload(self, h: HostCsr, csc=None) -> DeviceCsr:
+        return DeviceCsr(self, h, csc)
+
+    def rounds(self) -> list:
+        """Per-round frontier log of the last traversal: [(vertices, edges, pull)]."""
+        buf = (Round * 64)()
+        n = C.c_int()
+        _check(lib().apspie_ctx_rounds(self.h, buf, 64, C.byref(n)), self.h, "rounds")
+        return [(buf[i].frontier, buf[i].edges, buf[i].pull) for i in range(n.value)]
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().apspie_ctx_get_tuning(self.h, C.byref(t)), self.h, "get_tuning")
+        return {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta, "sssp_pull_ratio": t.sssp_pull_ratio}
+
+    def set_tuning(self, **kw) -> None:
+        cur = self.tuning()
+        cur.update(kw)
+        t = Tuning(cur["bfs_alpha"], cur["bfs_beta"], cur["sssp_pull_ratio"])
+        _check(lib().apspie_ctx_set_tuning(self.h, C.byref(t)), self.h, "set_tuning")
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
@@ -390,16 +468,16 @@ class Context:
         return out, st
 
     def spgemm_device(self, a: DeviceCsr, b: DeviceCsr):
-        """C = A * B left on the device: returns (_Csr owned by the library, Stats)."""
-        c = _Csr()
+        """C = A * B left on the device: returns (_DCsr owned by the library, Stats)."""
+        c = _DCsr()
         st = Stats()
         _check(lib().apspie_spgemm(self.h, C.byref(a.c), C.byref(b.c), C.byref(c), C.byref(st)), self.h, "spgemm")
         return c, st
 
-    def free_owned(self, c: _Csr) -> None:
+    def free_owned(self, c: _DCsr) -> None:
         lib().apspie_csr_free(self.h, C.byref(c))
 
-    def download_csr(self, c: _Csr) -> HostCsr:
+    def download_csr(self, c: _DCsr) -> HostCsr:
         m, nnz = c.m, c.nnz
         rp = np.empty(m + 1, np.int32)
         ci = np.empty(nnz, np.int32)
@@ -504,7 +582,7 @@ class Server:
                  max_bfs_batch: int = 256, max_sssp_batch: int = 64, device: int = 0):
         L = lib()
         self._graphs = list(graphs)  # keep the device CSRs alive
-        arr = (_Csr * len(graphs))(*[g.c for g in graphs])
+        arr = (_DCsr * len(graphs))(*[g.c for g in graphs])
         opts = ServerOpts(lanes, max_bfs_batch, max_sssp_batch, 0, budget_bytes)
         h = C.c_void_p()
         _check(L.apspie_server_create(device, arr, len(graphs), C.byref(opts), C.byref(h)), None, "server_create")

@@ -66,7 +66,6 @@ struct PickMs {
         }
         return true;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 
@@ -133,7 +132,6 @@ struct PickChanged {
         fr[v] = x;
         return true;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 

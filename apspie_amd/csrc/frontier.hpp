@@ -49,14 +49,47 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
 // come from ballots (mbcnt) and wave scans, the workgroup offset from the
 // look-back.
 //
-// Pick interface:
+// Pick interface, either one phase
 //   bool pick(int v, uint32_t &payload)       v may be >= m (padding lanes)
-//   void on_round(uint64_t ball, int v0, int lane)   ball: picked lanes of
-//                                                    vertices v0 .. v0+63
+// or two (all ROUNDS loads are issued before any is used)
+//   typedef Pre;  Pre load(int v);  bool pick(int v, const Pre &, uint32_t &payload)
+// plus
 //   void on_new(int v)                        for every picked vertex
+// and optionally
+//   void on_balls(uint64_t ball, int vbase, int lane, int rounds)
+//        lane r < rounds holds the ballot of picked lanes of vertices
+//        vbase + 64r .. +63 (visited / frontier bitmaps, one word pair each)
+//   int direction(const Ctl &c, unsigned long long edges_so_far, int prev_pull)
+//                                             1: run the next round as a pull
+//                                             step (written into Ctl::pull)
 // A negative level_arg means "device-driven": the round number is read from
 // C->cur_level and the compaction advances it when it completes, so a
 // captured sequence of launches can be replayed unchanged (hipGraph).
+template <class P, class = void>
+struct pick_phases {
+    struct Pre {};
+    static __device__ __forceinline__ Pre load(const P &, int) { return Pre{}; }
+    static __device__ __forceinline__ bool pick(const P &p, int v, const Pre &, uint32_t &payload) {
+        return p.pick(v, payload);
+    }
+};
+template <class P>
+struct pick_phases<P, std::void_t<typename P::Pre>> {
+    using Pre = typename P::Pre;
+    static __device__ __forceinline__ Pre load(const P &p, int v) { return p.load(v); }
+    static __device__ __forceinline__ bool pick(const P &p, int v, const Pre &x, uint32_t &payload) {
+        return p.pick(v, x, payload);
+    }
+};
+template <class P, class = void>
+struct has_on_balls : std::false_type {};
+template <class P>
+struct has_on_balls<P, std::void_t<decltype(&P::on_balls)>> : std::true_type {};
+template <class P, class = void>
+struct has_direction : std::false_type {};
+template <class P>
+struct has_direction<P, std::void_t<decltype(&P::direction)>> : std::true_type {};
+
 template <class Pick>
 __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, int nblk, int always,
                                                  const int32_t *__restrict__ row_ptr, Queue q,
@@ -88,37 +121,50 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     }
 
     const int vbase = bid * CHUNK + wid * SEGV;
-    uint32_t qb = 0, lane_cnt = 0, lane_es = 0, lane_new = 0;
-    int32_t rs[ROUNDS], dg[ROUNDS];
+    using PP = pick_phases<Pick>;
+    typename PP::Pre pre[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) pre[r] = PP::load(pk, vbase + r * WAVE + lane);
+    uint32_t newbits = 0;
+    uint64_t myball = 0;
     uint32_t vv[ROUNDS];
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const int v = vbase + r * WAVE + lane;
         uint32_t payload = 0;
-        const bool isnew = pk.pick(v, payload);
+        const bool isnew = PP::pick(pk, v, pre[r], payload);
         const uint64_t ball = __ballot(isnew);
-        rs[r] = 0;
-        dg[r] = 0;
+        if (lane == r) myball = ball;
         vv[r] = payload;
-        if (ball) {
-            pk.on_round(ball, vbase + r * WAVE, lane);
-            // row_ptr has m + 1 entries: lane v + 1 == m supplies the row end
-            // of vertex m - 1
-            const int a = v <= m ? row_ptr[v] : 0;
-            int b = __shfl_down(a, 1, WAVE);
-            if (lane == WAVE - 1) b = v + 1 <= m ? row_ptr[v + 1] : 0;
-            if (isnew) {
-                const int d = b - a;
-                rs[r] = a;
-                dg[r] = d;
-                pk.on_new(v);
-                lane_new++;
-                if (d > 0) {
-                    qb |= 1u << r;
-                    lane_cnt++;
-                    lane_es += d;
-                }
-            }
+        if (isnew) {
+            newbits |= 1u << r;
+            pk.on_new(v);
+        }
+    }
+    if constexpr (has_on_balls<Pick>::value) pk.on_balls(myball, vbase, lane, ROUNDS);
+    // row extents of the picked vertices (picked vertices are < m, so
+    // row_ptr[v + 1] exists); every load is issued before any is used
+    int32_t rs[ROUNDS], re[ROUNDS], dg[ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        rs[r] = 0;
+        re[r] = 0;
+        if ((newbits >> r) & 1u) {
+            const int v = vbase + r * WAVE + lane;
+            rs[r] = row_ptr[v];
+            re[r] = row_ptr[v + 1];
+        }
+    }
+    uint32_t qb = 0, lane_cnt = 0, lane_es = 0;
+    const uint32_t lane_new = __popc(newbits);
+#pragma unroll
+    for (int r = 0; r < ROUNDS; ++r) {
+        const int d = re[r] - rs[r];
+        dg[r] = d;
+        if (d > 0) {
+            qb |= 1u << r;
+            lane_cnt++;
+            lane_es += d;
         }
     }
     const uint32_t wcnt = wave_sum32(lane_cnt), wes = wave_sum32(lane_es), wnew = wave_sum32(lane_new);
@@ -177,11 +223,43 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
             c.nF = int(g >> 31);
             c.E = int(g & 0x7fffffffu);
             c.nTiles = (c.E + TILE - 1) / TILE;
-            c.pad = 0;
+            const unsigned long long before = atomicAdd(&C->acc.edges, (unsigned long long)c.E);
+            c.pull = 0;
+            if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, before + c.E, C->ctl[par ^ 1].pull);
             *next = c;
-            atomicAdd(&C->acc.edges, (unsigned long long)c.E);
+            if (level < LOG_ROUNDS) C->log[level] = c;
             if (level_arg < 0) C->cur_level = level + 1;
         }
+    }
+}
+
+// Inclusive max-scan of x over the TILE slots of a workgroup in slot order
+// (slot = wid*SEG + r*64 + lane); s_wmax holds one int per wave.  Contains a
+// workgroup barrier.
+__device__ __forceinline__ void slot_max_scan(int (&x)[IPT], int32_t *s_wmax) {
+    constexpr int NW = WG / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    int mx = 0;
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) mx = max(mx, x[r]);
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
+    if (lane == 0) s_wmax[wid] = mx;
+    __syncthreads();
+    int carry = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w)
+        if (w < wid) carry = max(carry, s_wmax[w]);
+#pragma unroll
+    for (int r = 0; r < IPT; ++r) {
+        int xo = max(x[r], carry);
+#pragma unroll
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const int y = __shfl_up(xo, d, WAVE);
+            if (lane >= d) xo = max(xo, y);
+        }
+        x[r] = xo;
+        carry = __shfl(xo, WAVE - 1, WAVE);
     }
 }
 
@@ -218,31 +296,9 @@ __device__ __forceinline__ int tile_owners(int t, const Ctl &cur, const Queue &q
         if (o > 0 && o < TILE) s_own[o] = k;
     }
     __syncthreads();
-    int mx = 0;
 #pragma unroll
-    for (int r = 0; r < IPT; ++r) {
-        own[r] = s_own[wid * SEG + r * WAVE + lane];
-        mx = max(mx, own[r]);
-    }
-#pragma unroll
-    for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
-    if (lane == 0) s_wmax[wid] = mx;
-    __syncthreads();
-    int carry = 0;
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-        if (w < wid) carry = max(carry, s_wmax[w]);
-#pragma unroll
-    for (int r = 0; r < IPT; ++r) {
-        int xo = max(own[r], carry);
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int y = __shfl_up(xo, d, WAVE);
-            if (lane >= d) xo = max(xo, y);
-        }
-        own[r] = xo;
-        carry = __shfl(xo, WAVE - 1, WAVE);
-    }
+    for (int r = 0; r < IPT; ++r) own[r] = s_own[wid * SEG + r * WAVE + lane];
+    slot_max_scan(own, s_wmax);
     return min(TILE, cur.E - e0);
 }
 
@@ -269,6 +325,7 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, cons
     __shared__ int32_t s_wmax[NW];
     const int level = level_arg >= 0 ? level_arg : C->cur_level;  // round of the next compaction
     const Ctl cur = C->ctl[(level - 1) & 1];
+    if (cur.pull) return;  // this round runs as a pull step
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
         int own[IPT];

@@ -344,6 +344,39 @@ int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint3
 
 // buildLower (test/tc/main.cu:39-55): keep the entries strictly below the
 // diagonal (row i keeps columns j < i), columns sorted.
+// Each worker owns a range of columns and walks the rows in order, so the
+// entries of every output row come out in increasing source-row order
+// (the stable counting sort csr2csc performs).
+int apspie_host_csr_transpose(const apspie_host_csr *a, apspie_host_csr *at) {
+    if (!a || !at) return APSPIE_EINVAL;
+    memset(at, 0, sizeof(*at));
+    return guarded(nullptr, [&] {
+        APSPIE_CHECK(a->m >= 0 && a->n >= 0 && a->nnz >= 0 && a->row_ptr, APSPIE_EINVAL, "bad input csr");
+        APSPIE_CHECK(a->nnz == 0 || a->col_ind, APSPIE_EINVAL, "bad input csr");
+        host_csr_alloc(at, a->n, a->m, a->nnz, a->val != nullptr);
+        std::vector<int64_t> cnt(size_t(a->n) + 1, 0);
+        parallel_ranges(a->n, [&](int64_t lo, int64_t hi) {
+            for (int64_t p = 0; p < a->nnz; ++p) {
+                const int64_t c = a->col_ind[p];
+                if (c >= lo && c < hi) cnt[c + 1]++;
+            }
+        });
+        for (int64_t c = 0; c < a->n; ++c) cnt[c + 1] += cnt[c];
+        APSPIE_CHECK(cnt[a->n] == a->nnz, APSPIE_EINVAL, "column index out of range");
+        for (int64_t c = 0; c <= a->n; ++c) at->row_ptr[c] = int32_t(cnt[c]);
+        parallel_ranges(a->n, [&](int64_t lo, int64_t hi) {
+            for (int32_t r = 0; r < a->m; ++r)
+                for (int32_t p = a->row_ptr[r]; p < a->row_ptr[r + 1]; ++p) {
+                    const int64_t c = a->col_ind[p];
+                    if (c < lo || c >= hi) continue;
+                    const int64_t d = cnt[c]++;
+                    at->col_ind[d] = r;
+                    if (a->val) at->val[d] = a->val[p];
+                }
+        });
+    });
+}
+
 int apspie_strict_lower(const apspie_host_csr *a, apspie_host_csr *l) {
     if (!a || !l) return APSPIE_EINVAL;
     memset(l, 0, sizeof(*l));

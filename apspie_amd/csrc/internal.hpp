@@ -50,6 +50,9 @@ enum Slot : int {
     S_STATUS,      // uint64 look-back status, two parities
     S_CTL,         // small control block
     S_PREV,        // uint32 per vertex: SSSP distance bits at the round start
+    S_FBITS,       // uint32 bitmap: the frontier a compaction produced (pull steps)
+    S_TROWS,       // int32 per CSC edge tile: row holding the tile's first edge
+    S_FVAL,        // uint32 per vertex: SSSP frontier snapshot for pull steps
     S_AUX0,
     S_AUX1,
     S_AUX2,
@@ -80,6 +83,13 @@ struct apspie_ctx {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_open;  // (class, (start, stop))
     double prof_ms[APSPIE_KCLASS_COUNT] = {};
     int64_t prof_launches[APSPIE_KCLASS_COUNT] = {};
+
+    // Direction thresholds of the traversals (apspie_ctx_set_tuning).
+    struct {
+        // SSSP pull rounds are exact but measured no faster than push on
+        // RMAT s22 (both are bound by random 4-byte accesses): off by default
+        int bfs_alpha = 14, bfs_beta = 24, sssp_pull_ratio = 0;
+    } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything
     // their kernel arguments depend on; dropped whenever a slot moves.

@@ -38,7 +38,6 @@ struct PickUndecided {
         payload = prio[v];
         return state[v] == UND;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 
@@ -69,7 +68,6 @@ struct PickJoin {
         state[v] = IN;
         return true;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 
@@ -87,7 +85,6 @@ struct PickNonzero {
     const int32_t *in;
     int m;
     __device__ bool pick(int v, uint32_t &) const { return v < m && in[v] != 0; }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 

@@ -77,6 +77,11 @@ void apspie::validate_csr(const apspie_csr *g, bool need_val) {
     APSPIE_CHECK((reinterpret_cast<uintptr_t>(g->row_ptr) & 15) == 0, APSPIE_EINVAL,
                  "row_ptr must be 16-byte aligned");
     APSPIE_CHECK(!need_val || g->nnz == 0 || g->val != nullptr, APSPIE_EINVAL, "weights required");
+    if (g->csc_col_ptr) {
+        APSPIE_CHECK(g->nnz == 0 || g->csc_row_ind != nullptr, APSPIE_EINVAL, "null csc_row_ind");
+        APSPIE_CHECK((reinterpret_cast<uintptr_t>(g->csc_col_ptr) & 15) == 0, APSPIE_EINVAL,
+                     "csc_col_ptr must be 16-byte aligned");
+    }
 }
 
 extern "C" {
@@ -136,6 +141,24 @@ int apspie_ctx_synchronize(apspie_ctx *ctx) {
 }
 
 int apspie_ctx_cu_count(const apspie_ctx *ctx) { return ctx ? ctx->cu_count : 0; }
+
+int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
+    if (!ctx || !t) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->tune.bfs_alpha = t->bfs_alpha;
+    ctx->tune.bfs_beta = t->bfs_beta;
+    ctx->tune.sssp_pull_ratio = t->sssp_pull_ratio;
+    return APSPIE_OK;
+}
+
+int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
+    if (!ctx || !t) return APSPIE_EINVAL;
+    memset(t, 0, sizeof(*t));
+    t->bfs_alpha = ctx->tune.bfs_alpha;
+    t->bfs_beta = ctx->tune.bfs_beta;
+    t->sssp_pull_ratio = ctx->tune.sssp_pull_ratio;
+    return APSPIE_OK;
+}
 
 int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable) {
     if (!ctx) return APSPIE_EINVAL;
@@ -244,6 +267,10 @@ int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c) {
     if (!ctx || !c) return APSPIE_EINVAL;
     return guarded(ctx, [&] {
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        // CSC arrays that alias the CSR ones (symmetric matrix) are freed once
+        if (c->csc_col_ptr && c->csc_col_ptr != c->row_ptr) APSPIE_HIP(hipFree(c->csc_col_ptr));
+        if (c->csc_row_ind && c->csc_row_ind != c->col_ind) APSPIE_HIP(hipFree(c->csc_row_ind));
+        if (c->csc_val && c->csc_val != c->val) APSPIE_HIP(hipFree(c->csc_val));
         if (c->row_ptr) APSPIE_HIP(hipFree(c->row_ptr));
         if (c->col_ind) APSPIE_HIP(hipFree(c->col_ind));
         if (c->val) APSPIE_HIP(hipFree(c->val));
@@ -260,7 +287,34 @@ apspie_csr apspie_csr_view(const apspie_csr_owned *c) {
     v.row_ptr = c->row_ptr;
     v.col_ind = c->col_ind;
     v.val = c->val;
+    v.csc_col_ptr = c->csc_col_ptr;
+    v.csc_row_ind = c->csc_row_ind;
+    v.csc_val = c->csc_val;
     return v;
+}
+
+int apspie_csr_attach_csc(apspie_ctx *ctx, apspie_csr_owned *c, const apspie_host_csr *csc) {
+    if (!ctx || !c) return APSPIE_EINVAL;
+    return guarded(ctx, [&] {
+        APSPIE_CHECK(c->csc_col_ptr == nullptr, APSPIE_EINVAL, "the matrix already has a CSC");
+        if (!csc) {
+            APSPIE_CHECK(c->m == c->n, APSPIE_EINVAL, "only a square matrix can be its own transpose");
+            c->csc_col_ptr = c->row_ptr;
+            c->csc_row_ind = c->col_ind;
+            c->csc_val = c->val;
+            return;
+        }
+        APSPIE_CHECK(csc->m == c->n && csc->n == c->m && csc->nnz == c->nnz, APSPIE_EINVAL,
+                     "the CSC must be the transpose of the matrix (n x m, same nnz)");
+        APSPIE_CHECK((csc->val != nullptr) == (c->val != nullptr), APSPIE_EINVAL,
+                     "the CSC must carry values exactly when the CSR does");
+        apspie_csr_owned t{};
+        const int rc = apspie_csr_upload(ctx, csc, &t);
+        if (rc != APSPIE_OK) throw Error(rc, ctx->err);
+        c->csc_col_ptr = t.row_ptr;
+        c->csc_row_ind = t.col_ind;
+        c->csc_val = t.val;
+    });
 }
 
 }  // extern "C"

@@ -64,7 +64,6 @@ struct OpUpper {
 struct PickRows {
     int m;
     __device__ bool pick(int v, uint32_t &) const { return v < m; }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 

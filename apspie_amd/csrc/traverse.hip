@@ -31,48 +31,84 @@ namespace trav {
 // ----------------------------------------------------------------------------
 // per-algorithm pieces
 
+// Direction rule (Beamer, Asanovic, Patterson, "Direction-Optimizing
+// Breadth-First Search", SC 2012): go bottom-up once the frontier's edges
+// exceed the unexplored edges / alpha, stay while the frontier holds at least
+// m / beta vertices.
+struct PullRuleBfs {
+    uint32_t *fbits;  // null: no CSC, push only
+    int64_t nnz;
+    int m, alpha, beta;
+    __device__ int decide(const Ctl &c, unsigned long long seen, int prev) const {
+        if (!fbits || alpha <= 0 || c.nF == 0) return 0;
+        if (!prev) return int64_t(c.E) * alpha > nnz - int64_t(seen) ? 1 : 0;
+        return int64_t(c.nF) * beta >= m ? 1 : 0;
+    }
+};
+
 struct PickBfs {
     uint8_t *flags;
     uint32_t *visited;
     int32_t *depth;
     int level, seed;
-    __device__ bool pick(int v, uint32_t &) const {
-        const uint8_t f = flags[v];  // flags are padded to nblk * CHUNK
+    PullRuleBfs rule;
+    using Pre = uint8_t;
+    __device__ Pre load(int v) const { return flags[v]; }  // flags are padded to nblk * CHUNK
+    __device__ bool pick(int v, Pre f, uint32_t &) const {
         if (f) flags[v] = 0;
         return f != 0 || v == seed;
     }
-    __device__ void on_round(uint64_t ball, int v0, int lane) const {
-        if (lane == 0) {  // this wave alone owns the two bitmap words of v0 .. v0+63
-            uint2 *w = (uint2 *)(visited + (v0 >> 5));
+    __device__ void on_new(int v) const { depth[v] = level; }
+    // this wave alone owns the bitmap words of its vertices: lane r ORs round
+    // r's ballot into the visited words and writes it as the frontier words
+    __device__ void on_balls(uint64_t ball, int vbase, int lane, int rounds) const {
+        if (lane >= rounds) return;
+        const int wi = (vbase >> 5) + 2 * lane;
+        if (ball) {
+            uint2 *w = (uint2 *)(visited + wi);
             uint2 o = *w;
             o.x |= uint32_t(ball);
             o.y |= uint32_t(ball >> 32);
             *w = o;
         }
+        if (rule.fbits) *(uint2 *)(rule.fbits + wi) = make_uint2(uint32_t(ball), uint32_t(ball >> 32));
     }
-    __device__ void on_new(int v) const { depth[v] = level; }
+    __device__ int direction(const Ctl &c, unsigned long long seen, int prev) const {
+        return rule.decide(c, seen, prev);
+    }
 };
+
+constexpr uint32_t NOT_IN_FRONTIER = 0xffffffffu;
 
 struct PickSssp {
     uint32_t *dist, *prev;
     int m, seed;
-    __device__ bool pick(int v, uint32_t &payload) const {
+    // null: no CSC, push only.  Else fval[v] = the distance snapshot of v if
+    // v entered the frontier, NOT_IN_FRONTIER otherwise (what a pull step reads)
+    uint32_t *fval;
+    int64_t nnz;
+    int ratio;  // pull when the frontier's edges * ratio exceed nnz
+    struct Pre {
+        uint32_t d, p;
+    };
+    __device__ Pre load(int v) const { return v < m ? Pre{dist[v], prev[v]} : Pre{0u, 0u}; }
+    __device__ bool pick(int v, const Pre &x, uint32_t &payload) const {
         if (v >= m) return false;
-        uint32_t d = dist[v];
-        const uint32_t p = prev[v];
+        uint32_t d = x.d;
         if (v == seed) {
             d = 0u;
             dist[v] = 0u;
         }
         payload = d;
-        if (d < p) {
-            prev[v] = d;
-            return true;
-        }
-        return false;
+        const bool isnew = d < x.p;
+        if (isnew) prev[v] = d;
+        if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
+        return isnew;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
+    __device__ int direction(const Ctl &c, unsigned long long, int) const {
+        return fval && ratio > 0 && int64_t(c.E) * ratio > nnz ? 1 : 0;
+    }
 };
 
 // frontier of one min-plus step: {u : -1 < x[u] < 1e38} (bitify,
@@ -86,7 +122,6 @@ struct PickStep {
         payload = __float_as_uint(xv);
         return xv > -1.0f && xv < 1e38f;
     }
-    __device__ void on_round(uint64_t, int, int) const {}
     __device__ void on_new(int) const {}
 };
 
@@ -115,6 +150,129 @@ struct OpRelax {
         if (nb < pr.d) __hip_atomic_fetch_min(&dist[c], nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 };
+
+// ----------------------------------------------------------------------------
+// pull steps (rounds whose compaction set Ctl::pull), over the CSC: cp[m+1]
+// column pointers, ri[nnz] source rows, cv[nnz] values
+
+// BFS bottom-up: every unvisited vertex scans its in-edges for a parent in
+// the frontier bitmap and stops at the first one.  On RMAT graphs nearly all
+// vertices find one at their first in-edge (the hubs, lowest ids, come
+// first), so a thread per vertex keeps the wave converged.
+__global__ __launch_bounds__(WG) void k_pull_bfs(int level_arg, int m, const Control *C,
+                                                 const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
+                                                 const uint32_t *__restrict__ visited,
+                                                 const uint32_t *__restrict__ fbits, uint8_t *flags) {
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    if (!C->ctl[(level - 1) & 1].pull) return;
+    for (int v = blockIdx.x * WG + threadIdx.x; v < m; v += gridDim.x * WG) {
+        if ((visited[v >> 5] >> (v & 31)) & 1u) continue;
+        const int a = cp[v], b = cp[v + 1];
+        for (int e = a; e < b; ++e) {
+            const int u = ri[e];
+            if ((unsigned)u < (unsigned)m && ((fbits[u >> 5] >> (u & 31)) & 1u)) {
+                flags[v] = 1;
+                break;
+            }
+        }
+    }
+}
+
+// Row holding the first edge of every TILE-edge tile of the CSC.
+__global__ __launch_bounds__(WG) void k_tile_rows(int m, const int32_t *__restrict__ cp, int32_t *trows) {
+    for (int v = blockIdx.x * WG + threadIdx.x; v < m; v += gridDim.x * WG) {
+        const int a = cp[v], b = cp[v + 1];
+        for (int t = (a + TILE - 1) / TILE; t * TILE < b; ++t) trows[t] = v;
+    }
+}
+
+// SSSP pull: y[v] = min(P[v], min over in-edges (u, v) with u in the frontier
+// of P[u] + w), P = the distances at the round start (every vertex's P equals
+// its live distance X after a compaction).  Q[u] holds P[u] for frontier
+// vertices and NOT_IN_FRONTIER for the rest, so an in-edge costs one random
+// 4-byte read.  Equal TILE-edge tiles of the CSC; row starts are marked in
+// LDS and a max-scan gives every slot the start slot of its row; a segmented
+// wave min per 64 slots, then one LDS atomicMin per segment, then one write
+// per row: a plain store for rows inside the tile, an atomicMin for the (at
+// most two) rows it shares with other tiles.  Only the improved vertices are
+// written, so X - P marks the next frontier exactly as a push round leaves it.
+__global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz, const Control *C,
+                                                   const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
+                                                   const float *__restrict__ cv, const int32_t *__restrict__ trows,
+                                                   const uint32_t *__restrict__ Q,
+                                                   const uint32_t *__restrict__ P, uint32_t *X) {
+    constexpr int NW = WG / WAVE, SEG = TILE / NW;
+    constexpr uint32_t NONE = NOT_IN_FRONTIER;
+    __shared__ int32_t s_rid[TILE];   // row (relative to the tile's first row) starting at a slot, 0 if none
+    __shared__ uint32_t s_best[TILE];  // min candidate of the row starting at a slot
+    __shared__ int32_t s_wmax[NW];
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    if (!C->ctl[(level - 1) & 1].pull) return;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int ntiles = (nnz + TILE - 1) / TILE;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int e0 = t * TILE;
+        const int elim = min(TILE, nnz - e0);
+        const int r0 = trows[t];
+        const int r1 = t + 1 < ntiles ? trows[t + 1] : m - 1;
+        for (int i = tid; i < TILE; i += WG) {
+            s_rid[i] = 0;
+            s_best[i] = NONE;
+        }
+        __syncthreads();
+        // an empty row shares its start with the next row: the larger index wins
+        for (int i = 1 + tid; i <= r1 - r0; i += WG) {
+            const int o = cp[r0 + i] - e0;
+            if (o > 0 && o < elim) atomicMax(&s_rid[o], i);
+        }
+        __syncthreads();
+        int st[IPT], u[IPT];
+        uint32_t cand[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            const int s = wid * SEG + r * WAVE + lane;
+            st[r] = (s == 0 || s_rid[s] > 0) ? s : 0;
+            u[r] = s < elim ? ri[e0 + s] : -1;
+        }
+        slot_max_scan(st, s_wmax);
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) cand[r] = (unsigned)u[r] < (unsigned)m ? Q[u[r]] : NONE;
+#pragma unroll
+        for (int r = 0; r < IPT; ++r)
+            if (cand[r] != NONE) {
+                const int s = wid * SEG + r * WAVE + lane;
+                const float w = cv ? cv[e0 + s] : 1.0f;
+                cand[r] = __float_as_uint(__uint_as_float(cand[r]) + w);
+            }
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            uint32_t x = cand[r];
+            const int sr = st[r];
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t y = __shfl_up(x, d, WAVE);
+                const int ys = __shfl_up(sr, d, WAVE);
+                if (lane >= d && ys == sr) x = min(x, y);
+            }
+            const int nxt = __shfl_down(sr, 1, WAVE);
+            if ((lane == WAVE - 1 || nxt != sr) && x != NONE) atomicMin(&s_best[sr], x);
+        }
+        __syncthreads();
+        for (int s = tid; s < elim; s += WG) {
+            if (s != 0 && s_rid[s] == 0) continue;
+            const uint32_t b = s_best[s];
+            if (b == NONE) continue;
+            const int v = r0 + s_rid[s];
+            if (b < P[v]) {
+                if (cp[v] >= e0 && cp[v + 1] <= e0 + elim)
+                    X[v] = b;
+                else
+                    __hip_atomic_fetch_min(&X[v], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        __syncthreads();
+    }
+}
 
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
@@ -213,24 +371,54 @@ void launch_expand(apspie_ctx *ctx, const Bufs &b, const apspie_csr *g, const Op
 template <int MODE>
 void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, float *dist_f,
          apspie_stats *stats) {
-    const Bufs b = get_bufs(ctx, g->m, g->nnz);
+    Bufs b = get_bufs(ctx, g->m, g->nnz);
+    const bool pull = g->csc_col_ptr != nullptr && g->nnz > 0 &&
+                      (MODE == MODE_BFS ? ctx->tune.bfs_alpha > 0 : ctx->tune.sssp_pull_ratio > 0);
+    uint32_t *fval = nullptr;
+    if (pull) {  // growing a slot leaves the others in place
+        if (MODE == MODE_BFS) {
+            b.fbits = (uint32_t *)ctx->slot(S_FBITS, size_t(b.nblk) * CHUNK / 8);
+        } else {
+            fval = (uint32_t *)ctx->slot(S_FVAL, size_t(g->m) * 4);
+            b.trows = (int32_t *)ctx->slot(S_TROWS, (size_t(g->nnz) / TILE + 2) * 4);
+        }
+    }
     uint32_t *dist = (uint32_t *)dist_f;
+    const int nnz32 = int(g->nnz);
+    const unsigned pull_grid = std::max(1u, std::min(ceil_div(g->nnz, TILE), (unsigned)ctx->cu_count * 8u));
+    const unsigned vert_grid = std::max(1u, std::min(ceil_div(g->m, WG), (unsigned)ctx->cu_count * 16u));
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     ctx->prof_begin(APSPIE_KCLASS_OTHER);
     k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE, g->m, b.nblk, depth, dist, b.prev, b.flags,
                                                        b.visited, b.status, b.C);
+    if (pull && MODE == MODE_SSSP)
+        k_tile_rows<<<vert_grid, WG, 0, ctx->stream>>>(g->m, g->csc_col_ptr, b.trows);
     ctx->prof_end();
     auto compact = [&](int level, int seed) {
         if (MODE == MODE_BFS)
-            launch_compact(ctx, b, g, PickBfs{b.flags, b.visited, depth, level, seed}, level, level == 0);
+            launch_compact(ctx, b, g,
+                           PickBfs{b.flags, b.visited, depth, level, seed,
+                                   PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}},
+                           level, level == 0);
         else
-            launch_compact(ctx, b, g, PickSssp{dist, b.prev, g->m, seed}, level, level == 0);
+            launch_compact(ctx, b, g, PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio},
+                           level, level == 0);
     };
     auto expand = [&](int level) {
         if (MODE == MODE_BFS)
             launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
         else
             launch_expand(ctx, b, g, OpRelax{g->val, dist}, level);
+        if (!pull) return;
+        ctx->prof_begin(APSPIE_KCLASS_PULL);
+        if (MODE == MODE_BFS)
+            k_pull_bfs<<<vert_grid, WG, 0, ctx->stream>>>(level, g->m, b.C, g->csc_col_ptr, g->csc_row_ind,
+                                                          b.visited, b.fbits, b.flags);
+        else
+            k_pull_relax<<<pull_grid, WG, 0, ctx->stream>>>(level, g->m, nnz32, b.C, g->csc_col_ptr,
+                                                            g->csc_row_ind, g->csc_val, b.trows, fval, b.prev,
+                                                            dist);
+        ctx->prof_end();
     };
     compact(0, src);
     int level = 1, chunk = 4;
@@ -282,6 +470,8 @@ int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist
     return guarded(ctx, [&] {
         trav::validate_traversal(g, src);
         APSPIE_CHECK(d_dist != nullptr, APSPIE_EINVAL, "null dist");
+        APSPIE_CHECK(!g->csc_col_ptr || (g->csc_val != nullptr) == (g->val != nullptr), APSPIE_EINVAL,
+                     "the CSC must carry weights exactly when the CSR does");
         trav::run<trav::MODE_SSSP>(ctx, g, src, nullptr, d_dist, stats);
     });
 }
@@ -307,6 +497,22 @@ int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, flo
         APSPIE_HIP(hipGetLastError());
         APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         fill_stats(ctx, b, MODE_STEP, g->m, stats);
+    });
+}
+
+int apspie_ctx_rounds(apspie_ctx *ctx, apspie_round *out, int max, int *n) {
+    if (!ctx || !n || max < 0 || (max > 0 && !out)) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    *n = 0;
+    return guarded(ctx, [&] {
+        using namespace trav;
+        if (!ctx->slot_ptr[S_CTL]) return;
+        Control c;
+        APSPIE_HIP(hipMemcpyAsync(&c, ctx->slot_ptr[S_CTL], sizeof(Control), hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        const int k = std::min({max, LOG_ROUNDS, std::max(c.acc.levels, 0)});
+        for (int i = 0; i < k; ++i) out[i] = apspie_round{c.log[i].nF, c.log[i].pull, c.log[i].E};
+        *n = k;
     });
 }
 

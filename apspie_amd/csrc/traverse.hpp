@@ -24,7 +24,7 @@ struct Ctl {
     int32_t nF;      // queued frontier vertices (out-degree > 0)
     int32_t E;       // edges of the queued frontier
     int32_t nTiles;  // ceil(E / TILE)
-    int32_t pad;
+    int32_t pull;    // 1: the next round runs as a pull step over in-edges
 };
 
 // Totals over one call.
@@ -35,6 +35,8 @@ struct Acc {
     int32_t pad;
 };
 
+constexpr int LOG_ROUNDS = 64;  // rounds whose frontier descriptor is kept for apspie_ctx_rounds
+
 struct Control {
     Ctl ctl[2];                 // by round parity
     unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
@@ -42,6 +44,7 @@ struct Control {
     Acc acc;
     int cur_level;  // device-driven rounds: the round the next compaction runs
     int pad2;
+    Ctl log[LOG_ROUNDS];  // the descriptor each compaction wrote, by round
 };
 
 struct Bufs {
@@ -50,6 +53,8 @@ struct Bufs {
     uint8_t *flags = nullptr;
     uint32_t *visited = nullptr;
     uint32_t *prev = nullptr;  // SSSP: distance bits at the start of the round
+    uint32_t *fbits = nullptr;  // frontier bitmap written by each compaction (pull steps only)
+    int32_t *trows = nullptr;   // SSSP pull: CSC row holding the first edge of each tile
     int32_t *queue = nullptr, *qstart = nullptr, *qoff = nullptr, *tiles = nullptr;
     float *qval = nullptr;
     uint64_t *status = nullptr;

@@ -51,18 +51,66 @@ def time_steps(ctx, fn, steps):
     return walls, kms, last
 
 
-def kernel_roofline(ctx, fn, steps, bytes_per_step_expand):
-    """Dominant kernel (frontier expansion): algorithmic bytes per launch over
-    its average launch duration, both from HIP events on the context stream."""
+def sssp_class_bytes(rounds, m, nnz):
+    """Algorithmic bytes per traversal of each SSSP kernel class, from the
+    round log (frontier i = (vertices, edges, pull) is expanded by round i+1).
+    push expansion: 12 B per relaxed edge (column, weight, distance probe),
+      12 B per frontier vertex (row start, edge offset, distance snapshot),
+      4 B per improved vertex (distance write);
+    pull step: 4 B per CSC edge (source row) + 4 B per vertex (column
+      pointer) + the m/8-byte frontier bitmap, 8 B per in-edge from the
+      frontier (weight, source distance), 4 B per improved vertex."""
+    out = {"expand": 0.0, "pull": 0.0}
+    for i, (nf, e, pull) in enumerate(rounds):
+        nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
+        if pull:
+            out["pull"] += 4.0 * nnz + 4.0 * (m + 1) + m / 8.0 + 8.0 * e + 4.0 * nxt
+        else:
+            out["expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
+    return out
+
+
+def bfs_class_bytes(rounds, m):
+    """Algorithmic bytes per traversal of each BFS kernel class.
+    push expansion: 8 B per expanded edge (column, visited-bitmap probe),
+      8 B per frontier vertex (row start, edge offset), 1 B per new vertex;
+    bottom-up step: the m/8-byte visited bitmap, 12 B per unvisited vertex
+      (column-pointer pair, first in-edge), 1 B per new vertex."""
+    out = {"expand": 0.0, "pull": 0.0}
+    seen = 0
+    for i, (nf, e, pull) in enumerate(rounds):
+        seen += nf
+        nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
+        if pull:
+            out["pull"] += m / 8.0 + 12.0 * (m - seen) + nxt
+        else:
+            out["expand"] += 8.0 * e + 8.0 * nf + nxt
+    return out
+
+
+KERNEL_NAMES = {
+    ("sssp", "expand"): "k_expand<OpRelax>",
+    ("sssp", "pull"): "k_pull_relax",
+    ("bfs", "expand"): "k_expand<OpBfs>",
+    ("bfs", "pull"): "k_pull_bfs",
+}
+
+
+def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
+    """Dominant kernel (the class with the most device time): algorithmic
+    bytes per launch over its average launch duration, both from HIP events on
+    the context stream (launches that find their round run the other way
+    exit at once and count in both the launches and the time)."""
     ctx.set_profiling(True)
     ctx.profile_reset()
     for _ in range(steps):
         fn()
     prof = ctx.profile()
     ctx.set_profiling(False)
-    ms, launches = prof["expand"]
+    cls = max(("expand", "pull"), key=lambda c: prof[c][0])
+    ms, launches = prof[cls]
     avg_ms = ms / max(launches, 1)
-    bytes_per_launch = bytes_per_step_expand * steps / max(launches, 1)
+    bytes_per_launch = bytes_by_class[cls] * steps / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     return {
         "bound": "hbm",
@@ -71,12 +119,12 @@ def kernel_roofline(ctx, fn, steps, bytes_per_step_expand):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": "k_expand",
+        "kernel": KERNEL_NAMES[(algo, cls)],
         "avg_launch_us": round(avg_ms * 1e3, 3),
         "launches_per_step": launches / steps,
         "bytes_per_launch": int(bytes_per_launch),
-        "compact_ms_per_step": round(prof["compact"][0] / steps, 4),
-        "expand_ms_per_step": round(ms / steps, 4),
+        "ms_per_step": {c: round(prof[c][0] / steps, 4) for c in ("expand", "pull", "compact", "other")},
+        "bytes_per_step": {c: int(v) for c, v in bytes_by_class.items()},
     }
 
 
@@ -207,7 +255,11 @@ def main():
     ctx = A.Context(0)
     copy_gbps = ctx.hbm_copy_bandwidth(1 << 30, 5)
     log(f"device copy: {copy_gbps:.0f} GB/s (read + write)")
-    g = ctx.upload(h)
+    t = time.time()
+    ht = A.transpose(h)  # the CSC the reference's SSSP driver builds beside the CSR
+    g = ctx.upload(h, csc=ht)
+    del ht
+    log(f"CSR + CSC uploaded ({time.time() - t:.1f}s)")
     d = ctx.alloc(4 * h.m)
     run = lambda: ctx.sssp_into(g, src, d)  # noqa: E731
     t = time.time()
@@ -223,19 +275,18 @@ def main():
     gteps = te / (ms_step * 1e-3) / 1e9
     log(f"sssp: {ms_step:.3f} ms/step, {gteps:.2f} GTEPS, iters={st.iterations} relax={st.edges}")
 
-    # expansion bytes: 12 B per relaxed edge (col, weight, distance probe),
-    # 12 B per frontier vertex (row start, edge offset, distance snapshot),
-    # 4 B per improved vertex (distance write)
-    exp_bytes = 12.0 * st.edges + 12.0 * st.vertices + 4.0 * st.improved
-    roof = kernel_roofline(ctx, run, max(3, args.steps // 2), exp_bytes)
-    traffic_file = os.path.join(ROOT, "profiles", "k_expand_sssp_rmat_s22_traffic.json")
+    rounds = ctx.rounds()
+    roof = kernel_roofline(ctx, run, max(3, args.steps // 2), sssp_class_bytes(rounds, h.m, h.nnz), "sssp")
+    kfile = roof["kernel"].split("<")[0]
+    traffic_file = os.path.join(ROOT, "profiles", f"{kfile}_sssp_rmat_s22_traffic.json")
     if os.path.exists(traffic_file) and args.scale == SSSP_SCALE:
         # HBM bytes per launch of the same kernel on the same workload, from
         # rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel
         tr = json.load(open(traffic_file))
         roof["traffic"] = int(tr["traffic_bytes_per_launch"])
         roof["traffic_source"] = os.path.relpath(traffic_file, ROOT)
-    log(f"roofline(k_expand): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak)")
+    log(f"roofline({roof['kernel']}): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak); "
+        f"rounds {rounds}")
 
     out = {
         "metric": "SSSP GTEPS at RMAT s22 (directed, weights 1..64)",
@@ -264,6 +315,7 @@ def main():
         "traversed_edges": te,
         "relaxed_edges": st.edges,
         "iterations": st.iterations,
+        "rounds": [{"frontier": f, "edges": e, "pull": p} for f, e, p in rounds],
         "relax_gteps": round(st.edges / (ms_step * 1e-3) / 1e9, 3),
         "device_ms_median": round(statistics.median(kms), 4),
         "cold_ms": round(cold.kernel_ms, 4),
@@ -278,7 +330,7 @@ def main():
         hb = A.rmat(BFS_SCALE, 16, BFS_SEED, undirected=True)
         log(f"generated RMAT s{BFS_SCALE} undirected: nnz={hb.nnz} ({time.time() - t:.1f}s)")
         del g
-        gb = ctx.upload(hb)
+        gb = ctx.upload(hb, csc="symmetric")
         db = ctx.alloc(4 * hb.m)
         runb = lambda: ctx.bfs_into(gb, 0, db)  # noqa: E731
         for _ in range(args.warmup + 1):
@@ -287,8 +339,8 @@ def main():
         ms_b = statistics.median(wb) * 1e3
         depth = db.download(np.int32, hb.m)
         te_b = reached_edges(hb, depth >= 0)
-        exp_b = 8.0 * sb.edges + 8.0 * sb.vertices + 1.0 * sb.vertices
-        roof_b = kernel_roofline(ctx, runb, max(3, args.steps // 2), exp_b)
+        rounds_b = ctx.rounds()
+        roof_b = kernel_roofline(ctx, runb, max(3, args.steps // 2), bfs_class_bytes(rounds_b, hb.m), "bfs")
         out["bfs"] = {
             "workload": f"bfs_rmat_s{BFS_SCALE}_undirected_ef16_src0",
             "nnz": hb.nnz,
@@ -296,6 +348,7 @@ def main():
             "gteps_expanded": round(sb.edges / (ms_b * 1e-3) / 1e9, 3),
             "gteps_graph500": round(te_b / 2 / (ms_b * 1e-3) / 1e9, 3),
             "levels": sb.iterations,
+            "rounds": [{"frontier": f, "edges": e, "pull": p} for f, e, p in rounds_b],
             "bytes_alg": sb.bytes_alg,
             "traversal_frac": round(sb.bytes_alg / (statistics.median(kb) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "roofline": roof_b,

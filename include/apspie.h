@@ -42,13 +42,24 @@ typedef struct apspie_ctx apspie_ctx;
  * (m, edge, d_csrRowPtr, d_csrColInd, d_csrVal) argument groups of
  * src/spmspvBfs.cuh:96, src/sssp.cuh:28, src/spmspvMis.cuh:150 and the
  * d_matrix fields of src/matrix.hpp:6-37.  col_ind values are < n;
- * nnz < 2^31.  val may be NULL (all ones). */
+ * nnz < 2^31.  val may be NULL (all ones).
+ *
+ * The csc_* fields are optional (NULL when absent): the same matrix by
+ * columns, csc_col_ptr[n+1] / csc_row_ind[nnz] / csc_val[nnz] -- the
+ * d_cscColPtrA / d_cscRowIndA / d_cscValA arrays the reference's drivers build
+ * beside the CSR with csr2csc (test/sssp/main.cu:110, src/util.cuh:474).  With
+ * them BFS and SSSP run their dense rounds as pull steps over in-edges; the
+ * results and statistics do not change.  For a symmetric matrix they may
+ * alias the CSR arrays. */
 typedef struct {
     int32_t m, n;
     int64_t nnz;
     const int32_t *row_ptr;
     const int32_t *col_ind;
     const float *val;
+    const int32_t *csc_col_ptr;
+    const int32_t *csc_row_ind;
+    const float *csc_val;
 } apspie_csr;
 
 /* Host CSR allocated by the library (apspie_host_csr_free releases it). */
@@ -60,13 +71,17 @@ typedef struct {
     float *val; /* NULL when the input is unweighted and no weights were asked for */
 } apspie_host_csr;
 
-/* Device CSR allocated by the library (apspie_csr_free releases it). */
+/* Device CSR allocated by the library (apspie_csr_free releases it), with
+ * the optional CSC of the same matrix (apspie_csr_attach_csc). */
 typedef struct {
     int32_t m, n;
     int64_t nnz;
     int32_t *row_ptr;
     int32_t *col_ind;
     float *val;
+    int32_t *csc_col_ptr;
+    int32_t *csc_row_ind;
+    float *csc_val;
 } apspie_csr_owned;
 
 /* Per-call statistics.  For BFS: edges = expanded edges (the reference's
@@ -106,11 +121,24 @@ int apspie_ctx_cu_count(const apspie_ctx *ctx);
 #define APSPIE_KCLASS_EXPAND 0   /* frontier edge expansion */
 #define APSPIE_KCLASS_COMPACT 1  /* frontier compaction + scan */
 #define APSPIE_KCLASS_OTHER 2    /* initialisation and everything else */
+#define APSPIE_KCLASS_PULL 3     /* pull steps over in-edges (BFS bottom-up, SSSP pull) */
 #define APSPIE_KCLASS_COUNT 8
 int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable);
 /* Totals since the last reset: device milliseconds and launch count. */
 int apspie_ctx_profile_read(apspie_ctx *ctx, int kclass, double *ms, int64_t *launches);
 int apspie_ctx_profile_reset(apspie_ctx *ctx);
+
+/* Direction thresholds of BFS / SSSP on graphs that carry their CSC (the
+ * reference always pushes).  bfs_alpha <= 0 turns BFS bottom-up steps off,
+ * sssp_pull_ratio <= 0 SSSP pull steps. */
+typedef struct {
+    int32_t bfs_alpha;        /* bottom-up once frontier edges * alpha > unexplored edges (default 14) */
+    int32_t bfs_beta;         /* stay bottom-up while frontier vertices * beta >= m (default 24) */
+    int32_t sssp_pull_ratio;  /* pull once frontier edges * ratio > nnz (default 0: off) */
+    int32_t reserved[5];
+} apspie_tuning;
+int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
+int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);
 
 /* Device memory helpers for callers without a framework allocator
  * (the reference's cudaMalloc / cudaMemcpy calls in test/<algo>/main.cu). */
@@ -153,8 +181,18 @@ int apspie_powerlaw_csr(int32_t m, int32_t n, double mean_len, int32_t max_len, 
  * the L of triangle counting.  Replaces buildLower (test/tc/main.cu:39-55). */
 int apspie_strict_lower(const apspie_host_csr *a, apspie_host_csr *l);
 
+/* Transpose of a host CSR (at = A^T by rows, i.e. A's CSC): rows of `at` list
+ * their entries in increasing source-row order, values moved with them.
+ * Replaces csr2csc (src/util.cuh:474-490, cusparseScsr2csc). */
+int apspie_host_csr_transpose(const apspie_host_csr *a, apspie_host_csr *at);
+
 /* Host CSR -> device (cudaMemcpy block of test/bfs/main.cu:104-120). */
 int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owned *out);
+/* Give an uploaded CSR its CSC: `csc` is the host transpose
+ * (apspie_host_csr_transpose), uploaded into c->csc_*; NULL declares the
+ * matrix symmetric, values included, and aliases the CSR arrays.  The d_cscColPtrA /
+ * d_cscRowIndA / d_cscValA set of test/sssp/main.cu:69-110. */
+int apspie_csr_attach_csc(apspie_ctx *ctx, apspie_csr_owned *c, const apspie_host_csr *csc);
 int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c);
 /* View an owned CSR as a borrowed one. */
 apspie_csr apspie_csr_view(const apspie_csr_owned *c);
@@ -164,16 +202,32 @@ apspie_csr apspie_csr_view(const apspie_csr_owned *c);
 /* Level-synchronous BFS over Boolean OR-AND SpMSpV from `src`; d_depth[m]
  * receives the level of each vertex, -1 if unreached.  Replaces
  * spmspvBfs (src/spmspvBfs.cuh:96) and bfs (src/bfs.cuh:33), which take the
- * level count from the CPU BFS; this runs until the frontier is empty. */
+ * level count from the CPU BFS; this runs until the frontier is empty.
+ * When g carries its CSC, levels whose frontier holds many edges run
+ * bottom-up (each unvisited vertex looks for a parent among its in-edges). */
 int apspie_bfs(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *d_depth,
                apspie_stats *stats);
 
 /* SSSP by synchronous min-plus Bellman-Ford over a changed-vertex frontier,
  * to convergence; d_dist[m] receives distances, APSPIE_SSSP_INF when
  * unreachable.  Weights must be finite and >= 0.  Replaces sssp
- * (src/sssp.cuh:28). */
+ * (src/sssp.cuh:28).  When g carries its CSC, rounds whose frontier holds
+ * many edges run as pull steps (each vertex takes the min over its in-edges
+ * from the frontier, no atomics); the rounds and their results are the same. */
 int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist,
                 apspie_stats *stats);
+
+/* Round log of the last frontier call (apspie_bfs, apspie_sssp, ...) on this context: for
+ * round i (i < 64), the frontier it produced (vertices with out-degree > 0
+ * and their out-edges) and whether the next round expanded it by push (0)
+ * or pull (1).  *n = rounds written.  The per-level frontier sizes the
+ * reference prints in its debug builds (src/spmspvBfs.cuh:209-210). */
+typedef struct {
+    int32_t frontier;
+    int32_t pull;
+    int64_t edges;
+} apspie_round;
+int apspie_ctx_rounds(apspie_ctx *ctx, apspie_round *out, int max, int *n);
 
 /* One min-plus SpMSpV step: frontier = {u : -1 < x[u] < 1e38},
  * y[v] = min_u x[u] + A(u,v), APSPIE_SSSP_INF where nothing reaches.

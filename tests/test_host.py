@@ -78,3 +78,49 @@ def test_rmat_limits():
         A.rmat(0, 16, 1)
     with pytest.raises(A.ApspieError):
         A.rmat(28, 16, 1)  # 2^32 edges: beyond int32 offsets
+
+
+def _dense_transpose_check(h, t):
+    """t must be h^T by rows with each row's entries in increasing source order."""
+    assert (t.m, t.n, t.nnz) == (h.n, h.m, h.nnz)
+    rows = np.repeat(np.arange(h.m, dtype=np.int64), np.diff(h.row_ptr))
+    order = np.lexsort((np.arange(h.nnz), rows, h.col_ind))  # by column, then source row, then position
+    assert np.array_equal(t.row_ptr, np.concatenate([[0], np.cumsum(np.bincount(h.col_ind, minlength=h.n))]))
+    assert np.array_equal(t.col_ind, rows[order])
+    if h.val is not None:
+        assert np.array_equal(t.val, h.val[order])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_transpose_small_files(name):
+    h = A.load_mtx(os.path.join(MTX, name + ".mtx"))
+    t = A.transpose(h)
+    _dense_transpose_check(h, t)
+    back = A.transpose(t)
+    assert np.array_equal(back.row_ptr, h.row_ptr)
+    # a row of h keeps its file order; the double transpose sorts it
+    for i in range(h.m):
+        a, b = h.row_ptr[i], h.row_ptr[i + 1]
+        assert np.array_equal(back.col_ind[a:b], np.sort(h.col_ind[a:b], kind="stable"))
+
+
+def test_transpose_rmat_weighted_and_rectangular():
+    h = A.rmat(13, 16, 31, weight_seed=32)
+    _dense_transpose_check(h, A.transpose(h))
+    p = A.powerlaw(3000, 500, 6.0, 200, 9)
+    _dense_transpose_check(p, A.transpose(p))
+    u = A.rmat(11, 8, 33, undirected=True)
+    t = A.transpose(u)  # symmetric: the transpose is the matrix itself
+    assert np.array_equal(t.row_ptr, u.row_ptr) and np.array_equal(t.col_ind, u.col_ind)
+
+
+def test_transpose_duplicates_and_bad_columns():
+    h = A.HostCsr(3, 4, np.array([0, 3, 3, 5], np.int32), np.array([2, 2, 0, 3, 2], np.int32),
+                  np.array([1, 2, 3, 4, 5], np.float32))
+    t = A.transpose(h)
+    _dense_transpose_check(h, t)
+    assert list(t.col_ind[t.row_ptr[2]:t.row_ptr[3]]) == [0, 0, 2]
+    assert list(t.val[t.row_ptr[2]:t.row_ptr[3]]) == [1, 2, 5]
+    bad = A.HostCsr(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 2], np.int32))
+    with pytest.raises(A.ApspieError):
+        A.transpose(bad)
