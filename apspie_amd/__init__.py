@@ -74,7 +74,8 @@ class Tuning(C.Structure):
         ("bfs_alpha", C.c_int32),
         ("bfs_beta", C.c_int32),
         ("sssp_pull_ratio", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("sssp_bin_ratio", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -371,7 +372,7 @@ class Context:
         _check(lib().apspie_ctx_synchronize(self.h), self.h, "synchronize")
 
     # -- per-kernel-class event timing ---------------------------------------
-    KCLASS = {"expand": 0, "compact": 1, "other": 2, "pull": 3}
+    KCLASS = {"expand": 0, "compact": 1, "other": 2, "pull": 3, "bin": 4, "apply": 5}
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().apspie_ctx_set_profiling(self.h, int(on)), self.h, "set_profiling")
@@ -407,12 +408,13 @@ class Context:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().apspie_ctx_get_tuning(self.h, C.byref(t)), self.h, "get_tuning")
-        return {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta, "sssp_pull_ratio": t.sssp_pull_ratio}
+        return {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta, "sssp_pull_ratio": t.sssp_pull_ratio,
+                "sssp_bin_ratio": t.sssp_bin_ratio}
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
         cur.update(kw)
-        t = Tuning(cur["bfs_alpha"], cur["bfs_beta"], cur["sssp_pull_ratio"])
+        t = Tuning(cur["bfs_alpha"], cur["bfs_beta"], cur["sssp_pull_ratio"], cur["sssp_bin_ratio"])
         _check(lib().apspie_ctx_set_tuning(self.h, C.byref(t)), self.h, "set_tuning")
 
     def alloc(self, nbytes: int) -> DeviceBuffer:

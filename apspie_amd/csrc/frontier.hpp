@@ -103,7 +103,10 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     // every block reads the round before it takes a dynamic id, so the last
     // id's block may advance cur_level once it has finished
-    const int level = level_arg >= 0 ? level_arg : *(volatile int *)&C->cur_level;
+    // (an atomic load keeps level_arg in a register: a select between the
+    // two addresses of a volatile load would put it in scratch memory)
+    const int level = level_arg >= 0 ? level_arg
+                                     : __hip_atomic_load(&C->cur_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = level & 1;
     Ctl *next = &C->ctl[par];
     if (!always && C->ctl[par ^ 1].nF == 0) {
@@ -115,9 +118,9 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     const int bid = s_bid;
     // state for the launch after this one (its previous user has finished)
     if (tid == 0) status[(par ^ 1) * nblk + bid] = 0;
-    if (bid == 0 && tid == 0) {
-        C->blk_counter[par ^ 1] = 0;
-        C->nnew[par ^ 1] = 0;
+    if (bid == 0 && tid < SPREAD) {
+        if (tid == 0) C->blk_counter[par ^ 1] = 0;
+        C->nnew[par ^ 1][tid] = 0;
     }
 
     const int vbase = bid * CHUNK + wid * SEGV;
@@ -144,22 +147,22 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     if constexpr (has_on_balls<Pick>::value) pk.on_balls(myball, vbase, lane, ROUNDS);
     // row extents of the picked vertices (picked vertices are < m, so
     // row_ptr[v + 1] exists); every load is issued before any is used
-    int32_t rs[ROUNDS], re[ROUNDS], dg[ROUNDS];
+    int32_t rs[ROUNDS], dg[ROUNDS];  // dg holds the row end until it becomes the degree
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         rs[r] = 0;
-        re[r] = 0;
+        dg[r] = 0;
         if ((newbits >> r) & 1u) {
             const int v = vbase + r * WAVE + lane;
             rs[r] = row_ptr[v];
-            re[r] = row_ptr[v + 1];
+            dg[r] = row_ptr[v + 1];
         }
     }
     uint32_t qb = 0, lane_cnt = 0, lane_es = 0;
     const uint32_t lane_new = __popc(newbits);
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
-        const int d = re[r] - rs[r];
+        const int d = dg[r] - rs[r];
         dg[r] = d;
         if (d > 0) {
             qb |= 1u << r;
@@ -213,9 +216,10 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     }
     if (tid == 0) {
         if (bnew) {
-            atomicAdd(&C->nnew[par], bnew);
-            atomicAdd(&C->acc.vertices, (unsigned long long)bnew);
-            atomicMax(&C->acc.levels, level + 1);
+            atomicAdd(&C->nnew[par][bid % SPREAD], bnew);
+            atomicAdd(&C->acc.vertices[bid % SPREAD], (unsigned long long)bnew);
+            // levels only grows: most workgroups see it already set
+            if (*(volatile int *)&C->acc.levels < level + 1) atomicMax(&C->acc.levels, level + 1);
         }
         if (bid == nblk - 1) {
             const uint64_t g = s_prefix + total;

@@ -53,6 +53,9 @@ enum Slot : int {
     S_FBITS,       // uint32 bitmap: the frontier a compaction produced (pull steps)
     S_TROWS,       // int32 per CSC edge tile: row holding the tile's first edge
     S_FVAL,        // uint32 per vertex: SSSP frontier snapshot for pull steps
+    S_BINOUT,      // uint2 per frontier edge: binned SSSP candidates
+    S_BINTBL,      // int32 per (bin workgroup, bucket): offsets of the binned runs
+    S_BINTOT,      // uint32 per (parity, bucket): binned entries per bucket
     S_AUX0,
     S_AUX1,
     S_AUX2,
@@ -86,9 +89,10 @@ struct apspie_ctx {
 
     // Direction thresholds of the traversals (apspie_ctx_set_tuning).
     struct {
-        // SSSP pull rounds are exact but measured no faster than push on
-        // RMAT s22 (both are bound by random 4-byte accesses): off by default
-        int bfs_alpha = 14, bfs_beta = 24, sssp_pull_ratio = 0;
+        // SSSP pull and binned rounds are exact but measured slower than push
+        // on RMAT s22 (the push round's distance probes mostly hit L2): off by
+        // default
+        int bfs_alpha = 14, bfs_beta = 24, sssp_pull_ratio = 0, sssp_bin_ratio = 0;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

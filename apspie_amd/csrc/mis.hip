@@ -169,10 +169,12 @@ int apspie_mis(apspie_ctx *ctx, const apspie_csr *g, const uint32_t *d_prio, int
             }
             APSPIE_HIP(hipGetLastError());
             // vertices that joined in the last round; none => nothing undecided
-            APSPIE_HIP(hipMemcpyAsync((void *)host, &b.C->nnew[(level - 1) & 1], 4, hipMemcpyDeviceToHost,
-                                      ctx->stream));
+            APSPIE_HIP(hipMemcpyAsync((void *)host, &b.C->nnew[(level - 1) & 1][0], 4 * SPREAD,
+                                      hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-            if (*host == 0) break;
+            uint32_t joined = 0;
+            for (int i = 0; i < SPREAD; ++i) joined += host[i];
+            if (joined == 0) break;
             APSPIE_CHECK(rounds <= g->m + 1, APSPIE_ENOCONV, "MIS did not converge");
             chunk = std::min(chunk * 2, 16);
         }

@@ -148,6 +148,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.bfs_alpha = t->bfs_alpha;
     ctx->tune.bfs_beta = t->bfs_beta;
     ctx->tune.sssp_pull_ratio = t->sssp_pull_ratio;
+    ctx->tune.sssp_bin_ratio = t->sssp_bin_ratio;
     return APSPIE_OK;
 }
 
@@ -157,6 +158,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->bfs_alpha = ctx->tune.bfs_alpha;
     t->bfs_beta = ctx->tune.bfs_beta;
     t->sssp_pull_ratio = ctx->tune.sssp_pull_ratio;
+    t->sssp_bin_ratio = ctx->tune.sssp_bin_ratio;
     return APSPIE_OK;
 }
 

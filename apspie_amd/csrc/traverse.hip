@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "binned.hpp"
 #include "frontier.hpp"
 #include "internal.hpp"
 
@@ -87,7 +88,8 @@ struct PickSssp {
     // v entered the frontier, NOT_IN_FRONTIER otherwise (what a pull step reads)
     uint32_t *fval;
     int64_t nnz;
-    int ratio;  // pull when the frontier's edges * ratio exceed nnz
+    int ratio;      // pull when the frontier's edges * ratio exceed nnz
+    int bin_ratio;  // binned relaxation when the frontier's edges * bin_ratio exceed nnz (0: off)
     struct Pre {
         uint32_t d, p;
     };
@@ -107,7 +109,9 @@ struct PickSssp {
     }
     __device__ void on_new(int) const {}
     __device__ int direction(const Ctl &c, unsigned long long, int) const {
-        return fval && ratio > 0 && int64_t(c.E) * ratio > nnz ? 1 : 0;
+        if (bin_ratio > 0 && int64_t(c.E) * bin_ratio > nnz) return DIR_BINNED;
+        if (fval && ratio > 0 && int64_t(c.E) * ratio > nnz) return DIR_PULL;
+        return DIR_PUSH;
     }
 };
 
@@ -164,7 +168,7 @@ __global__ __launch_bounds__(WG) void k_pull_bfs(int level_arg, int m, const Con
                                                  const uint32_t *__restrict__ visited,
                                                  const uint32_t *__restrict__ fbits, uint8_t *flags) {
     const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    if (!C->ctl[(level - 1) & 1].pull) return;
+    if (C->ctl[(level - 1) & 1].pull != DIR_PULL) return;
     for (int v = blockIdx.x * WG + threadIdx.x; v < m; v += gridDim.x * WG) {
         if ((visited[v >> 5] >> (v & 31)) & 1u) continue;
         const int a = cp[v], b = cp[v + 1];
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
     __shared__ uint32_t s_best[TILE];  // min candidate of the row starting at a slot
     __shared__ int32_t s_wmax[NW];
     const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    if (!C->ctl[(level - 1) & 1].pull) return;
+    if (C->ctl[(level - 1) & 1].pull != DIR_PULL) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int ntiles = (nnz + TILE - 1) / TILE;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
 
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
-                                             uint64_t *status, Control *C) {
+                                             uint64_t *status, Control *C, uint32_t *bin_totals) {
     const int64_t gtid = int64_t(blockIdx.x) * WG + threadIdx.x;
     const int64_t stride = int64_t(gridDim.x) * WG;
     const int64_t mpad = int64_t(nblk) * CHUNK;
@@ -293,6 +297,8 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
         }
     }
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
+    if (bin_totals)
+        for (int64_t i = gtid; i < 2 * MAX_BUCKETS; i += stride) bin_totals[i] = 0;
     if (gtid == 0) {
         Control z{};
         *C = z;
@@ -336,7 +342,7 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
     APSPIE_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     memset(stats, 0, sizeof(*stats));
     stats->edges = (int64_t)acc.edges;
-    stats->vertices = (int64_t)acc.vertices;
+    for (int i = 0; i < SPREAD; ++i) stats->vertices += (int64_t)acc.vertices[i];
     stats->iterations = acc.levels;
     stats->kernel_ms = ms;
     if (mode == MODE_BFS) {
@@ -383,14 +389,30 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             b.trows = (int32_t *)ctx->slot(S_TROWS, (size_t(g->nnz) / TILE + 2) * 4);
         }
     }
+    // binned relaxation of dense SSSP rounds: entry buffer, per-unit offset
+    // table and per-bucket totals
+    const int nb = (int)ceil_div(g->m, BUCKET);
+    const bool binned = MODE == MODE_SSSP && ctx->tune.sssp_bin_ratio > 0 && g->nnz > 0 && nb <= MAX_BUCKETS;
+    uint2 *bin_out = nullptr;
+    int32_t *bin_tbl = nullptr;
+    uint32_t *bin_tot = nullptr;
+    const int max_tiles = (int)ceil_div(g->nnz, TILE);
+    // (the apply gives each bin workgroup's run one thread: bin_grid <= WGA)
+    const int bin_grid = std::max(1, std::min({max_tiles, ctx->cu_count * 3, WGA}));
+    if (binned) {
+        bin_out = (uint2 *)ctx->slot(S_BINOUT, size_t(max_tiles) * TILE * sizeof(uint2));
+        bin_tbl = (int32_t *)ctx->slot(S_BINTBL, size_t(bin_grid) * (nb + 1) * sizeof(int32_t));
+        bin_tot = (uint32_t *)ctx->slot(S_BINTOT, 2 * MAX_BUCKETS * sizeof(uint32_t));
+    }
     uint32_t *dist = (uint32_t *)dist_f;
     const int nnz32 = int(g->nnz);
     const unsigned pull_grid = std::max(1u, std::min(ceil_div(g->nnz, TILE), (unsigned)ctx->cu_count * 8u));
     const unsigned vert_grid = std::max(1u, std::min(ceil_div(g->m, WG), (unsigned)ctx->cu_count * 16u));
+    const unsigned apply_grid = unsigned(nb + g->nnz / PART_ENTRIES + 1);
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     ctx->prof_begin(APSPIE_KCLASS_OTHER);
     k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE, g->m, b.nblk, depth, dist, b.prev, b.flags,
-                                                       b.visited, b.status, b.C);
+                                                       b.visited, b.status, b.C, bin_tot);
     if (pull && MODE == MODE_SSSP)
         k_tile_rows<<<vert_grid, WG, 0, ctx->stream>>>(g->m, g->csc_col_ptr, b.trows);
     ctx->prof_end();
@@ -401,7 +423,9 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                                    PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}},
                            level, level == 0);
         else
-            launch_compact(ctx, b, g, PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio},
+            launch_compact(ctx, b, g,
+                           PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
+                                    binned ? ctx->tune.sssp_bin_ratio : 0},
                            level, level == 0);
     };
     auto expand = [&](int level) {
@@ -409,6 +433,16 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
         else
             launch_expand(ctx, b, g, OpRelax{g->val, dist}, level);
+        if (binned) {
+            ctx->prof_begin(APSPIE_KCLASS_BIN);
+            k_bin_relax<<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b), g->col_ind, g->val,
+                                                          bin_out, bin_tbl, bin_tot);
+            ctx->prof_end();
+            ctx->prof_begin(APSPIE_KCLASS_APPLY);
+            k_apply_relax<<<apply_grid, WGA, 0, ctx->stream>>>(level, g->m, nb, bin_grid, b.C, bin_out, bin_tbl,
+                                                               bin_tot, dist);
+            ctx->prof_end();
+        }
         if (!pull) return;
         ctx->prof_begin(APSPIE_KCLASS_PULL);
         if (MODE == MODE_BFS)
@@ -490,7 +524,7 @@ int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, flo
         APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         ctx->prof_begin(APSPIE_KCLASS_OTHER);
         k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE_STEP, g->m, b.nblk, nullptr, y, nullptr,
-                                                           b.flags, b.visited, b.status, b.C);
+                                                           b.flags, b.visited, b.status, b.C, nullptr);
         ctx->prof_end();
         launch_compact(ctx, b, g, PickStep{d_x, g->m}, 0, 1);
         launch_expand(ctx, b, g, OpRelax{g->val, y}, 1);

@@ -10,8 +10,8 @@ namespace apspie {
 namespace trav {
 
 constexpr int WG = 256;           // threads per expansion workgroup (4 waves of 64)
-constexpr int WGC = 512;          // threads per compaction workgroup (8 waves)
-constexpr int VPT = 16;           // vertices per thread in the compaction pass
+constexpr int WGC = 1024;         // threads per compaction workgroup (16 waves)
+constexpr int VPT = 8;           // vertices per thread in the compaction pass
 constexpr int CHUNK = WGC * VPT;  // vertices per compaction workgroup
 constexpr int TILE = 2048;        // frontier edges per expansion tile
 constexpr int IPT = TILE / WG;    // edges per thread per tile
@@ -24,14 +24,25 @@ struct Ctl {
     int32_t nF;      // queued frontier vertices (out-degree > 0)
     int32_t E;       // edges of the queued frontier
     int32_t nTiles;  // ceil(E / TILE)
-    int32_t pull;    // 1: the next round runs as a pull step over in-edges
+    int32_t pull;    // how the next round expands this frontier: DIR_*
 };
+
+enum {
+    DIR_PUSH = 0,    // k_expand over the queued frontier
+    DIR_PULL = 1,    // pull step over the CSC
+    DIR_BINNED = 2,  // SSSP: binned relaxation (k_bin_relax + k_apply_relax)
+};
+
+// Counters every compaction workgroup adds to are spread over SPREAD slots
+// (workgroup id mod SPREAD): a single address serialises the atomics of all
+// workgroups.  Readers sum the slots.
+constexpr int SPREAD = 16;
 
 // Totals over one call.
 struct Acc {
-    unsigned long long edges;     // sum of frontier edge counts
-    unsigned long long vertices;  // vertices that entered a frontier
-    int32_t levels;               // rounds with a non-empty frontier
+    unsigned long long edges;             // sum of frontier edge counts
+    unsigned long long vertices[SPREAD];  // vertices that entered a frontier
+    int32_t levels;                       // rounds with a non-empty frontier
     int32_t pad;
 };
 
@@ -40,7 +51,7 @@ constexpr int LOG_ROUNDS = 64;  // rounds whose frontier descriptor is kept for 
 struct Control {
     Ctl ctl[2];                 // by round parity
     unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
-    unsigned int nnew[2];         // vertices picked by the compaction, by round parity
+    unsigned int nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
     Acc acc;
     int cur_level;  // device-driven rounds: the round the next compaction runs
     int pad2;

@@ -59,12 +59,19 @@ def sssp_class_bytes(rounds, m, nnz):
       4 B per improved vertex (distance write);
     pull step: 4 B per CSC edge (source row) + 4 B per vertex (column
       pointer) + the m/8-byte frontier bitmap, 8 B per in-edge from the
-      frontier (weight, source distance), 4 B per improved vertex."""
-    out = {"expand": 0.0, "pull": 0.0}
-    for i, (nf, e, pull) in enumerate(rounds):
+      frontier (weight, source distance), 4 B per improved vertex;
+    binned round, bin kernel: 8 B per relaxed edge (column, weight) + 12 B
+      per frontier vertex + 8 B per candidate written;
+    binned round, apply kernel: 8 B per candidate read + 4 B per vertex
+      (distance slices) + 4 B per improved vertex."""
+    out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0}
+    for i, (nf, e, mode) in enumerate(rounds):
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
-        if pull:
+        if mode == 1:
             out["pull"] += 4.0 * nnz + 4.0 * (m + 1) + m / 8.0 + 8.0 * e + 4.0 * nxt
+        elif mode == 2:
+            out["bin"] += 16.0 * e + 12.0 * nf
+            out["apply"] += 8.0 * e + 4.0 * m + 4.0 * nxt
         else:
             out["expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
     return out
@@ -93,6 +100,8 @@ KERNEL_NAMES = {
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",
     ("bfs", "pull"): "k_pull_bfs",
+    ("sssp", "bin"): "k_bin_relax",
+    ("sssp", "apply"): "k_apply_relax",
 }
 
 
@@ -107,7 +116,7 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
         fn()
     prof = ctx.profile()
     ctx.set_profiling(False)
-    cls = max(("expand", "pull"), key=lambda c: prof[c][0])
+    cls = max(bytes_by_class, key=lambda c: prof[c][0])
     ms, launches = prof[cls]
     avg_ms = ms / max(launches, 1)
     bytes_per_launch = bytes_by_class[cls] * steps / max(launches, 1)
@@ -123,7 +132,7 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
         "avg_launch_us": round(avg_ms * 1e3, 3),
         "launches_per_step": launches / steps,
         "bytes_per_launch": int(bytes_per_launch),
-        "ms_per_step": {c: round(prof[c][0] / steps, 4) for c in ("expand", "pull", "compact", "other")},
+        "ms_per_step": {c: round(prof[c][0] / steps, 4) for c in prof if prof[c][1]},
         "bytes_per_step": {c: int(v) for c, v in bytes_by_class.items()},
     }
 

@@ -122,6 +122,8 @@ int apspie_ctx_cu_count(const apspie_ctx *ctx);
 #define APSPIE_KCLASS_COMPACT 1  /* frontier compaction + scan */
 #define APSPIE_KCLASS_OTHER 2    /* initialisation and everything else */
 #define APSPIE_KCLASS_PULL 3     /* pull steps over in-edges (BFS bottom-up, SSSP pull) */
+#define APSPIE_KCLASS_BIN 4      /* SSSP binned relaxation: candidates written by bucket */
+#define APSPIE_KCLASS_APPLY 5    /* SSSP binned relaxation: buckets folded into the distances */
 #define APSPIE_KCLASS_COUNT 8
 int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable);
 /* Totals since the last reset: device milliseconds and launch count. */
@@ -130,12 +132,14 @@ int apspie_ctx_profile_reset(apspie_ctx *ctx);
 
 /* Direction thresholds of BFS / SSSP on graphs that carry their CSC (the
  * reference always pushes).  bfs_alpha <= 0 turns BFS bottom-up steps off,
- * sssp_pull_ratio <= 0 SSSP pull steps. */
+ * sssp_pull_ratio <= 0 SSSP pull steps, sssp_bin_ratio <= 0 binned SSSP
+ * rounds (those need m <= 2^23). */
 typedef struct {
     int32_t bfs_alpha;        /* bottom-up once frontier edges * alpha > unexplored edges (default 14) */
     int32_t bfs_beta;         /* stay bottom-up while frontier vertices * beta >= m (default 24) */
     int32_t sssp_pull_ratio;  /* pull once frontier edges * ratio > nnz (default 0: off) */
-    int32_t reserved[5];
+    int32_t sssp_bin_ratio;   /* binned relaxation once frontier edges * ratio > nnz (default 0: off) */
+    int32_t reserved[4];
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

@@ -15,7 +15,9 @@ pytestmark = pytest.mark.gpu
 
 INF = np.float32(1.70141e38)
 NAMES = ["test_cc", "test_bc", "test_pr", "test_mesh", "test_mis", "test_spgemm", "chesapeake"]
-ALWAYS = dict(bfs_alpha=1 << 20, bfs_beta=1 << 20, sssp_pull_ratio=1 << 20)  # pull every round
+ALWAYS = dict(bfs_alpha=1 << 20, bfs_beta=1 << 20, sssp_pull_ratio=1 << 20, sssp_bin_ratio=0)  # pull every round
+BINNED = dict(sssp_pull_ratio=0, sssp_bin_ratio=1 << 20)  # binned relaxation every SSSP round
+PUSH = dict(bfs_alpha=0, sssp_pull_ratio=0, sssp_bin_ratio=0)
 
 
 def ocsr(h):
@@ -70,13 +72,15 @@ def test_bfs_pull_small_files(tuned, name, und):
         assert np.array_equal(depth, O.bfs(ocsr(h), src)[0]), (name, und, src)
 
 
-@pytest.mark.parametrize("mode", ["default", "always"])
+MODES = {"default": {}, "always": ALWAYS, "binned": BINNED, "push": PUSH}
+
+
+@pytest.mark.parametrize("mode", ["default", "always", "binned", "push"])
 @pytest.mark.parametrize("scale,und,seed", [(8, False, 2008), (12, False, 2012), (13, True, 3013),
-                                            (15, False, 2015)])
+                                            (15, False, 2015), (17, False, 2017)])
 def test_sssp_pull_rmat(tuned, mode, scale, und, seed):
     ctx = tuned
-    if mode == "always":
-        ctx.set_tuning(**ALWAYS)
+    ctx.set_tuning(**MODES[mode])
     h = A.rmat(scale, 16, seed, undirected=und, weight_seed=seed + 1000)
     g = with_csc(ctx, h, und)
     o = ocsr(h)
@@ -87,15 +91,31 @@ def test_sssp_pull_rmat(tuned, mode, scale, und, seed):
         assert (st.iterations, st.edges, st.vertices, st.improved) == (it, er, sf, ri)
 
 
-def test_sssp_pull_unweighted_is_bfs(tuned):
+@pytest.mark.parametrize("mode", ["always", "binned"])
+def test_sssp_pull_unweighted_is_bfs(tuned, mode):
     ctx = tuned
-    ctx.set_tuning(**ALWAYS)
+    ctx.set_tuning(**MODES[mode])
     h = A.rmat(12, 16, 5, undirected=True)
     g = with_csc(ctx, h, True)
     dist, _ = ctx.sssp(g, 0)
     b = O.bfs(ocsr(h), 0)[0].astype(np.float32)
     b[b < 0] = INF
     assert np.array_equal(dist, b)
+
+
+def test_binned_rounds_without_csc_and_round_log(tuned):
+    """Binned relaxation needs no CSC; the round log records its direction."""
+    ctx = tuned
+    ctx.set_tuning(**BINNED)
+    h = A.rmat(14, 16, 714, undirected=False, weight_seed=715)
+    g = ctx.upload(h)
+    src = h.argmax_degree()
+    dist, st = ctx.sssp(g, src)
+    assert np.array_equal(dist, oracle_dist(h, src))
+    rounds = ctx.rounds()
+    assert len(rounds) == st.iterations and all(p == 2 for f, e, p in rounds if e > 0)
+    _, it, er, sf, ri = O.sssp_bf(ocsr(h), src)
+    assert (st.iterations, st.edges, st.vertices, st.improved) == (it, er, sf, ri)
 
 
 def _star_with_gaps(n_leaves, n_isolated):
@@ -118,9 +138,10 @@ def _star_with_gaps(n_leaves, n_isolated):
 
 
 @pytest.mark.parametrize("leaves,isolated", [(5000, 0), (5000, 20000), (3, 9000), (2047, 1), (2048, 4096)])
-def test_pull_tiles_with_hubs_and_empty_rows(tuned, leaves, isolated):
+@pytest.mark.parametrize("mode", ["always", "binned"])
+def test_pull_tiles_with_hubs_and_empty_rows(tuned, leaves, isolated, mode):
     ctx = tuned
-    ctx.set_tuning(**ALWAYS)
+    ctx.set_tuning(**MODES[mode])
     h = _star_with_gaps(leaves, isolated)
     g = with_csc(ctx, h, False)
     for src in [1, 0, leaves, h.m - 3]:
@@ -130,6 +151,26 @@ def test_pull_tiles_with_hubs_and_empty_rows(tuned, leaves, isolated):
         assert np.array_equal(dist, oracle_dist(h, src)), (leaves, isolated, src)
         _, it, er, sf, ri = O.sssp_bf(ocsr(h), src)
         assert (st.iterations, st.edges, st.vertices, st.improved) == (it, er, sf, ri)
+
+
+def test_binned_bucket_split_across_workgroups(tuned):
+    """600,000 candidates land in the first 16,384-vertex bucket in round 2,
+    so several apply workgroups share it and merge with atomicMin."""
+    ctx = tuned
+    ctx.set_tuning(**BINNED)
+    n = 600_000
+    m = n + 1  # vertex 0 has n out-edges, every other vertex one
+    rp = np.zeros(m + 1, np.int64)
+    rp[1] = n
+    rp[2:] = n + np.arange(1, m)
+    ci = np.concatenate([np.arange(1, m), np.arange(1, m) % 97 + 3]).astype(np.int32)
+    w = np.concatenate([np.arange(n) % 64 + 1, np.arange(n) % 5 + 1]).astype(np.float32)
+    h = A.HostCsr(m, m, rp.astype(np.int32), ci, w)
+    g = ctx.upload(h)
+    dist, st = ctx.sssp(g, 0)
+    assert np.array_equal(dist, oracle_dist(h, 0))
+    _, it, er, sf, ri = O.sssp_bf(ocsr(h), 0)
+    assert (st.iterations, st.edges, st.vertices, st.improved) == (it, er, sf, ri)
 
 
 def test_push_and_pull_agree_and_thresholds_round_trip(tuned):
