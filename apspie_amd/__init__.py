@@ -75,8 +75,17 @@ class Tuning(C.Structure):
         ("bfs_beta", C.c_int32),
         ("sssp_pull_ratio", C.c_int32),
         ("sssp_bin_ratio", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("expand_wg_per_cu", C.c_int32),
+        ("round_chunk", C.c_int32),
+        ("batch_chunk", C.c_int32),
+        ("sssp_batch_lanes", C.c_int32),
     ]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+TUNE_BFS, TUNE_SSSP, TUNE_BFS_BATCH, TUNE_SSSP_BATCH = 1, 2, 4, 8
 
 
 class Round(C.Structure):
@@ -130,6 +139,7 @@ def lib() -> C.CDLL:
         "apspie_ctx_set_tuning": [vp, P(Tuning)],
         "apspie_ctx_rounds": [vp, P(Round), C.c_int, P(C.c_int)],
         "apspie_ctx_get_tuning": [vp, P(Tuning)],
+        "apspie_autotune": [vp, P(_DCsr), i32, i32, u32, P(Tuning)],
         "apspie_host_csr_transpose": [P(_Csr), P(_Csr)],
         "apspie_csr_upload": [vp, P(_Csr), P(_DCsr)],
         "apspie_csr_attach_csc": [vp, P(_DCsr), P(_Csr)],
@@ -408,14 +418,23 @@ class Context:
     def tuning(self) -> dict:
         t = Tuning()
         _check(lib().apspie_ctx_get_tuning(self.h, C.byref(t)), self.h, "get_tuning")
-        return {"bfs_alpha": t.bfs_alpha, "bfs_beta": t.bfs_beta, "sssp_pull_ratio": t.sssp_pull_ratio,
-                "sssp_bin_ratio": t.sssp_bin_ratio}
+        return t.as_dict()
 
     def set_tuning(self, **kw) -> None:
         cur = self.tuning()
+        unknown = set(kw) - set(cur)
+        if unknown:
+            raise KeyError(f"unknown tuning knobs {sorted(unknown)}")
         cur.update(kw)
-        t = Tuning(cur["bfs_alpha"], cur["bfs_beta"], cur["sssp_pull_ratio"], cur["sssp_bin_ratio"])
+        t = Tuning(**cur)
         _check(lib().apspie_ctx_set_tuning(self.h, C.byref(t)), self.h, "set_tuning")
+
+    def autotune(self, g: "DeviceCsr", src: int, kinds: int = TUNE_BFS | TUNE_SSSP, k: int = 64) -> dict:
+        """Time candidate knob values on g (apspie_autotune), keep the fastest
+        in this context and return them."""
+        t = Tuning()
+        _check(lib().apspie_autotune(self.h, C.byref(g.c), src, k, kinds, C.byref(t)), self.h, "autotune")
+        return t.as_dict()
 
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)

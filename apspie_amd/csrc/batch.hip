@@ -265,8 +265,10 @@ __global__ __launch_bounds__(256) void k_mss_transpose(int m, int k, const uint3
 // every address and size its kernel arguments hold (slot growth drops the
 // cache), so repeated calls on one graph skip capture and instantiation.
 template <class F>
-int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &key, int max_rounds, F &&enqueue_round) {
-    constexpr int CH = 4;
+int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &base_key, int max_rounds, F &&enqueue_round) {
+    const int CH = ctx->tune.batch_chunk;
+    // the chunk length and the launch shape are part of what was captured
+    const std::string key = base_key + "|ch" + std::to_string(CH) + "|eg" + std::to_string(b.expand_grid);
     hipGraphExec_t exec = nullptr;
     auto it = ctx->graphs.find(key);
     if (it != ctx->graphs.end()) {
@@ -417,7 +419,8 @@ int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_src
         APSPIE_CHECK(k >= 1 && k <= 64, APSPIE_ELIMIT, "batched SSSP takes 1..64 sources");
         APSPIE_CHECK(d_srcs && d_dist, APSPIE_EINVAL, "null sources or output");
         trav::check_sources(ctx, g, d_srcs, k);
-        if (k <= 16) trav::sssp_batch<16>(ctx, g, d_srcs, k, d_dist, stats);
+        const int lanes = ctx->tune.sssp_batch_lanes;
+        if (k <= 16 && lanes != 64) trav::sssp_batch<16>(ctx, g, d_srcs, k, d_dist, stats);
         else trav::sssp_batch<64>(ctx, g, d_srcs, k, d_dist, stats);
     });
 }

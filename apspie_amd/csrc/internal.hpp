@@ -87,12 +87,13 @@ struct apspie_ctx {
     double prof_ms[APSPIE_KCLASS_COUNT] = {};
     int64_t prof_launches[APSPIE_KCLASS_COUNT] = {};
 
-    // Direction thresholds of the traversals (apspie_ctx_set_tuning).
+    // Knobs of the traversals (apspie_ctx_set_tuning, apspie_autotune).
     struct {
         // SSSP pull and binned rounds are exact but measured slower than push
         // on RMAT s22 (the push round's distance probes mostly hit L2): off by
         // default
         int bfs_alpha = 14, bfs_beta = 24, sssp_pull_ratio = 0, sssp_bin_ratio = 0;
+        int expand_wg_per_cu = 8, round_chunk = 4, batch_chunk = 4, sssp_batch_lanes = 0;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

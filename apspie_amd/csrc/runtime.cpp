@@ -144,11 +144,21 @@ int apspie_ctx_cu_count(const apspie_ctx *ctx) { return ctx ? ctx->cu_count : 0;
 
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     if (!ctx || !t) return APSPIE_EINVAL;
+    if (t->expand_wg_per_cu < 1 || t->expand_wg_per_cu > 32 || t->round_chunk < 1 || t->round_chunk > 32 ||
+        t->batch_chunk < 1 || t->batch_chunk > 16 ||
+        (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64)) {
+        ctx->err = "tuning knob out of range";
+        return APSPIE_EINVAL;
+    }
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->tune.bfs_alpha = t->bfs_alpha;
     ctx->tune.bfs_beta = t->bfs_beta;
     ctx->tune.sssp_pull_ratio = t->sssp_pull_ratio;
     ctx->tune.sssp_bin_ratio = t->sssp_bin_ratio;
+    ctx->tune.expand_wg_per_cu = t->expand_wg_per_cu;
+    ctx->tune.round_chunk = t->round_chunk;
+    ctx->tune.batch_chunk = t->batch_chunk;
+    ctx->tune.sssp_batch_lanes = t->sssp_batch_lanes;
     return APSPIE_OK;
 }
 
@@ -159,6 +169,10 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->bfs_beta = ctx->tune.bfs_beta;
     t->sssp_pull_ratio = ctx->tune.sssp_pull_ratio;
     t->sssp_bin_ratio = ctx->tune.sssp_bin_ratio;
+    t->expand_wg_per_cu = ctx->tune.expand_wg_per_cu;
+    t->round_chunk = ctx->tune.round_chunk;
+    t->batch_chunk = ctx->tune.batch_chunk;
+    t->sssp_batch_lanes = ctx->tune.sssp_batch_lanes;
     return APSPIE_OK;
 }
 

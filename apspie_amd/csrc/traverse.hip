@@ -322,7 +322,8 @@ Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz) {
     b.tiles = (int32_t *)ctx->slot(S_TILES, (size_t(nnz) / TILE + 4) * 4);
     b.status = (uint64_t *)ctx->slot(S_STATUS, size_t(b.nblk) * 2 * 8);
     b.C = (Control *)ctx->slot(S_CTL, sizeof(Control));
-    b.expand_grid = std::max(1u, std::min(ceil_div(nnz, TILE) + 1, (unsigned)ctx->cu_count * 8u));
+    b.expand_grid =
+        std::max(1u, std::min(ceil_div(nnz, TILE) + 1, unsigned(ctx->cu_count * ctx->tune.expand_wg_per_cu)));
     return b;
 }
 
@@ -455,7 +456,10 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         ctx->prof_end();
     };
     compact(0, src);
-    int level = 1, chunk = 4;
+    // chunks of 4, 2, 2, 4, 8, ... rounds between frontier checks: RMAT
+    // traversals end within 6-12 rounds, and a round launched past the end
+    // costs its kernels' launch
+    int level = 1, chunk = ctx->tune.round_chunk, checks = 0;
     volatile int32_t *host_nf = (volatile int32_t *)ctx->pinned;
     for (;;) {
         for (int i = 0; i < chunk; ++i, ++level) {
@@ -468,7 +472,8 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         if (*host_nf == 0) break;
         APSPIE_CHECK(level <= g->m + 2, APSPIE_ENOCONV, "traversal did not converge");
-        chunk = std::min(chunk * 2, 32);
+        ++checks;
+        chunk = checks <= 2 ? 2 : std::min(chunk * 2, 32);
     }
     APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     fill_stats(ctx, b, MODE, g->m, stats);

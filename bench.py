@@ -146,7 +146,7 @@ def batch_sources(h, k, seed):
     return cand[(d * np.uint64(len(cand))) >> np.uint64(32)].astype(np.int32)
 
 
-def bench_batched(ctx):
+def bench_batched(ctx, tune):
     """256-source BFS and 64-source SSSP on RMAT s20 as one batched call each,
     against the same sources through plain one-at-a-time launches."""
     res = {}
@@ -154,6 +154,10 @@ def bench_batched(ctx):
         h = A.rmat(20, 16, seed, undirected=und, weight_seed=None if und else seed + 1_000_000)
         g = ctx.upload(h)
         srcs = batch_sources(h, k, seed)
+        tuning = None
+        if tune:
+            kinds = A.TUNE_BFS_BATCH if kind == "bfs" else A.TUNE_SSSP_BATCH
+            tuning = ctx.autotune(g, int(srcs[0]), kinds, k=k)
         dsrc = ctx.alloc(4 * k).upload(srcs)
         dout = ctx.alloc(4 * k * h.m)
         one = ctx.alloc(4 * h.m)
@@ -197,6 +201,7 @@ def bench_batched(ctx):
             "plain_qps": round(k / tp, 1),
             "speedup": round(tp / b, 2),
             "rows_match_plain": ok,
+            "tuning": tuning,
         }
         log(f"batched {kind}: {res[kind]}")
         for x in (dsrc, dout, one):
@@ -244,6 +249,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS/SSSP comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
     ap.add_argument("--serve-requests", type=int, default=20000)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
@@ -269,6 +275,11 @@ def main():
     g = ctx.upload(h, csc=ht)
     del ht
     log(f"CSR + CSC uploaded ({time.time() - t:.1f}s)")
+    tunings = {}
+    if not args.no_tune:
+        t = time.time()
+        tunings["sssp"] = ctx.autotune(g, src, A.TUNE_SSSP)
+        log(f"autotuned sssp ({time.time() - t:.1f}s): {tunings['sssp']}")
     d = ctx.alloc(4 * h.m)
     run = lambda: ctx.sssp_into(g, src, d)  # noqa: E731
     t = time.time()
@@ -331,6 +342,7 @@ def main():
         "bytes_alg": st.bytes_alg,
         "traversal_frac": round(st.bytes_alg / (statistics.median(kms) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "measured_copy_gbps": round(copy_gbps, 1),
+        "tuning": tunings,
         "roofline": roof,
     }
 
@@ -340,6 +352,9 @@ def main():
         log(f"generated RMAT s{BFS_SCALE} undirected: nnz={hb.nnz} ({time.time() - t:.1f}s)")
         del g
         gb = ctx.upload(hb, csc="symmetric")
+        if not args.no_tune:
+            tunings["bfs"] = ctx.autotune(gb, 0, A.TUNE_BFS)
+            log(f"autotuned bfs: {tunings['bfs']}")
         db = ctx.alloc(4 * hb.m)
         runb = lambda: ctx.bfs_into(gb, 0, db)  # noqa: E731
         for _ in range(args.warmup + 1):
@@ -365,7 +380,7 @@ def main():
         log(f"bfs: {ms_b:.3f} ms/step, {out['bfs']['gteps_expanded']} GTEPS (expanded)")
 
     if not args.no_batch:
-        out["batched"] = bench_batched(ctx)
+        out["batched"] = bench_batched(ctx, not args.no_tune)
     if not args.no_serve:
         out["served"] = bench_served(ctx, args.serve_requests)
 

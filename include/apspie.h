@@ -130,19 +130,35 @@ int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable);
 int apspie_ctx_profile_read(apspie_ctx *ctx, int kclass, double *ms, int64_t *launches);
 int apspie_ctx_profile_reset(apspie_ctx *ctx);
 
-/* Direction thresholds of BFS / SSSP on graphs that carry their CSC (the
- * reference always pushes).  bfs_alpha <= 0 turns BFS bottom-up steps off,
- * sssp_pull_ratio <= 0 SSSP pull steps, sssp_bin_ratio <= 0 binned SSSP
- * rounds (those need m <= 2^23). */
+/* Tunable knobs of the traversals (the reference has none: it always pushes
+ * and fixes its launch shapes).  Direction thresholds apply to graphs that
+ * carry their CSC, except binned SSSP rounds (m <= 2^23).  bfs_alpha <= 0
+ * turns BFS bottom-up steps off, sssp_pull_ratio <= 0 SSSP pull steps,
+ * sssp_bin_ratio <= 0 binned SSSP rounds.  Results never depend on them. */
 typedef struct {
     int32_t bfs_alpha;        /* bottom-up once frontier edges * alpha > unexplored edges (default 14) */
     int32_t bfs_beta;         /* stay bottom-up while frontier vertices * beta >= m (default 24) */
     int32_t sssp_pull_ratio;  /* pull once frontier edges * ratio > nnz (default 0: off) */
     int32_t sssp_bin_ratio;   /* binned relaxation once frontier edges * ratio > nnz (default 0: off) */
-    int32_t reserved[4];
+    int32_t expand_wg_per_cu; /* expansion workgroups per compute unit, 1..32 (default 8) */
+    int32_t round_chunk;      /* rounds launched before the first frontier check, 1..32 (default 4) */
+    int32_t batch_chunk;      /* rounds per captured hipGraph of the batched forms, 1..16 (default 4) */
+    int32_t sssp_batch_lanes; /* batched SSSP lanes per edge: 16 (k <= 16 only) or 64; 0 = by k (default) */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);
+
+/* Autotuner: times candidate values of the knobs above on graph g -- BFS
+ * and SSSP from `src` (kinds bit 0 and 1), batched BFS and SSSP from `k`
+ * vertices with out-edges spread over the graph (bits 2 and 3) -- one knob at
+ * a time, keeps the fastest in the context and returns it in *out.  Each
+ * candidate is the median of three timed calls after a warm-up. */
+#define APSPIE_TUNE_BFS 1u
+#define APSPIE_TUNE_SSSP 2u
+#define APSPIE_TUNE_BFS_BATCH 4u
+#define APSPIE_TUNE_SSSP_BATCH 8u
+int apspie_autotune(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t k, uint32_t kinds,
+                    apspie_tuning *out);
 
 /* Device memory helpers for callers without a framework allocator
  * (the reference's cudaMalloc / cudaMemcpy calls in test/<algo>/main.cu). */
