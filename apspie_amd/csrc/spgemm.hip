@@ -1,19 +1,19 @@
 // spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned hash
-// tables (LDS for rows up to 4096 products, global memory beyond).
+// tables (LDS for rows up to 2048 products, global memory beyond).
 //
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
 //                edge-parallel over A with the frontier kernels.
-//   2. bins    rows by ub: <= 32, <= 256, <= 2048, <= 4096 (LDS tables of
+//   2. bins    rows by ub: <= 16, <= 128, <= 1024, <= 2048 (LDS tables of
 //              64 / 512 / 4096 / 8192 slots, one wave .. 512 threads per row)
-//              and > 4096 (global table of 2 * pow2ceil(ub) slots per row).
+//              and > 2048 (global table of 4 * pow2ceil(ub) slots per row).
 //   3. symbolic: insert every product's column; nnz(C(i,:)) = distinct count.
 //   4. row_ptr(C) = exclusive scan (decoupled look-back).
 //   5. numeric: insert (column, a * b) with FP64 accumulation (products of
 //      FP32 inputs are exact in FP64; the sum is rounded to FP32 once), then
 //      write the row out with sorted columns.
 // The table uses an order-preserving home slot, home(c) = c * H / n, with
-// linear probing and no wrap-around (2H slots for at most H keys): a key's
+// linear probing and no wrap-around (2H slots for at most H / 2 keys): a key's
 // slot lies in the cluster (run of occupied slots) holding its home, so all
 // keys of an earlier cluster are smaller and the row comes out sorted after
 // an insertion sort inside each cluster.  Correct for any column
@@ -46,8 +46,11 @@ struct CsrD {
     int m, n;
 };
 
+// A table of S slots has H = S / 2 home slots and takes rows of at most S / 4
+// products, so keys fill at most half of the home range (linear probing with
+// order-preserving homes clusters badly when that range is nearly full).
 constexpr int NBINS = 5;
-constexpr int BIN_UB[NBINS - 1] = {32, 256, 2048, 4096};
+constexpr int BIN_UB[NBINS - 1] = {16, 128, 1024, 2048};
 
 // ub[i] += nnz(B(c,:)) for every edge (i -> c) of A
 struct OpUpper {
@@ -83,19 +86,32 @@ __global__ __launch_bounds__(256) void k_init_ctl(int nblk, uint64_t *status, Co
 __global__ __launch_bounds__(256) void k_bin(int m, const int32_t *ub, int32_t *nnzc, int32_t *lists,
                                              int32_t *counts, unsigned long long *products) {
     unsigned long long mine = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
-        const int u = ub[i];
+    const int lane = threadIdx.x & (WAVE - 1);
+    // whole waves walk the rows together so each bin's slots are claimed with
+    // one atomic per wave (a counter every row hits serialises otherwise)
+    const int m_up = (m + WAVE - 1) / WAVE * WAVE;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m_up; i += gridDim.x * 256) {
+        const int u = i < m ? ub[i] : 0;
         mine += (unsigned long long)u;
-        if (u == 0) {
-            nnzc[i] = 0;
-            continue;
-        }
+        if (i < m && u == 0) nnzc[i] = 0;
         int b = NBINS - 1;
 #pragma unroll
         for (int k = NBINS - 2; k >= 0; --k)
             if (u <= BIN_UB[k]) b = k;
-        const int pos = atomicAdd(&counts[b], 1);
-        lists[int64_t(b) * m + pos] = i;
+        if (u == 0) b = -1;
+#pragma unroll
+        for (int k = 0; k < NBINS; ++k) {
+            const uint64_t ball = __ballot(b == k);
+            if (!ball) continue;
+            int base = 0;
+            if (lane == __ffsll((unsigned long long)ball) - 1) base = atomicAdd(&counts[k], __popcll(ball));
+            base = __shfl(base, __ffsll((unsigned long long)ball) - 1, WAVE);
+            if (b == k) {
+                const int rank = __builtin_amdgcn_mbcnt_hi(uint32_t(ball >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo(uint32_t(ball), 0));
+                lists[int64_t(k) * m + base + rank] = i;
+            }
+        }
     }
     mine = wave_sum(mine);
     if ((threadIdx.x & 63) == 0 && mine) atomicAdd(products, mine);
@@ -231,26 +247,49 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
             s_a[tid] = a;
             const int ne = min(T, ae - jb);
             __syncthreads();
-            for (int t = tid; t < tot; t += T) {
-                int lo = 0, hi = ne - 1;  // last entry whose prefix <= t
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= t) lo = mid;
-                    else hi = mid - 1;
-                }
-                const int q = s_bs[lo] + (t - s_pre[lo]);
-                const int c = B.col[q];
-                if ((unsigned)c >= (unsigned)B.n) continue;
-                const double prod = NUMERIC ? double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f) : 0.0;
-                int s = int((uint64_t(uint32_t(c)) * uint64_t(H)) / uint64_t(uint32_t(B.n)));
-                for (;;) {
-                    const int old = atomicCAS(&key[s], -1, c);
-                    if (old == -1 || old == c) {
-                        if (NUMERIC) atomicAdd(&val[s], prod);
-                        else if (old == -1) atomicAdd(&s_cnt, 1);
-                        break;
+            // U products per thread in flight: their entries come from one
+            // binary search and a forward walk (a thread's products only move
+            // right), their loads are issued before any insertion
+            constexpr int U = 4;
+            int lo = 0;
+            for (int tb = tid; tb < tot; tb += U * T) {
+                int cq[U];
+                double pq[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = tb + u * T;
+                    cq[u] = -1;
+                    pq[u] = 0.0;
+                    if (t < tot) {
+                        if (u == 0) {
+                            int hi = ne - 1;  // last entry whose prefix <= t
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (s_pre[mid] <= t) lo = mid;
+                                else hi = mid - 1;
+                            }
+                        } else {
+                            while (lo + 1 < ne && s_pre[lo + 1] <= t) ++lo;
+                        }
+                        const int q = s_bs[lo] + (t - s_pre[lo]);
+                        cq[u] = B.col[q];
+                        if (NUMERIC) pq[u] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
                     }
-                    ++s;  // < slots: at most H keys, homes < H
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int c = cq[u];
+                    if ((unsigned)c >= (unsigned)B.n) continue;
+                    int s = int((uint64_t(uint32_t(c)) * uint64_t(H)) / uint64_t(uint32_t(B.n)));
+                    for (;;) {
+                        const int old = atomicCAS(&key[s], -1, c);
+                        if (old == -1 || old == c) {
+                            if (NUMERIC) atomicAdd(&val[s], pq[u]);
+                            else if (old == -1) atomicAdd(&s_cnt, 1);
+                            break;
+                        }
+                        ++s;  // < slots: at most H / 2 keys, homes < H
+                    }
                 }
             }
             __syncthreads();
@@ -406,10 +445,10 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             for (int r = 0; r < ng; ++r) {
                 int64_t h = 1;
                 while (h < hub[rows[r]]) h <<= 1;
-                APSPIE_CHECK(2 * h < (int64_t(1) << 31), APSPIE_ELIMIT, "row with too many products");
-                sl[r] = int32_t(2 * h);
+                APSPIE_CHECK(4 * h < (int64_t(1) << 31), APSPIE_ELIMIT, "row with too many products");
+                sl[r] = int32_t(4 * h);  // at most a quarter full, as the LDS tables
                 off[r] = tot;
-                tot += 2 * h;
+                tot += 4 * h;
             }
             goff = (int64_t *)ctx->slot(S_AUX4, size_t(ng) * 8);
             gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);

@@ -102,18 +102,33 @@ __device__ __forceinline__ int bin_of(int deg) {
 __global__ __launch_bounds__(256) void k_items(int pass, int m, const int32_t *__restrict__ rp,
                                                const int64_t *__restrict__ gpre, int32_t *counts, int2 *lists,
                                                const int64_t *bin_base, int32_t *bad) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
-        const int64_t wi = gpre[rp[i + 1]] - gpre[rp[i]];
-        if (wi == 0) continue;
-        if (wi >= (int64_t(1) << 31)) {  // per-row wedge offsets are int32
-            *bad = 1;
-            continue;
+    // whole waves walk the rows together and claim each bin's slots with one
+    // atomic per wave (a counter every row hits serialises otherwise)
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int m_up = (m + WAVE - 1) / WAVE * WAVE;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m_up; i += gridDim.x * 256) {
+        int n = 0, b = -1;
+        if (i < m) {
+            const int64_t wi = gpre[rp[i + 1]] - gpre[rp[i]];
+            if (wi >= (int64_t(1) << 31)) {  // per-row wedge offsets are int32
+                *bad = 1;
+            } else if (wi > 0) {
+                n = int((wi + CW - 1) / CW);
+                b = bin_of(rp[i + 1] - rp[i]);
+            }
         }
-        const int n = int((wi + CW - 1) / CW);
-        const int b = bin_of(rp[i + 1] - rp[i]);
-        const int base = atomicAdd(&counts[(pass ? NB : 0) + b], n);
-        if (pass)
-            for (int c = 0; c < n; ++c) lists[bin_base[b] + base + c] = make_int2(i, c);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const int mine = b == k ? n : 0;
+            const uint32_t incl = wave_incl_scan32(uint32_t(mine), lane);
+            const uint32_t wtot = __shfl(incl, WAVE - 1, WAVE);
+            if (!wtot) continue;
+            int base = 0;
+            if (lane == WAVE - 1) base = atomicAdd(&counts[(pass ? NB : 0) + k], int(wtot));
+            base = __shfl(base, WAVE - 1, WAVE) + int(incl) - mine;
+            if (pass && mine)
+                for (int c = 0; c < n; ++c) lists[bin_base[k] + base + c] = make_int2(i, c);
+        }
     }
 }
 
@@ -179,40 +194,61 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
             __syncthreads();
             const int ne = min(T, ie - eb);
             const int t0 = max(lo, base), t1 = min(hi, base + tot);
-            for (int t = t0 + tid; t < t1; t += T) {
-                int x = 0, y = ne - 1;  // last entry of the block whose prefix <= t
-                while (x < y) {
-                    const int mid = (x + y + 1) >> 1;
-                    if (s_pre[mid] <= t) x = mid;
-                    else y = mid - 1;
-                }
-                const int j = col[s_bs[x] + (t - s_pre[x])];
-                bool hit = false;
-                if (LDSMASK) {
-                    uint32_t s = hslot(j, mask);
-                    for (;;) {
-                        const int kk = s_set[s];
-                        if (kk == j) {
-                            hit = true;
-                            break;
+            // U wedges per thread in flight: their entries come from one
+            // binary search and a forward walk (a thread's wedges only move
+            // right), their column loads are issued together
+            constexpr int U = 4;
+            int x = 0;
+            for (int tb = t0 + tid; tb < t1; tb += U * T) {
+                int j[U];
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    const int t = tb + q * T;
+                    j[q] = -1;
+                    if (t < t1) {
+                        if (q == 0) {
+                            int y = ne - 1;  // last entry of the block whose prefix <= t
+                            while (x < y) {
+                                const int mid = (x + y + 1) >> 1;
+                                if (s_pre[mid] <= t) x = mid;
+                                else y = mid - 1;
+                            }
+                        } else {
+                            while (x + 1 < ne && s_pre[x + 1] <= t) ++x;
                         }
-                        if (kk == -1) break;
-                        s = (s + 1) & mask;
-                    }
-                } else {
-                    int u = ib, v = ie - 1;
-                    while (u <= v) {
-                        const int mid = (u + v) >> 1;
-                        const int val = col[mid];
-                        if (val == j) {
-                            hit = true;
-                            break;
-                        }
-                        if (val < j) u = mid + 1;
-                        else v = mid - 1;
+                        j[q] = col[s_bs[x] + (t - s_pre[x])];
                     }
                 }
-                cnt += hit ? 1u : 0u;
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    if (j[q] < 0) continue;
+                    bool hit = false;
+                    if (LDSMASK) {
+                        uint32_t s = hslot(j[q], mask);
+                        for (;;) {
+                            const int kk = s_set[s];
+                            if (kk == j[q]) {
+                                hit = true;
+                                break;
+                            }
+                            if (kk == -1) break;
+                            s = (s + 1) & mask;
+                        }
+                    } else {
+                        int u = ib, v = ie - 1;
+                        while (u <= v) {
+                            const int mid = (u + v) >> 1;
+                            const int val = col[mid];
+                            if (val == j[q]) {
+                                hit = true;
+                                break;
+                            }
+                            if (val < j[q]) u = mid + 1;
+                            else v = mid - 1;
+                        }
+                    }
+                    cnt += hit ? 1u : 0u;
+                }
             }
             base += tot;
             __syncthreads();

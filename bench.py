@@ -210,6 +210,68 @@ def bench_batched(ctx, tune):
     return res
 
 
+def timed(fn, reps=3):
+    """Median wall time (s) of reps calls after one warm-up, and the last result."""
+    out = fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts), out
+
+
+def bench_matrix_ops(ctx):
+    """Hash SpGEMM C = A * A on the 1M x 1M power-law matrix (mean 8, max
+    4096 nnz per row), triangle counting on the lower triangle of the
+    symmetrized RMAT s20 graph, and MIS on the symmetrized RMAT s21 graph
+    with MT19937 priorities; each against HBM from its algorithmic bytes."""
+    res = {}
+    h = A.powerlaw(1 << 20, 1 << 20, 8.0, 4096, 3003)
+    a = ctx.upload(h)
+
+    def spgemm():
+        c, st = ctx.spgemm_device(a, a)
+        ctx.free_owned(c)
+        return st
+
+    wall, st = timed(spgemm)
+    res["spgemm"] = {
+        "workload": "spgemm_powerlaw_1M_mean8_max4096_AxA",
+        "nnz_a": h.nnz, "products": st.edges, "nnz_c": st.vertices,
+        "ms": round(wall * 1e3, 3), "device_ms": round(st.kernel_ms, 3),
+        "gflops": round(2 * st.edges / st.kernel_ms / 1e6, 2),
+        "bytes_alg": st.bytes_alg,
+        "hbm_frac": round(st.bytes_alg / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    log(f"spgemm: {res['spgemm']}")
+    del a
+    hu = A.rmat(20, 16, 3020, undirected=True)
+    lo = A.strict_lower(hu)
+    gl = ctx.upload(lo)
+    wall, (count, st) = timed(lambda: ctx.tc(gl))
+    res["tc"] = {
+        "workload": "tc_rmat_s20_ef16_lower", "triangles": count, "wedges": st.edges, "nnz_l": lo.nnz,
+        "ms": round(wall * 1e3, 3), "device_ms": round(st.kernel_ms, 3),
+        "bytes_alg": st.bytes_alg,
+        "hbm_frac": round(st.bytes_alg / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+    log(f"tc: {res['tc']}")
+    del gl, lo, hu
+    from apspie_amd.serving import mt19937_stream
+
+    hm = A.rmat(21, 16, 4021, undirected=True)
+    gm = ctx.upload(hm)
+    prio = mt19937_stream(4021, hm.m).astype(np.uint32)
+    wall, (mis, st) = timed(lambda: ctx.mis(gm, prio))
+    res["mis"] = {
+        "workload": "mis_rmat_s21_ef16_undirected_mt19937_prio", "in_set": int(mis.sum()),
+        "rounds": st.iterations, "ms": round(wall * 1e3, 3), "device_ms": round(st.kernel_ms, 3),
+    }
+    log(f"mis: {res['mis']}")
+    return res
+
+
 def bench_served(ctx, n):
     """Served stream (16-graph pool, Poisson 2000 req/s, 50/50 BFS/SSSP)
     against plain one-at-a-time launches of the same requests."""
@@ -248,6 +310,7 @@ def main():
     ap.add_argument("--scale", type=int, default=SSSP_SCALE)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched BFS/SSSP comparison")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream comparison")
+    ap.add_argument("--no-matrix", action="store_true", help="skip the SpGEMM / triangle count / MIS section")
     ap.add_argument("--serve-requests", type=int, default=20000)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--copy-only", action="store_true",
@@ -383,6 +446,8 @@ def main():
         out["batched"] = bench_batched(ctx, not args.no_tune)
     if not args.no_serve:
         out["served"] = bench_served(ctx, args.serve_requests)
+    if not args.no_matrix:
+        out.update(bench_matrix_ops(ctx))
 
     if not args.no_cpu:
         import oracle as O
