@@ -565,7 +565,8 @@ class ServerOpts(C.Structure):
         ("lanes", C.c_int),
         ("max_bfs_batch", C.c_int),
         ("max_sssp_batch", C.c_int),
-        ("reserved", C.c_int),
+        ("plain_bfs_upto", C.c_int),
+        ("plain_sssp_upto", C.c_int),
         ("budget_bytes", C.c_size_t),
     ]
 
@@ -582,6 +583,7 @@ class ServerReport(C.Structure):
         ("mean_ms", C.c_double),
         ("max_ms", C.c_double),
         ("qps", C.c_double),
+        ("trivial", C.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -600,11 +602,14 @@ class Server:
     """apspie_server: batched BFS/SSSP service over device-resident graphs."""
 
     def __init__(self, graphs: Sequence[DeviceCsr], lanes: int = 4, budget_bytes: int = 32 << 30,
-                 max_bfs_batch: int = 256, max_sssp_batch: int = 64, device: int = 0):
+                 max_bfs_batch: int = 256, max_sssp_batch: int = 64, device: int = 0,
+                 plain_bfs_upto: int = 0, plain_sssp_upto: int = 0):
+        """plain_*_upto: batches of at most that many requests run as plain
+        launches (0: the library default, -1: never)."""
         L = lib()
         self._graphs = list(graphs)  # keep the device CSRs alive
         arr = (_DCsr * len(graphs))(*[g.c for g in graphs])
-        opts = ServerOpts(lanes, max_bfs_batch, max_sssp_batch, 0, budget_bytes)
+        opts = ServerOpts(lanes, max_bfs_batch, max_sssp_batch, plain_bfs_upto, plain_sssp_upto, budget_bytes)
         h = C.c_void_p()
         _check(L.apspie_server_create(device, arr, len(graphs), C.byref(opts), C.byref(h)), None, "server_create")
         self.h = h.value

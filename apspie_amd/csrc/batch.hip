@@ -1,10 +1,13 @@
 // batch.hip -- many traversals of one graph at once.
 //
 // Batched BFS (k <= 256 sources): bit-parallel multi-source BFS.  Per vertex
-// W = ceil(k/64) words of visited / next / frontier bits (vertex-major); the
-// union frontier goes through the generic compaction and expansion, and every
-// edge (u -> c) of it serves all sources at once:
+// W = ceil(k/64) words of visited / next / frontier bits (vertex-major, the
+// visited and next words of a vertex side by side in one record); the union
+// frontier goes through the generic compaction and expansion, and every edge
+// (u -> c) of it serves all sources at once:
 //     next[c] |= frontier[u] & ~visited[c]      (64-bit atomicOr per word)
+// skipping the atomic when next[c], read with visited[c], already holds the
+// bits (most edges of a dense round bring bits another edge brought first).
 // The compaction turns next into the new frontier bits, marks them visited and
 // writes depth[s * m + c] = level for each new (s, c).
 //
@@ -31,22 +34,24 @@ namespace apspie {
 namespace trav {
 namespace {
 
+// st[v * 2W + w] = visited word w of v, st[v * 2W + W + w] = next word w
 template <int W>
 struct PickMs {
-    uint64_t *next, *vis, *fr;
+    uint64_t *st, *fr;
     int32_t *depth;
     const int *lvl;
     int m, k;
     __device__ bool pick(int v, uint32_t &) const {
         if (v >= m) return false;
+        uint64_t *vis = st + int64_t(v) * 2 * W, *next = vis + W;
         uint64_t n[W];
         bool any = false;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            uint64_t x = next[int64_t(v) * W + w];
+            uint64_t x = next[w];
             if (x) {
-                next[int64_t(v) * W + w] = 0;
-                x &= ~vis[int64_t(v) * W + w];
+                next[w] = 0;
+                x &= ~vis[w];
             }
             n[w] = x;
             any |= x != 0;
@@ -56,7 +61,7 @@ struct PickMs {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             fr[int64_t(v) * W + w] = n[w];
-            if (n[w]) vis[int64_t(v) * W + w] |= n[w];
+            if (n[w]) vis[w] |= n[w];
             uint64_t b = n[w];
             while (b) {
                 const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
@@ -74,16 +79,17 @@ struct OpMs {
     static constexpr bool kVert = true, kVal = false;
     static constexpr int kBatch = W >= 2 ? 2 : IPT;
     struct Pre {
-        uint64_t f[W], v[W];
+        uint64_t f[W], v[W];  // frontier words of u; visited | next words of c
     };
-    const uint64_t *fr, *vis;
-    uint64_t *next;
+    const uint64_t *fr;
+    uint64_t *st;
     __device__ Pre load(int c, int, uint32_t, int u) const {
         Pre p;
+        const uint64_t *rec = st + int64_t(c) * 2 * W;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             p.f[w] = fr[int64_t(u) * W + w];
-            p.v[w] = vis[int64_t(c) * W + w];
+            p.v[w] = rec[w] | rec[W + w];
         }
         return p;
     }
@@ -91,19 +97,16 @@ struct OpMs {
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             const uint64_t x = p.f[w] & ~p.v[w];
-            if (x) atomicOr((unsigned long long *)&next[int64_t(c) * W + w], (unsigned long long)x);
+            if (x) atomicOr((unsigned long long *)&st[int64_t(c) * 2 * W + W + w], (unsigned long long)x);
         }
     }
 };
 
 template <int W>
-__global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *srcs, uint64_t *next, uint64_t *vis,
-                                                 int32_t *depth, int nblk, uint64_t *status, Control *C) {
+__global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *srcs, uint64_t *st, int32_t *depth,
+                                                 int nblk, uint64_t *status, Control *C) {
     const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, stride = int64_t(gridDim.x) * 256;
-    for (int64_t i = gtid; i < int64_t(m) * W; i += stride) {
-        next[i] = 0;
-        vis[i] = 0;
-    }
+    for (int64_t i = gtid; i < int64_t(m) * 2 * W; i += stride) st[i] = 0;
     for (int64_t i = gtid; i < int64_t(m) * k; i += stride) depth[i] = -1;
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
     if (gtid == 0) {
@@ -113,9 +116,9 @@ __global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *sr
 }
 
 template <int W>
-__global__ void k_ms_seed(int k, const int32_t *srcs, uint64_t *next) {
+__global__ void k_ms_seed(int k, const int32_t *srcs, uint64_t *st) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < k) atomicOr((unsigned long long *)&next[int64_t(srcs[s]) * W + s / 64], 1ull << (s % 64));
+    if (s < k) atomicOr((unsigned long long *)&st[int64_t(srcs[s]) * 2 * W + W + s / 64], 1ull << (s % 64));
 }
 
 // ---------------------------------------------------------------------------
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
                                                        const int32_t *__restrict__ col_ind,
                                                        const float *__restrict__ val, uint32_t *dist,
                                                        const uint64_t *__restrict__ fr, uint64_t *chg) {
-    constexpr int NW = WG / WAVE, SEG = TILE / NW, E = WAVE / K, U = 4;
+    constexpr int NW = WG / WAVE, SEG = TILE / NW, E = WAVE / K, U = K == 64 ? 8 : 4;
     static_assert(WAVE % K == 0, "K divides the wave");
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];
@@ -318,15 +321,14 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
                apspie_stats *stats) {
     const Bufs b = get_bufs(ctx, g->m, g->nnz);
     const size_t words = size_t(g->m) * W;
-    uint64_t *next = (uint64_t *)ctx->slot(S_AUX0, words * 8);
-    uint64_t *vis = (uint64_t *)ctx->slot(S_AUX1, words * 8);
+    uint64_t *st = (uint64_t *)ctx->slot(S_AUX0, words * 2 * 8);
     uint64_t *fr = (uint64_t *)ctx->slot(S_AUX2, words * 8);
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * std::max(W, k), 256), 8192u));
-    k_ms_init<W><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, next, vis, depth, b.nblk, b.status, b.C);
-    k_ms_seed<W><<<ceil_div(k, 256), 256, 0, ctx->stream>>>(k, srcs, next);
-    const PickMs<W> pk{next, vis, fr, depth, &b.C->cur_level, g->m, k};
-    const OpMs<W> op{fr, vis, next};
+    k_ms_init<W><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, st, depth, b.nblk, b.status, b.C);
+    k_ms_seed<W><<<ceil_div(k, 256), 256, 0, ctx->stream>>>(k, srcs, st);
+    const PickMs<W> pk{st, fr, depth, &b.C->cur_level, g->m, k};
+    const OpMs<W> op{fr, st};
     auto compact = [&] {
         k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
                                                                b.status, b.C);
@@ -335,7 +337,7 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
     k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
                                                            b.status, b.C);
     APSPIE_HIP(hipGetLastError());
-    const std::string key = graph_key("msbfs", g, b, {next, vis, fr, depth, b.status, b.tiles}, W * 1000 + k);
+    const std::string key = graph_key("msbfs", g, b, {st, fr, depth, b.status, b.tiles}, W * 1000 + k);
     run_rounds(ctx, b, key, g->m + 2, [&] {
         k_expand<OpMs<W>><<<b.expand_grid, WG, 0, ctx->stream>>>(op, -1, g->m, b.C, queue_of(b), g->col_ind);
         compact();

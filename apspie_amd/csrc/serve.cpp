@@ -1,6 +1,9 @@
 // serve.cpp -- serving loop over a device-resident graph pool.
 //
 // Requests (BFS or SSSP from one source on one pooled graph) enter a FIFO.
+// A batch of at most plain_bfs_upto / plain_sssp_upto requests runs as plain
+// single-source launches instead (one source: the batched kernels' lanes
+// would idle).
 // `lanes` worker threads each own a context (HIP stream + scratch) and loop:
 // take the oldest pending request, gather every other pending request for the
 // same (graph, kind) up to the batch limit while the device-memory budget
@@ -9,6 +12,12 @@
 // Requests that arrive while batches run are picked up by the next lane that
 // frees, and batches of different lanes overlap on the device (one stream
 // each), so the device is kept busy without waiting for a full batch.
+//
+// A source without out-edges reaches only itself: its row (depth 0 / distance
+// 0 at the source, -1 / APSPIE_SSSP_INF elsewhere) is known without a
+// traversal.  Such requests take no lane of a batch and no device memory; the
+// worker that meets them in the queue answers them on the host.  A host bitmap
+// per graph (one bit per vertex: has out-edges) decides this at submission.
 //
 // Latency = completion time - arrival time on one steady clock.  A request
 // whose footprint exceeds the whole budget is refused at submission.
@@ -41,15 +50,18 @@ struct Request {
     void *out;
     double arrive, done_at = 0;
     size_t footprint;
-    bool done = false;
+    bool done = false, trivial = false;
     int status = 0;
 };
+
+constexpr size_t TRIVIAL_PER_PASS = 4096;  // host-answered requests per worker pass
 
 }  // namespace
 
 struct apspie_server {
     apspie_server_opts opts{};
     std::vector<apspie_csr> graphs;
+    std::vector<std::vector<uint64_t>> has_out;  // per graph: bit v = row v non-empty
     int device = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -62,9 +74,10 @@ struct apspie_server {
     std::vector<std::thread> workers;
     std::vector<double> lat_ms;
     double first_arrive = 0, last_done = 0;
-    int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_reqs = 0;
+    int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_reqs = 0, trivial = 0;
     std::string err;
 
+    bool live(int g, int32_t v) const { return (has_out[g][size_t(v) >> 6] >> (v & 63)) & 1u; }
     size_t footprint(int kind, int g) const {
         const size_t m = size_t(graphs[g].m);
         // output row + share of the batch state (vertex-major masks or
@@ -72,7 +85,39 @@ struct apspie_server {
         return kind == 0 ? m * 5 + 4096 : m * 13 + 4096;
     }
     void worker(int lane);
+    void answer_trivial(std::vector<Request *> &rs);
 };
+
+void apspie_server::answer_trivial(std::vector<Request *> &rs) {
+    for (Request *r : rs) {
+        if (!r->out) continue;
+        const size_t m = size_t(graphs[r->graph].m);
+        if (r->kind == APSPIE_REQ_BFS) {
+            int32_t *o = (int32_t *)r->out;
+            std::fill(o, o + m, -1);
+            o[r->src] = 0;
+        } else {
+            float *o = (float *)r->out;
+            std::fill(o, o + m, APSPIE_SSSP_INF);
+            o[r->src] = 0.0f;
+        }
+    }
+    const double t = now_s();
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (Request *r : rs) {
+            r->done = true;
+            r->done_at = t;
+            r->status = APSPIE_OK;
+            completed++;
+            trivial++;
+            lat_ms.push_back((t - r->arrive) * 1e3);
+            last_done = std::max(last_done, t);
+        }
+    }
+    cv_done.notify_all();
+    rs.clear();
+}
 
 void apspie_server::worker(int lane) {
     apspie_ctx *ctx = nullptr;
@@ -84,7 +129,7 @@ void apspie_server::worker(int lane) {
     void *d_srcs = nullptr, *d_out = nullptr;
     size_t out_cap = 0;
     apspie_malloc(ctx, &d_srcs, 256 * 4);
-    std::vector<Request *> batch;
+    std::vector<Request *> batch, easy;
     std::vector<int32_t> srcs;
     for (;;) {
         batch.clear();
@@ -92,27 +137,51 @@ void apspie_server::worker(int lane) {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || !pending.empty(); });
             if (stop && pending.empty()) break;
-            // oldest request decides (graph, kind); gather the same kind in
-            // arrival order while budget and batch limit allow
-            Request *head = pending.front();
-            const int limit = head->kind == 0 ? opts.max_bfs_batch : opts.max_sssp_batch;
-            for (auto it = pending.begin(); it != pending.end() && (int)batch.size() < limit;) {
+            // in arrival order: take host-answered requests of any graph, and
+            // let the oldest traversal request decide (graph, kind) and gather
+            // more of that kind while budget and batch limit allow
+            Request *head = nullptr;
+            int limit = 0;
+            bool full = false;
+            for (auto it = pending.begin(); it != pending.end();) {
                 Request *r = *it;
-                if (r->kind == head->kind && r->graph == head->graph) {
-                    if (used + r->footprint > opts.budget_bytes) break;  // admit no more now
-                    used += r->footprint;
-                    batch.push_back(r);
-                    it = pending.erase(it);
-                } else {
-                    ++it;
+                if (r->trivial) {
+                    if (easy.size() < TRIVIAL_PER_PASS) {
+                        easy.push_back(r);
+                        it = pending.erase(it);
+                        continue;
+                    }
+                } else if (!full) {
+                    if (!head) {
+                        head = r;
+                        limit = r->kind == APSPIE_REQ_BFS ? opts.max_bfs_batch : opts.max_sssp_batch;
+                    }
+                    if (r->kind == head->kind && r->graph == head->graph) {
+                        if (used + r->footprint > opts.budget_bytes) {
+                            full = true;  // admit no more now
+                        } else {
+                            used += r->footprint;
+                            batch.push_back(r);
+                            it = pending.erase(it);
+                            full = (int)batch.size() == limit;
+                            continue;
+                        }
+                    }
                 }
+                if (full && easy.size() == TRIVIAL_PER_PASS) break;
+                ++it;
             }
-            if (batch.empty()) {
+            if (batch.empty() && easy.empty()) {
                 // budget full: wait for another lane to release memory
                 cv_work.wait(lk, [&] { return stop || inflight == 0 || used == 0; });
                 continue;
             }
-            inflight++;
+            if (!batch.empty()) inflight++;
+        }
+        if (!easy.empty()) answer_trivial(easy);
+        if (batch.empty()) {
+            cv_work.notify_all();
+            continue;
         }
         const int kind = batch[0]->kind, gi = batch[0]->graph;
         const apspie_csr &g = graphs[gi];
@@ -128,7 +197,15 @@ void apspie_server::worker(int lane) {
             rc = apspie_malloc(ctx, &d_out, need);
             if (rc == APSPIE_OK) out_cap = need;
         }
-        if (rc == APSPIE_OK) {
+        // a batch of very few sources runs faster as plain launches
+        const int plain_upto = kind == APSPIE_REQ_BFS ? opts.plain_bfs_upto : opts.plain_sssp_upto;
+        if (rc == APSPIE_OK && k <= plain_upto) {
+            for (int i = 0; i < k && rc == APSPIE_OK; ++i) {
+                void *row = (char *)d_out + size_t(i) * g.m * 4;
+                rc = kind == APSPIE_REQ_BFS ? apspie_bfs(ctx, &g, srcs[i], (int32_t *)row, nullptr)
+                                            : apspie_sssp(ctx, &g, srcs[i], (float *)row, nullptr);
+            }
+        } else if (rc == APSPIE_OK) {
             apspie_stats st;
             rc = kind == 0 ? apspie_bfs_batch(ctx, &g, (const int32_t *)d_srcs, k, (int32_t *)d_out, &st)
                            : apspie_sssp_batch(ctx, &g, (const int32_t *)d_srcs, k, (float *)d_out, &st);
@@ -179,11 +256,15 @@ int apspie_server_create(int device, const apspie_csr *graphs, int ngraphs, cons
     o.lanes = 4;
     o.max_bfs_batch = 256;
     o.max_sssp_batch = 64;
+    o.plain_bfs_upto = 3;
+    o.plain_sssp_upto = 1;
     o.budget_bytes = size_t(32) << 30;
     if (opts) {
         if (opts->lanes > 0) o.lanes = opts->lanes;
         if (opts->max_bfs_batch > 0) o.max_bfs_batch = std::min(opts->max_bfs_batch, 256);
         if (opts->max_sssp_batch > 0) o.max_sssp_batch = std::min(opts->max_sssp_batch, 64);
+        if (opts->plain_bfs_upto != 0) o.plain_bfs_upto = opts->plain_bfs_upto;
+        if (opts->plain_sssp_upto != 0) o.plain_sssp_upto = opts->plain_sssp_upto;
         if (opts->budget_bytes > 0) o.budget_bytes = opts->budget_bytes;
     }
     s->opts = o;
@@ -193,6 +274,23 @@ int apspie_server_create(int device, const apspie_csr *graphs, int ngraphs, cons
             return APSPIE_EINVAL;
         }
         s->graphs.push_back(graphs[i]);
+    }
+    // which vertices have out-edges, from each graph's row offsets
+    if (hipSetDevice(device) != hipSuccess) {
+        delete s;
+        return APSPIE_EHIP;
+    }
+    std::vector<int32_t> rp;
+    for (const apspie_csr &g : s->graphs) {
+        rp.resize(size_t(g.m) + 1);
+        if (hipMemcpy(rp.data(), g.row_ptr, rp.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            delete s;
+            return APSPIE_EHIP;
+        }
+        std::vector<uint64_t> bits((size_t(g.m) + 63) / 64, 0);
+        for (int v = 0; v < g.m; ++v)
+            if (rp[v + 1] > rp[v]) bits[size_t(v) >> 6] |= 1ull << (v & 63);
+        s->has_out.push_back(std::move(bits));
     }
     for (int l = 0; l < o.lanes; ++l) s->workers.emplace_back([s, l] { s->worker(l); });
     *out = s;
@@ -209,7 +307,8 @@ int apspie_server_submit(apspie_server *s, int kind, int graph, int32_t src, voi
     r->src = src;
     r->out = host_out;
     r->arrive = arrival >= 0 ? arrival : now_s();
-    r->footprint = s->footprint(kind, graph);
+    r->trivial = !s->live(graph, src);
+    r->footprint = r->trivial ? 0 : s->footprint(kind, graph);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (r->footprint > s->opts.budget_bytes) {
@@ -257,6 +356,7 @@ int apspie_server_stats(apspie_server *s, apspie_server_report *st) {
     st->completed = s->completed;
     st->rejected = s->rejected;
     st->batches = s->batches;
+    st->trivial = s->trivial;
     st->mean_batch = s->batches ? double(s->batched_reqs) / double(s->batches) : 0.0;
     if (!s->lat_ms.empty()) {
         std::vector<double> v = s->lat_ms;
@@ -278,7 +378,7 @@ int apspie_server_reset_stats(apspie_server *s) {
     if (!s) return APSPIE_EINVAL;
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_ms.clear();
-    s->submitted = s->completed = s->rejected = s->batches = s->batched_reqs = 0;
+    s->submitted = s->completed = s->rejected = s->batches = s->batched_reqs = s->trivial = 0;
     s->first_arrive = s->last_done = 0;
     return APSPIE_OK;
 }

@@ -66,6 +66,9 @@ def run_served(server: Server, reqs: List[Req], open_loop: bool, outs: Optional[
     t0 = now() + 0.05
 
     def submitter():
+        delay = t0 - now()  # the stream starts at t0 in both modes
+        if delay > 0:
+            time.sleep(delay)
         for i, r in enumerate(reqs):
             arr = t0 + r.arrival if open_loop else t0
             if open_loop:

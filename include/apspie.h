@@ -307,20 +307,25 @@ int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_src
  * served in batches by `lanes` worker threads, each with its own stream, so
  * batches overlap on the device and requests that arrive meanwhile join the
  * next batch a lane forms.  Admission is by device-memory footprint against
- * budget_bytes.  All entry points are thread-safe. */
+ * budget_bytes.  A request whose source has no out-edges needs no traversal
+ * and is answered on the host without a lane or device memory.  All entry
+ * points are thread-safe. */
 typedef struct apspie_server apspie_server;
 typedef struct {
     int lanes;            /* concurrent batches (default 4) */
     int max_bfs_batch;    /* <= 256 (default 256) */
     int max_sssp_batch;   /* <= 64 (default 64) */
-    int reserved;
+    int plain_bfs_upto;   /* BFS batches of at most this many requests run as plain single-source
+                             launches, which are faster there (default 3; < 0: never) */
+    int plain_sssp_upto;  /* the same for SSSP (default 1; < 0: never) */
     size_t budget_bytes;  /* device memory for in-flight requests (default 32 GiB) */
 } apspie_server_opts;
 typedef struct {
     int64_t submitted, completed, rejected, batches;
     double mean_batch;
     double p50_ms, p99_ms, mean_ms, max_ms;
-    double qps; /* completed / (last completion - first arrival) */
+    double qps;      /* completed / (last completion - first arrival) */
+    int64_t trivial; /* answered on the host: the source has no out-edges */
 } apspie_server_report;
 
 #define APSPIE_REQ_BFS 0

@@ -22,15 +22,19 @@ def solo(ctx, graphs, r):
     return ctx.bfs(g, r.src)[0] if r.kind == A.REQ_BFS else ctx.sssp(g, r.src)[0]
 
 
-def test_served_equals_solo_in_a_mix(ctx, pool):
+@pytest.mark.parametrize("plain", [0, -1, 1000])
+def test_served_equals_solo_in_a_mix(ctx, pool, plain):
+    """plain: batches up to that size as plain launches (0 default, -1
+    never, 1000 always)."""
     hosts, graphs = pool
     reqs = serving.make_stream([h.m for h in hosts], n=600, rate=50000.0, seed=7)
     outs = {i: np.empty(hosts[r.graph].m, np.int32 if r.kind == A.REQ_BFS else np.float32)
             for i, r in enumerate(reqs)}
-    with A.Server(graphs, lanes=3) as srv:
+    with A.Server(graphs, lanes=3, plain_bfs_upto=plain, plain_sssp_upto=plain) as srv:
         rep = serving.run_served(srv, reqs, open_loop=True, outs=outs)
     assert rep["completed"] == len(reqs) and rep["rejected"] == 0
     assert rep["batches"] < len(reqs)  # requests were batched together
+    assert 0 < rep["trivial"] < len(reqs)  # sources without out-edges answered on the host
     assert rep["p50_ms"] <= rep["p99_ms"]
     for i, r in enumerate(reqs):
         assert np.array_equal(outs[i], solo(ctx, graphs, r)), i
@@ -43,6 +47,26 @@ def test_alone_equals_mix(ctx, pool):
     with A.Server(graphs, lanes=1) as srv:
         srv.wait(srv.submit(r.kind, r.graph, r.src, alone))
     assert np.array_equal(alone, solo(ctx, graphs, r))
+
+
+def test_sources_without_out_edges(ctx, pool):
+    hosts, graphs = pool
+    h = hosts[2]
+    dead = np.nonzero(h.degrees() == 0)[0][:3]
+    live = np.nonzero(h.degrees() > 0)[0][:3]
+    assert len(dead) == 3 and len(live) == 3
+    with A.Server(graphs, lanes=2) as srv:
+        outs, ts = [], []
+        for kind in (A.REQ_BFS, A.REQ_SSSP):
+            for v in list(dead) + list(live):
+                o = np.empty(h.m, np.int32 if kind == A.REQ_BFS else np.float32)
+                outs.append((kind, int(v), o))
+                ts.append(srv.submit(kind, 2, int(v), o))
+        for t in ts:
+            srv.wait(t)
+        assert srv.report()["trivial"] == 6
+    for kind, v, o in outs:
+        assert np.array_equal(o, solo(ctx, graphs, serving.Req(kind, 2, v, 0.0))), (kind, v)
 
 
 def test_admission_by_footprint(ctx, pool):
