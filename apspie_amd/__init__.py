@@ -79,6 +79,7 @@ class Tuning(C.Structure):
         ("round_chunk", C.c_int32),
         ("batch_chunk", C.c_int32),
         ("sssp_batch_lanes", C.c_int32),
+        ("sssp_delta", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

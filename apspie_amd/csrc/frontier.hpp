@@ -62,6 +62,11 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
 //   int direction(const Ctl &c, unsigned long long edges_so_far, int prev_pull)
 //                                             1: run the next round as a pull
 //                                             step (written into Ctl::pull)
+//   uint32_t defer_key(int v, const Pre &)    a changed vertex left out of the
+//                                             frontier to wait (its distance
+//                                             bits; all ones: not waiting)
+//   uint32_t raised(uint32_t dmin)            the threshold that admits the
+//                                             smallest waiting distance
 // A negative level_arg means "device-driven": the round number is read from
 // C->cur_level and the compaction advances it when it completes, so a
 // captured sequence of launches can be replayed unchanged (hipGraph).
@@ -89,6 +94,10 @@ template <class P, class = void>
 struct has_direction : std::false_type {};
 template <class P>
 struct has_direction<P, std::void_t<decltype(&P::direction)>> : std::true_type {};
+template <class P, class = void>
+struct has_defer : std::false_type {};
+template <class P>
+struct has_defer<P, std::void_t<decltype(&P::defer_key)>> : std::true_type {};
 
 template <class Pick>
 __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, int nblk, int always,
@@ -99,6 +108,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     __shared__ int s_bid;
     __shared__ uint64_t s_wsum[NW];
     __shared__ uint32_t s_new[NW];
+    __shared__ uint32_t s_def[NW], s_dmin[NW];
     __shared__ uint64_t s_prefix;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     // every block reads the round before it takes a dynamic id, so the last
@@ -109,7 +119,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
                                      : __hip_atomic_load(&C->cur_level, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = level & 1;
     Ctl *next = &C->ctl[par];
-    if (!always && C->ctl[par ^ 1].nF == 0) {
+    if (!always && C->ctl[par ^ 1].nF == 0 && C->ctl[par ^ 1].pull != DIR_REPICK) {
         if (blockIdx.x == 0 && tid == 0) *next = Ctl{0, 0, 0, 0};
         return;
     }
@@ -119,7 +129,11 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     // state for the launch after this one (its previous user has finished)
     if (tid == 0) status[(par ^ 1) * nblk + bid] = 0;
     if (bid == 0 && tid < SPREAD) {
-        if (tid == 0) C->blk_counter[par ^ 1] = 0;
+        if (tid == 0) {
+            C->blk_counter[par ^ 1] = 0;
+            C->ndefer[par ^ 1] = 0;
+            C->dmin[par ^ 1] = ~0u;
+        }
         C->nnew[par ^ 1][tid] = 0;
     }
 
@@ -131,10 +145,16 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     uint32_t newbits = 0;
     uint64_t myball = 0;
     uint32_t vv[ROUNDS];
+    uint32_t ndef = 0, dmin = ~0u;
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const int v = vbase + r * WAVE + lane;
         uint32_t payload = 0;
+        if constexpr (has_defer<Pick>::value) {
+            const uint32_t key = pk.defer_key(v, pre[r]);
+            ndef += key != ~0u;
+            dmin = min(dmin, key);
+        }
         const bool isnew = PP::pick(pk, v, pre[r], payload);
         const uint64_t ball = __ballot(isnew);
         if (lane == r) myball = ball;
@@ -171,9 +191,16 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         }
     }
     const uint32_t wcnt = wave_sum32(lane_cnt), wes = wave_sum32(lane_es), wnew = wave_sum32(lane_new);
+    if constexpr (has_defer<Pick>::value) {
+        ndef = wave_sum32(ndef);
+#pragma unroll
+        for (int d = WAVE / 2; d >= 1; d >>= 1) dmin = min(dmin, (uint32_t)__shfl_xor((int)dmin, d, WAVE));
+    }
     if (lane == 0) {
         s_wsum[wid] = (uint64_t(wcnt) << 31) | wes;
         s_new[wid] = wnew;
+        s_def[wid] = ndef;
+        s_dmin[wid] = dmin;
     }
     __syncthreads();
     uint64_t wpre = 0, total = 0;
@@ -185,6 +212,23 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         bnew += s_new[w];
     }
     if (wid == 0) {
+        if constexpr (has_defer<Pick>::value) {
+            // waiting vertices are counted before this workgroup publishes its
+            // prefix, so the last workgroup sees every count
+            if (lane == 0) {
+                uint32_t bdef = 0, bmin = ~0u;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    bdef += s_def[w];
+                    bmin = min(bmin, s_dmin[w]);
+                }
+                if (bdef) {
+                    atomicAdd(&C->ndefer[par], bdef);
+                    atomicMin(&C->dmin[par], bmin);
+                }
+            }
+            __threadfence();
+        }
         const uint64_t p = lookback_exclusive(status + par * nblk, bid, total, lane);
         if (lane == 0) s_prefix = p;
     }
@@ -230,6 +274,15 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
             const unsigned long long before = atomicAdd(&C->acc.edges, (unsigned long long)c.E);
             c.pull = 0;
             if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, before + c.E, C->ctl[par ^ 1].pull);
+            if constexpr (has_defer<Pick>::value) {
+                __threadfence();
+                // nothing left below the threshold: raise it past the smallest
+                // waiting distance and pick again next round
+                if (c.nF == 0 && __hip_atomic_load(&C->ndefer[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    C->thr = pk.raised(__hip_atomic_load(&C->dmin[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+                    c.pull = DIR_REPICK;
+                }
+            }
             *next = c;
             if (level < LOG_ROUNDS) C->log[level] = c;
             if (level_arg < 0) C->cur_level = level + 1;

@@ -146,7 +146,8 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     if (!ctx || !t) return APSPIE_EINVAL;
     if (t->expand_wg_per_cu < 1 || t->expand_wg_per_cu > 32 || t->round_chunk < 1 || t->round_chunk > 32 ||
         t->batch_chunk < 1 || t->batch_chunk > 16 ||
-        (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64)) {
+        (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64) ||
+        t->sssp_delta < 0 || t->sssp_delta > (1 << 24)) {
         ctx->err = "tuning knob out of range";
         return APSPIE_EINVAL;
     }
@@ -159,6 +160,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.round_chunk = t->round_chunk;
     ctx->tune.batch_chunk = t->batch_chunk;
     ctx->tune.sssp_batch_lanes = t->sssp_batch_lanes;
+    ctx->tune.sssp_delta = t->sssp_delta;
     return APSPIE_OK;
 }
 
@@ -173,6 +175,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->round_chunk = ctx->tune.round_chunk;
     t->batch_chunk = ctx->tune.batch_chunk;
     t->sssp_batch_lanes = ctx->tune.sssp_batch_lanes;
+    t->sssp_delta = ctx->tune.sssp_delta;
     return APSPIE_OK;
 }
 

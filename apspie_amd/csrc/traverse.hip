@@ -90,10 +90,16 @@ struct PickSssp {
     int64_t nnz;
     int ratio;      // pull when the frontier's edges * ratio exceed nnz
     int bin_ratio;  // binned relaxation when the frontier's edges * bin_ratio exceed nnz (0: off)
+    // near/far: a changed vertex whose distance is at or past *thr waits
+    // (keeps its old snapshot, so it stays changed) until the frontier below
+    // the threshold runs dry and the threshold rises by delta (0: off)
+    const uint32_t *thr;
+    float delta;
     struct Pre {
         uint32_t d, p;
     };
     __device__ Pre load(int v) const { return v < m ? Pre{dist[v], prev[v]} : Pre{0u, 0u}; }
+    __device__ uint32_t threshold() const { return __hip_atomic_load(thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ bool pick(int v, const Pre &x, uint32_t &payload) const {
         if (v >= m) return false;
         uint32_t d = x.d;
@@ -102,11 +108,17 @@ struct PickSssp {
             dist[v] = 0u;
         }
         payload = d;
-        const bool isnew = d < x.p;
+        const bool isnew = d < x.p && d < threshold();
         if (isnew) prev[v] = d;
         if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
         return isnew;
     }
+    __device__ uint32_t defer_key(int v, const Pre &x) const {
+        if (!(delta > 0.0f) || v >= m) return ~0u;
+        const uint32_t d = v == seed ? 0u : x.d;
+        return d < x.p && d >= threshold() ? d : ~0u;
+    }
+    __device__ uint32_t raised(uint32_t dmin) const { return __float_as_uint(__uint_as_float(dmin) + delta); }
     __device__ void on_new(int) const {}
     __device__ int direction(const Ctl &c, unsigned long long, int) const {
         if (bin_ratio > 0 && int64_t(c.E) * bin_ratio > nnz) return DIR_BINNED;
@@ -190,21 +202,21 @@ __global__ __launch_bounds__(WG) void k_tile_rows(int m, const int32_t *__restri
     }
 }
 
-// SSSP pull: y[v] = min(P[v], min over in-edges (u, v) with u in the frontier
-// of P[u] + w), P = the distances at the round start (every vertex's P equals
-// its live distance X after a compaction).  Q[u] holds P[u] for frontier
-// vertices and NOT_IN_FRONTIER for the rest, so an in-edge costs one random
-// 4-byte read.  Equal TILE-edge tiles of the CSC; row starts are marked in
+// SSSP pull: X[v] = min(X[v], min over in-edges (u, v) with u in the frontier
+// of Q[u] + w), Q[u] = u's distance when the compaction put it in the
+// frontier (NOT_IN_FRONTIER for the rest), so an in-edge costs one random
+// 4-byte read.  (X[v] is compared live: a vertex waiting past the near/far
+// threshold already holds a distance below its snapshot.)  Equal TILE-edge tiles of the CSC; row starts are marked in
 // LDS and a max-scan gives every slot the start slot of its row; a segmented
 // wave min per 64 slots, then one LDS atomicMin per segment, then one write
 // per row: a plain store for rows inside the tile, an atomicMin for the (at
 // most two) rows it shares with other tiles.  Only the improved vertices are
-// written, so X - P marks the next frontier exactly as a push round leaves it.
+// written, so X below the snapshot marks the next frontier exactly as a push
+// round leaves it.
 __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz, const Control *C,
                                                    const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
                                                    const float *__restrict__ cv, const int32_t *__restrict__ trows,
-                                                   const uint32_t *__restrict__ Q,
-                                                   const uint32_t *__restrict__ P, uint32_t *X) {
+                                                   const uint32_t *__restrict__ Q, uint32_t *X) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     constexpr uint32_t NONE = NOT_IN_FRONTIER;
     __shared__ int32_t s_rid[TILE];   // row (relative to the tile's first row) starting at a slot, 0 if none
@@ -267,7 +279,7 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
             const uint32_t b = s_best[s];
             if (b == NONE) continue;
             const int v = r0 + s_rid[s];
-            if (b < P[v]) {
+            if (b < X[v]) {  // a row inside the tile has no other writer
                 if (cp[v] >= e0 && cp[v + 1] <= e0 + elim)
                     X[v] = b;
                 else
@@ -280,7 +292,7 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
 
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
-                                             uint64_t *status, Control *C, uint32_t *bin_totals) {
+                                             uint64_t *status, Control *C, uint32_t *bin_totals, uint32_t thr0) {
     const int64_t gtid = int64_t(blockIdx.x) * WG + threadIdx.x;
     const int64_t stride = int64_t(gridDim.x) * WG;
     const int64_t mpad = int64_t(nblk) * CHUNK;
@@ -301,6 +313,8 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
         for (int64_t i = gtid; i < 2 * MAX_BUCKETS; i += stride) bin_totals[i] = 0;
     if (gtid == 0) {
         Control z{};
+        z.thr = thr0;
+        z.dmin[0] = z.dmin[1] = ~0u;
         *C = z;
     }
 }
@@ -412,8 +426,12 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     const unsigned apply_grid = unsigned(nb + g->nnz / PART_ENTRIES + 1);
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     ctx->prof_begin(APSPIE_KCLASS_OTHER);
+    // near/far threshold: the first delta of distance (none for BFS or delta 0)
+    const float delta = MODE == MODE_SSSP ? float(ctx->tune.sssp_delta) : 0.0f;
+    uint32_t thr0 = ~0u;
+    if (delta > 0.0f) memcpy(&thr0, &delta, 4);
     k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE, g->m, b.nblk, depth, dist, b.prev, b.flags,
-                                                       b.visited, b.status, b.C, bin_tot);
+                                                       b.visited, b.status, b.C, bin_tot, thr0);
     if (pull && MODE == MODE_SSSP)
         k_tile_rows<<<vert_grid, WG, 0, ctx->stream>>>(g->m, g->csc_col_ptr, b.trows);
     ctx->prof_end();
@@ -426,7 +444,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         else
             launch_compact(ctx, b, g,
                            PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
-                                    binned ? ctx->tune.sssp_bin_ratio : 0},
+                                    binned ? ctx->tune.sssp_bin_ratio : 0, &b.C->thr, delta},
                            level, level == 0);
     };
     auto expand = [&](int level) {
@@ -451,8 +469,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                                                           b.visited, b.fbits, b.flags);
         else
             k_pull_relax<<<pull_grid, WG, 0, ctx->stream>>>(level, g->m, nnz32, b.C, g->csc_col_ptr,
-                                                            g->csc_row_ind, g->csc_val, b.trows, fval, b.prev,
-                                                            dist);
+                                                            g->csc_row_ind, g->csc_val, b.trows, fval, dist);
         ctx->prof_end();
     };
     compact(0, src);
@@ -460,18 +477,20 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     // traversals end within 6-12 rounds, and a round launched past the end
     // costs its kernels' launch
     int level = 1, chunk = ctx->tune.round_chunk, checks = 0;
-    volatile int32_t *host_nf = (volatile int32_t *)ctx->pinned;
+    volatile Ctl *host_ctl = (volatile Ctl *)ctx->pinned;
     for (;;) {
         for (int i = 0; i < chunk; ++i, ++level) {
             expand(level);
             compact(level, -1);
         }
         APSPIE_HIP(hipGetLastError());
-        APSPIE_HIP(hipMemcpyAsync((void *)host_nf, &b.C->ctl[(level - 1) & 1].nF, 4, hipMemcpyDeviceToHost,
+        APSPIE_HIP(hipMemcpyAsync((void *)host_ctl, &b.C->ctl[(level - 1) & 1], sizeof(Ctl), hipMemcpyDeviceToHost,
                                   ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-        if (*host_nf == 0) break;
-        APSPIE_CHECK(level <= g->m + 2, APSPIE_ENOCONV, "traversal did not converge");
+        // done: an empty frontier with nothing waiting past a threshold
+        if (host_ctl->nF == 0 && host_ctl->pull != DIR_REPICK) break;
+        // every round settles a vertex or, repicking, admits a waiting one
+        APSPIE_CHECK(level <= 2 * int64_t(g->m) + 4, APSPIE_ENOCONV, "traversal did not converge");
         ++checks;
         chunk = checks <= 2 ? 2 : std::min(chunk * 2, 32);
     }
@@ -529,7 +548,7 @@ int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, flo
         APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         ctx->prof_begin(APSPIE_KCLASS_OTHER);
         k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE_STEP, g->m, b.nblk, nullptr, y, nullptr,
-                                                           b.flags, b.visited, b.status, b.C, nullptr);
+                                                           b.flags, b.visited, b.status, b.C, nullptr, ~0u);
         ctx->prof_end();
         launch_compact(ctx, b, g, PickStep{d_x, g->m}, 0, 1);
         launch_expand(ctx, b, g, OpRelax{g->val, y}, 1);

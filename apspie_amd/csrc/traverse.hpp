@@ -31,6 +31,8 @@ enum {
     DIR_PUSH = 0,    // k_expand over the queued frontier
     DIR_PULL = 1,    // pull step over the CSC
     DIR_BINNED = 2,  // SSSP: binned relaxation (k_bin_relax + k_apply_relax)
+    DIR_REPICK = 3,  // SSSP near/far: nothing to expand; changed vertices wait at or past the
+                     // threshold, which was raised, and the next compaction picks again
 };
 
 // Counters every compaction workgroup adds to are spread over SPREAD slots
@@ -54,7 +56,12 @@ struct Control {
     unsigned int nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
     Acc acc;
     int cur_level;  // device-driven rounds: the round the next compaction runs
-    int pad2;
+    // SSSP near/far: a changed vertex enters the frontier only while its
+    // distance is below thr (float bits; all ones: no threshold); the others
+    // wait, counted (ndefer) with their smallest distance (dmin) by parity
+    uint32_t thr;
+    unsigned int ndefer[2];
+    uint32_t dmin[2];
     Ctl log[LOG_ROUNDS];  // the descriptor each compaction wrote, by round
 };
 

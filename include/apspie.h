@@ -144,6 +144,9 @@ typedef struct {
     int32_t round_chunk;      /* rounds launched before the first frontier check, 1..32 (default 4) */
     int32_t batch_chunk;      /* rounds per captured hipGraph of the batched forms, 1..16 (default 4) */
     int32_t sssp_batch_lanes; /* batched SSSP lanes per edge: 16 (k <= 16 only) or 64; 0 = by k (default) */
+    int32_t sssp_delta;       /* SSSP near/far step: a changed vertex waits while its distance is at or
+                                 past a threshold that rises by this much when the frontier below it
+                                 runs dry; 0 = off (plain frontier Bellman-Ford), 0..2^24 */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

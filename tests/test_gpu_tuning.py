@@ -39,6 +39,8 @@ def sources(h, k, seed):
     dict(expand_wg_per_cu=1), dict(expand_wg_per_cu=32), dict(round_chunk=1), dict(round_chunk=32),
     dict(batch_chunk=1), dict(batch_chunk=16), dict(sssp_batch_lanes=64), dict(sssp_batch_lanes=16),
     dict(bfs_alpha=1, bfs_beta=1000), dict(sssp_pull_ratio=2, sssp_bin_ratio=0), dict(sssp_bin_ratio=2),
+    dict(sssp_delta=1), dict(sssp_delta=40), dict(sssp_delta=40, sssp_pull_ratio=2),
+    dict(sssp_delta=40, sssp_bin_ratio=2),
 ])
 def test_every_knob_setting_is_exact(tuned, graph, knobs):
     ctx = tuned
@@ -66,7 +68,8 @@ def test_every_knob_setting_is_exact(tuned, graph, knobs):
 def test_out_of_range_knobs_are_refused(tuned):
     ctx = tuned
     before = ctx.tuning()
-    for bad in (dict(expand_wg_per_cu=0), dict(round_chunk=33), dict(batch_chunk=0), dict(sssp_batch_lanes=32)):
+    for bad in (dict(expand_wg_per_cu=0), dict(round_chunk=33), dict(batch_chunk=0), dict(sssp_batch_lanes=32),
+                dict(sssp_delta=-1), dict(sssp_delta=(1 << 24) + 1)):
         with pytest.raises(A.ApspieError):
             ctx.set_tuning(**bad)
         assert ctx.tuning() == before
