@@ -9,7 +9,8 @@
 //      by nnz(L(i,:)): <= 256 (one wave per item, 512-slot LDS set), <= 4096
 //      (256 threads, 8192-slot LDS set), larger (binary search of the sorted
 //      row in global memory).
-//   3. Per item: the mask row L(i,:) goes into an LDS hash set; the item's
+//   3. Per item: the mask row L(i,:) goes into an LDS hash set behind a bit
+//      filter (most probes are of absent columns and stop there); the item's
 //      first edge comes from a binary search of gpre; its wedges (k, j) are
 //      enumerated in blocks of entries with a block scan of nnz(L(k,:)) and a
 //      binary search for each wedge's k; each j found in the set closes a
@@ -134,11 +135,18 @@ __global__ __launch_bounds__(256) void k_items(int pass, int m, const int32_t *_
 
 __device__ __forceinline__ uint32_t hslot(int j, uint32_t mask) { return (uint32_t(j) * 2654435761u) & mask; }
 
+// Bit filter in front of the LDS hash set: a row whose set has sz slots sets
+// one of 32 sz bits per entry (index from the high bits of another
+// multiplicative hash), so at most 1 in 64 probes of an absent column passes
+// it; most wedges do not close a triangle and end at one LDS read.
+__device__ __forceinline__ uint32_t fslot(int j, int shift) { return (uint32_t(j) * 0x85ebca6bu) >> shift; }
+
 template <int T, int S, bool LDSMASK>
 __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int nitems, const int32_t *__restrict__ rp,
                                              const int32_t *__restrict__ col, const int64_t *__restrict__ gpre,
                                              unsigned long long *total) {
     __shared__ int32_t s_set[LDSMASK ? S : 1];
+    __shared__ uint32_t s_filt[LDSMASK ? S : 1];
     __shared__ int32_t s_bs[T];
     __shared__ int32_t s_pre[T];
     __shared__ int32_t s_tmp[T / WAVE + 1];
@@ -151,16 +159,21 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
         const int wi = int(gpre[ie] - g0);
         const int lo = c * CW, hi = min(lo + CW, wi);
         uint32_t mask = 0;
+        int fshift = 0;  // filter of 32 sz bits: index = top log2(32 sz) bits of fslot's product
         if (LDSMASK) {
             int sz = 64;
             while (sz < 2 * deg) sz <<= 1;
             mask = uint32_t(sz - 1);
+            fshift = __clz(sz) - 4;  // 32 - log2(32 sz)
             for (int s = tid; s < sz; s += T) s_set[s] = -1;
+            for (int s = tid; s < sz; s += T) s_filt[s] = 0u;
             __syncthreads();
             for (int p = ib + tid; p < ie; p += T) {
                 const int j = col[p];
                 uint32_t s = hslot(j, mask);
                 while (atomicCAS(&s_set[s], -1, j) != -1) s = (s + 1) & mask;
+                const uint32_t f = fslot(j, fshift);
+                atomicOr(&s_filt[f >> 5], 1u << (f & 31));
             }
             __syncthreads();
         }
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
             // U wedges per thread in flight: their entries come from one
             // binary search and a forward walk (a thread's wedges only move
             // right), their column loads are issued together
-            constexpr int U = 4;
+            constexpr int U = 8;
             int x = 0;
             for (int tb = t0 + tid; tb < t1; tb += U * T) {
                 int j[U];
@@ -224,6 +237,8 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
                     if (j[q] < 0) continue;
                     bool hit = false;
                     if (LDSMASK) {
+                        const uint32_t f = fslot(j[q], fshift);
+                        if (!((s_filt[f >> 5] >> (f & 31)) & 1u)) continue;  // absent
                         uint32_t s = hslot(j[q], mask);
                         for (;;) {
                             const int kk = s_set[s];
