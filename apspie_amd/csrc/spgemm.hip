@@ -190,6 +190,22 @@ __device__ __forceinline__ double ld_val(const double *val, int s) {
     return val[s];
 }
 
+// insertion sort of the keys (and their values) in [a, b) of an LDS table
+__device__ __forceinline__ void sort_run(int32_t *key, double *val, int a, int b) {
+    for (int x = a + 1; x < b; ++x) {
+        const int c = key[x];
+        const double v = val[x];
+        int w = x;
+        while (w > a && key[w - 1] > c) {
+            key[w] = key[w - 1];
+            val[w] = val[w - 1];
+            --w;
+        }
+        key[w] = c;
+        val[w] = v;
+    }
+}
+
 // One row per workgroup iteration.  T threads, S table slots (LDS) or the
 // row's own slice of a global table.
 template <int T, int S, bool NUMERIC, bool GLOBAL>
@@ -204,6 +220,9 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
     __shared__ float s_a[T];
     __shared__ int32_t s_tmp[T / WAVE + 1];
     __shared__ int32_t s_cnt;
+    constexpr int TS = (GLOBAL && NUMERIC) ? 4 * T : 1;  // global tables stream through LDS in tiles
+    __shared__ int32_t s_tk[TS];
+    __shared__ double s_tv[TS];
     const int tid = threadIdx.x;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
         const int i = rows[ri];
@@ -299,6 +318,69 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
             __syncthreads();
             continue;
         }
+        if (GLOBAL) {
+            // a global table streams through LDS in tiles of 4 T slots
+            // (coalesced loads); a thread owns 4 consecutive slots of a tile
+            // and emits the clusters that start in them, finishing a cluster
+            // that runs past the tile from global memory
+            int run = crp[i];
+            for (int t0 = 0; t0 < slots; t0 += TS) {
+                for (int s = tid; s < TS; s += T) {
+                    const int q = t0 + s;
+                    s_tk[s] = q < slots ? ld_key<GLOBAL>(key, q) : -1;
+                    s_tv[s] = q < slots ? ld_val<GLOBAL>(val, q) : 0.0;
+                }
+                const int left = (t0 > 0 && tid == 0) ? ld_key<GLOBAL>(key, t0 - 1) : -1;
+                __syncthreads();
+                auto key_at = [&](int q) { return q - t0 < TS ? s_tk[q - t0] : ld_key<GLOBAL>(key, q); };
+                auto starts = [&](int s) {  // slot t0 + s holds the first key of a cluster
+                    return t0 + s < slots && s_tk[s] != -1 && (s > 0 ? s_tk[s - 1] : left) == -1;
+                };
+                int cnt = 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (starts(tid * 4 + r))
+                        for (int q = t0 + tid * 4 + r; q < slots && key_at(q) != -1; ++q) ++cnt;
+                int tot;
+                int out = run + block_excl_scan<T>(cnt, s_tmp, tot);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int s0 = tid * 4 + r;
+                    if (!starts(s0)) continue;
+                    int s1 = s0;
+                    while (s1 < TS && s_tk[s1] != -1) ++s1;
+                    if (s1 < TS || t0 + TS == slots) {
+                        // the cluster ends inside the tile: sort it in LDS (its
+                        // slots are this thread's alone), then write it out
+                        sort_run(s_tk, s_tv, s0, s1);
+                        for (int a = s0; a < s1; ++a, ++out) {
+                            ccol[out] = s_tk[a];
+                            cval[out] = float(s_tv[a]);
+                        }
+                        continue;
+                    }
+                    const int o0 = out;
+                    for (int q = t0 + s0; q < slots; ++q) {
+                        const int c = key_at(q);
+                        if (c == -1) break;
+                        // insertion into the sorted run [o0, out)
+                        const float v = float(q - t0 < TS ? s_tv[q - t0] : ld_val<GLOBAL>(val, q));
+                        int w = out;
+                        while (w > o0 && ccol[w - 1] > c) {
+                            ccol[w] = ccol[w - 1];
+                            cval[w] = cval[w - 1];
+                            --w;
+                        }
+                        ccol[w] = c;
+                        cval[w] = v;
+                        ++out;
+                    }
+                }
+                run += tot;
+                __syncthreads();
+            }
+            continue;
+        }
         // extraction: each thread owns a segment of slots and emits the
         // clusters that start in it (a cluster running in from the left
         // belongs to the previous segment)
@@ -321,26 +403,18 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
         int tot;
         int out = crp[i] + block_excl_scan<T>(cnt, s_tmp, tot);
         for (int q = p; q < se;) {
-            if (ld_key<GLOBAL>(key, q) == -1) {
+            if (key[q] == -1) {
                 ++q;
                 continue;
             }
-            const int o0 = out;
-            while (q < slots) {
-                const int c = ld_key<GLOBAL>(key, q);
-                if (c == -1) break;
-                // insertion into the sorted run [o0, out)
-                const float v = float(ld_val<GLOBAL>(val, q));
-                int w = out;
-                while (w > o0 && ccol[w - 1] > c) {
-                    ccol[w] = ccol[w - 1];
-                    cval[w] = cval[w - 1];
-                    --w;
-                }
-                ccol[w] = c;
-                cval[w] = v;
-                ++out;
-                ++q;
+            // the cluster [q0, q) is this thread's alone: sort it in LDS,
+            // then write it out
+            const int q0 = q;
+            while (q < slots && key[q] != -1) ++q;
+            sort_run(key, val, q0, q);
+            for (int a = q0; a < q; ++a, ++out) {
+                ccol[out] = key[a];
+                cval[out] = float(val[a]);
             }
         }
         __syncthreads();
