@@ -223,6 +223,9 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
     constexpr int TS = (GLOBAL && NUMERIC) ? 4 * T : 1;  // global tables stream through LDS in tiles
     __shared__ int32_t s_tk[TS];
     __shared__ double s_tv[TS];
+    // A global table outlives its kernel: the counting pass already
+    // accumulates the values, and the numeric pass only extracts the row.
+    constexpr bool VALS = NUMERIC || GLOBAL, INSERT = !(NUMERIC && GLOBAL);
     const int tid = threadIdx.x;
     for (int ri = blockIdx.x; ri < nrows; ri += gridDim.x) {
         const int i = rows[ri];
@@ -235,84 +238,86 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
             slots = gslots[ri];
         }
         const int H = slots / 2;
-        for (int s = tid; s < slots; s += T) {
-            if (GLOBAL) {
-                __hip_atomic_store(key + s, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (NUMERIC) __hip_atomic_store(val + s, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                key[s] = -1;
-                if (NUMERIC) val[s] = 0.0;
-            }
-        }
-        if (tid == 0) s_cnt = 0;
-        __syncthreads();
-        const int ab = A.rp[i], ae = A.rp[i + 1];
-        for (int jb = ab; jb < ae; jb += T) {
-            const int j = jb + tid;
-            int L = 0, bs = 0;
-            float a = 0.f;
-            if (j < ae) {
-                const int k = A.col[j];
-                if ((unsigned)k < (unsigned)B.m) {
-                    a = A.val ? A.val[j] : 1.0f;
-                    bs = B.rp[k];
-                    L = B.rp[k + 1] - bs;
+        if (INSERT) {
+            for (int s = tid; s < slots; s += T) {
+                if (GLOBAL) {
+                    __hip_atomic_store(key + s, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (VALS) __hip_atomic_store(val + s, 0.0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    key[s] = -1;
+                    if (VALS) val[s] = 0.0;
                 }
             }
-            int tot;
-            const int pre = block_excl_scan<T>(L, s_tmp, tot);
-            s_bs[tid] = bs;
-            s_pre[tid] = pre;
-            s_a[tid] = a;
-            const int ne = min(T, ae - jb);
+            if (tid == 0) s_cnt = 0;
             __syncthreads();
-            // U products per thread in flight: their entries come from one
-            // binary search and a forward walk (a thread's products only move
-            // right), their loads are issued before any insertion
-            constexpr int U = 4;
-            int lo = 0;
-            for (int tb = tid; tb < tot; tb += U * T) {
-                int cq[U];
-                double pq[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int t = tb + u * T;
-                    cq[u] = -1;
-                    pq[u] = 0.0;
-                    if (t < tot) {
-                        if (u == 0) {
-                            int hi = ne - 1;  // last entry whose prefix <= t
-                            while (lo < hi) {
-                                const int mid = (lo + hi + 1) >> 1;
-                                if (s_pre[mid] <= t) lo = mid;
-                                else hi = mid - 1;
+            const int ab = A.rp[i], ae = A.rp[i + 1];
+            for (int jb = ab; jb < ae; jb += T) {
+                const int j = jb + tid;
+                int L = 0, bs = 0;
+                float a = 0.f;
+                if (j < ae) {
+                    const int k = A.col[j];
+                    if ((unsigned)k < (unsigned)B.m) {
+                        a = A.val ? A.val[j] : 1.0f;
+                        bs = B.rp[k];
+                        L = B.rp[k + 1] - bs;
+                    }
+                }
+                int tot;
+                const int pre = block_excl_scan<T>(L, s_tmp, tot);
+                s_bs[tid] = bs;
+                s_pre[tid] = pre;
+                s_a[tid] = a;
+                const int ne = min(T, ae - jb);
+                __syncthreads();
+                // U products per thread in flight: their entries come from one
+                // binary search and a forward walk (a thread's products only move
+                // right), their loads are issued before any insertion
+                constexpr int U = 4;
+                int lo = 0;
+                for (int tb = tid; tb < tot; tb += U * T) {
+                    int cq[U];
+                    double pq[U];
+    #pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int t = tb + u * T;
+                        cq[u] = -1;
+                        pq[u] = 0.0;
+                        if (t < tot) {
+                            if (u == 0) {
+                                int hi = ne - 1;  // last entry whose prefix <= t
+                                while (lo < hi) {
+                                    const int mid = (lo + hi + 1) >> 1;
+                                    if (s_pre[mid] <= t) lo = mid;
+                                    else hi = mid - 1;
+                                }
+                            } else {
+                                while (lo + 1 < ne && s_pre[lo + 1] <= t) ++lo;
                             }
-                        } else {
-                            while (lo + 1 < ne && s_pre[lo + 1] <= t) ++lo;
+                            const int q = s_bs[lo] + (t - s_pre[lo]);
+                            cq[u] = B.col[q];
+                            if (VALS) pq[u] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
                         }
-                        const int q = s_bs[lo] + (t - s_pre[lo]);
-                        cq[u] = B.col[q];
-                        if (NUMERIC) pq[u] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
+                    }
+    #pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const int c = cq[u];
+                        if ((unsigned)c >= (unsigned)B.n) continue;
+                        int s = int((uint64_t(uint32_t(c)) * uint64_t(H)) / uint64_t(uint32_t(B.n)));
+                        for (;;) {
+                            const int old = atomicCAS(&key[s], -1, c);
+                            if (old == -1 || old == c) {
+                                if (VALS) atomicAdd(&val[s], pq[u]);
+                                if (!NUMERIC && old == -1) atomicAdd(&s_cnt, 1);
+                                break;
+                            }
+                            ++s;  // < slots: at most H / 2 keys, homes < H
+                        }
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const int c = cq[u];
-                    if ((unsigned)c >= (unsigned)B.n) continue;
-                    int s = int((uint64_t(uint32_t(c)) * uint64_t(H)) / uint64_t(uint32_t(B.n)));
-                    for (;;) {
-                        const int old = atomicCAS(&key[s], -1, c);
-                        if (old == -1 || old == c) {
-                            if (NUMERIC) atomicAdd(&val[s], pq[u]);
-                            else if (old == -1) atomicAdd(&s_cnt, 1);
-                            break;
-                        }
-                        ++s;  // < slots: at most H / 2 keys, homes < H
-                    }
-                }
+                __syncthreads();
             }
-            __syncthreads();
-        }
+        }  // INSERT
         if (!NUMERIC) {
             if (tid == 0) nnzc[i] = s_cnt;
             __syncthreads();
