@@ -82,30 +82,61 @@ __global__ __launch_bounds__(256) void k_init_ctl(int nblk, uint64_t *status, Co
     }
 }
 
-// rows -> bin lists; rows without products get nnz 0 directly
+// rows -> bin lists; rows without products get nnz 0 directly.  Each
+// workgroup takes a contiguous chunk of rows: it counts its rows per bin in
+// LDS, claims each bin's range with one global atomic, then walks the chunk
+// again placing rows through LDS cursors (counters every wave hit serialise
+// otherwise).
+__device__ __forceinline__ int bin_of_ub(int u) {
+    if (u == 0) return -1;
+    int b = NBINS - 1;
+#pragma unroll
+    for (int k = NBINS - 2; k >= 0; --k)
+        if (u <= BIN_UB[k]) b = k;
+    return b;
+}
+
 __global__ __launch_bounds__(256) void k_bin(int m, const int32_t *ub, int32_t *nnzc, int32_t *lists,
                                              int32_t *counts, unsigned long long *products) {
-    unsigned long long mine = 0;
+    __shared__ int s_cnt[NBINS], s_cur[NBINS];
+    __shared__ unsigned long long s_prod;
     const int lane = threadIdx.x & (WAVE - 1);
-    // whole waves walk the rows together so each bin's slots are claimed with
-    // one atomic per wave (a counter every row hits serialises otherwise)
-    const int m_up = (m + WAVE - 1) / WAVE * WAVE;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < m_up; i += gridDim.x * 256) {
-        const int u = i < m ? ub[i] : 0;
+    const int chunk = (m + gridDim.x - 1) / gridDim.x;
+    const int r0 = min(m, int(blockIdx.x) * chunk), r1 = min(m, r0 + chunk);
+    if (threadIdx.x < NBINS) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_prod = 0;
+    __syncthreads();
+    unsigned long long mine = 0;
+    for (int i = r0 + threadIdx.x; i < r1; i += 256) {
+        const int u = ub[i];
         mine += (unsigned long long)u;
-        if (i < m && u == 0) nnzc[i] = 0;
-        int b = NBINS - 1;
+        const int b = bin_of_ub(u);
+        if (b < 0) nnzc[i] = 0;
 #pragma unroll
-        for (int k = NBINS - 2; k >= 0; --k)
-            if (u <= BIN_UB[k]) b = k;
-        if (u == 0) b = -1;
+        for (int k = 0; k < NBINS; ++k) {
+            const uint64_t ball = __ballot(b == k);
+            if (ball && lane == __ffsll((unsigned long long)ball) - 1) atomicAdd(&s_cnt[k], __popcll(ball));
+        }
+    }
+    mine = wave_sum(mine);
+    if (lane == 0 && mine) atomicAdd(&s_prod, mine);
+    __syncthreads();
+    if (threadIdx.x < NBINS) {
+        const int n = s_cnt[threadIdx.x];
+        s_cur[threadIdx.x] = n ? atomicAdd(&counts[threadIdx.x], n) : 0;
+    }
+    if (threadIdx.x == 0 && s_prod) atomicAdd(products, s_prod);
+    __syncthreads();
+    for (int i = r0 + threadIdx.x; i < r1; i += 256) {
+        const int b = bin_of_ub(ub[i]);
 #pragma unroll
         for (int k = 0; k < NBINS; ++k) {
             const uint64_t ball = __ballot(b == k);
             if (!ball) continue;
+            const int lead = __ffsll((unsigned long long)ball) - 1;
             int base = 0;
-            if (lane == __ffsll((unsigned long long)ball) - 1) base = atomicAdd(&counts[k], __popcll(ball));
-            base = __shfl(base, __ffsll((unsigned long long)ball) - 1, WAVE);
+            if (lane == lead) base = atomicAdd(&s_cur[k], __popcll(ball));
+            base = __shfl(base, lead, WAVE);
             if (b == k) {
                 const int rank = __builtin_amdgcn_mbcnt_hi(uint32_t(ball >> 32),
                                                            __builtin_amdgcn_mbcnt_lo(uint32_t(ball), 0));
@@ -113,8 +144,12 @@ __global__ __launch_bounds__(256) void k_bin(int m, const int32_t *ub, int32_t *
             }
         }
     }
-    mine = wave_sum(mine);
-    if ((threadIdx.x & 63) == 0 && mine) atomicAdd(products, mine);
+}
+
+// out[r] = x[idx[r]]
+__global__ __launch_bounds__(256) void k_gather(const int32_t *__restrict__ idx, int n, const int32_t *__restrict__ x,
+                                                int32_t *out) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) out[r] = x[idx[r]];
 }
 
 // exclusive scan of n int32 into out[0..n], out[n] = total (single pass)
@@ -495,7 +530,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         trav::k_expand<OpUpper><<<fb.expand_grid, WG, 0, ctx->stream>>>(OpUpper{b->row_ptr, ub}, 1, a->n, fb.C,
                                                                           trav::queue_of(fb), a->col_ind);
         // 2. bins
-        k_bin<<<g256, 256, 0, ctx->stream>>>(m, ub, nnzc, lists, counts, products);
+        k_bin<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 4096), (unsigned)ctx->cu_count * 2)), 256, 0,
+                ctx->stream>>>(m, ub, nnzc, lists, counts, products);
         APSPIE_HIP(hipGetLastError());
         Plan P;
         P.lists = lists;
@@ -511,26 +547,26 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         double *gvals = nullptr;
         if (P.counts[4]) {
             const int ng = P.counts[4];
-            std::vector<int32_t> rows(ng), ubs(ng);
-            APSPIE_HIP(hipMemcpyAsync(rows.data(), lists + 4 * int64_t(m), size_t(ng) * 4, hipMemcpyDeviceToHost,
-                                      ctx->stream));
-            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-            std::vector<int32_t> hub(size_t(m) + 1);
-            APSPIE_HIP(hipMemcpyAsync(hub.data(), ub, size_t(m) * 4, hipMemcpyDeviceToHost, ctx->stream));
+            // the product bounds of these rows only (gathered into the slot
+            // that later holds their table sizes)
+            gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
+            k_gather<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(
+                lists + 4 * int64_t(m), ng, ub, gslots);
+            std::vector<int32_t> gub(ng);
+            APSPIE_HIP(hipMemcpyAsync(gub.data(), gslots, size_t(ng) * 4, hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
             std::vector<int64_t> off(ng);
             std::vector<int32_t> sl(ng);
             int64_t tot = 0;
             for (int r = 0; r < ng; ++r) {
                 int64_t h = 1;
-                while (h < hub[rows[r]]) h <<= 1;
+                while (h < gub[r]) h <<= 1;
                 APSPIE_CHECK(4 * h < (int64_t(1) << 31), APSPIE_ELIMIT, "row with too many products");
                 sl[r] = int32_t(4 * h);  // at most a quarter full, as the LDS tables
                 off[r] = tot;
                 tot += 4 * h;
             }
             goff = (int64_t *)ctx->slot(S_AUX4, size_t(ng) * 8);
-            gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
             gkeys = (int32_t *)ctx->slot(S_AUX6, size_t(tot) * 4);
             gvals = (double *)ctx->slot(S_AUX7, size_t(tot) * 8);
             APSPIE_HIP(hipMemcpyAsync(goff, off.data(), size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
