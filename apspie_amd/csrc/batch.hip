@@ -9,7 +9,9 @@
 // skipping the atomic when next[c], read with visited[c], already holds the
 // bits (most edges of a dense round bring bits another edge brought first).
 // The compaction turns next into the new frontier bits, marks them visited and
-// writes depth[s * m + c] = level for each new (s, c).
+// writes depth[s * m + c] = level for each new (s, c): per 64-vertex slice of
+// a wave, one store instruction per source with a new vertex in the slice,
+// so each store covers consecutive vertices of one depth row.
 //
 // Batched SSSP (k <= 64 sources): synchronous-round Bellman-Ford over the
 // union frontier with vertex-major distances dist[v * K + s], so the probes of
@@ -57,21 +59,44 @@ struct PickMs {
             any |= x != 0;
         }
         if (!any) return false;
-        const int level = *(volatile const int *)lvl;
 #pragma unroll
         for (int w = 0; w < W; ++w) {
             fr[int64_t(v) * W + w] = n[w];
             if (n[w]) vis[w] |= n[w];
-            uint64_t b = n[w];
-            while (b) {
-                const int s = w * 64 + __ffsll((unsigned long long)b) - 1;
-                if (s < k) depth[int64_t(s) * m + v] = level;
-                b &= b - 1;
-            }
         }
         return true;
     }
     __device__ void on_new(int) const {}
+    // depths of the new (source, vertex) pairs of the wave's vertices
+    // vbase + 64 r + lane, r < rounds (lane r holds round r's ballot of picked
+    // vertices): the wave ORs the new-source words of a 64-vertex slice and
+    // stores, per source present, the level at the lanes holding it
+    __device__ void on_balls(uint64_t ball, int vbase, int lane, int rounds) const {
+        const int level = *(volatile const int *)lvl;
+        for (int r = 0; r < rounds; ++r) {
+            const uint64_t picked = __shfl(ball, r, WAVE);
+            if (!picked) continue;
+            const int v = vbase + r * WAVE + lane;
+            const bool mine = (picked >> lane) & 1ull;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint64_t nw = mine ? fr[int64_t(v) * W + w] : 0ull;
+                uint64_t any = nw;
+#pragma unroll
+                for (int d = 1; d < WAVE; d <<= 1) {
+                    const uint32_t lo = __shfl_xor(uint32_t(any), d, WAVE);
+                    const uint32_t hi = __shfl_xor(uint32_t(any >> 32), d, WAVE);
+                    any |= (uint64_t(hi) << 32) | lo;
+                }
+                while (any) {  // wave-uniform
+                    const int b = __ffsll((unsigned long long)any) - 1;
+                    const int src = w * 64 + b;
+                    if (((nw >> b) & 1ull) && src < k) depth[int64_t(src) * m + v] = level;
+                    any &= any - 1;
+                }
+            }
+        }
+    }
 };
 
 template <int W>
