@@ -221,7 +221,33 @@ def timed(fn, reps=3):
     return statistics.median(ts), out
 
 
-def bench_matrix_ops(ctx):
+def cpu_port(fn, sample):
+    """Times one call of an oracle function (single-threaded C, restating the
+    reference's CPU path) on the host: (seconds, result, record)."""
+    t0 = time.perf_counter()
+    out = fn()
+    dt = time.perf_counter() - t0
+    rec = {"cores": 1, "kind": "port", "sample": sample, "seconds": round(dt, 3)}
+    return dt, out, rec
+
+
+def ocsr(h):
+    import oracle as O
+
+    return O.Csr(h.m, h.n, h.row_ptr, h.col_ind, h.val)
+
+
+def tc_sample_rows(lo, wedges):
+    """First rows of L whose wedges reach about `wedges` (a bounded CPU sample)."""
+    rl = np.diff(lo.row_ptr).astype(np.int64)
+    per_entry = rl[lo.col_ind]
+    csum = np.concatenate([[0], np.cumsum(per_entry)])
+    row_w = csum[lo.row_ptr[1:]] - csum[lo.row_ptr[:-1]]
+    r1 = int(np.searchsorted(np.cumsum(row_w), wedges)) + 1
+    return min(r1, lo.m)
+
+
+def bench_matrix_ops(ctx, cpu=True):
     """Hash SpGEMM C = A * A on the 1M x 1M power-law matrix (mean 8, max
     4096 nnz per row), triangle counting on the lower triangle of the
     symmetrized RMAT s20 graph, and MIS on the symmetrized RMAT s21 graph
@@ -244,7 +270,18 @@ def bench_matrix_ops(ctx):
         "bytes_alg": st.bytes_alg,
         "hbm_frac": round(st.bytes_alg / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-    log(f"spgemm: {res['spgemm']}")
+    if cpu:
+        import oracle as O
+
+        c_gpu, _ = ctx.spgemm(a, a)
+        dt, (c_cpu, prods), rec = cpu_port(lambda: O.spgemm(ocsr(h), ocsr(h)),
+                                           "the whole product, row-wise Gustavson in double (the cuSPARSE "
+                                           "csrgemm oracle of src/spgemm.cuh:344-445 restated)")
+        same = bool(np.array_equal(c_gpu.row_ptr, c_cpu.row_ptr) and np.array_equal(c_gpu.col_ind, c_cpu.col_ind)
+                    and np.allclose(c_gpu.val, c_cpu.val, rtol=1e-5, atol=0))
+        rec.update(value=round(prods / dt / 1e9, 4), unit="G products/s", matches_gpu=same)
+        res["spgemm"]["cpu"] = rec
+        log(f"cpu spgemm: {rec}")
     del a
     hu = A.rmat(20, 16, 3020, undirected=True)
     lo = A.strict_lower(hu)
@@ -256,7 +293,17 @@ def bench_matrix_ops(ctx):
         "bytes_alg": st.bytes_alg,
         "hbm_frac": round(st.bytes_alg / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
-    log(f"tc: {res['tc']}")
+    if cpu:
+        import oracle as O
+
+        r1 = tc_sample_rows(lo, 1.5e9)
+        dt, (tri, w), rec = cpu_port(lambda: O.tc_rows(ocsr(lo), 0, r1),
+                                     "")
+        rec.update(sample=f"mask rows 0..{r1 - 1} of L ({w} of {st.edges} wedges), merge intersection "
+                          f"(the masked count of src/tc.cuh:168-199 restated)",
+                   value=round(w / dt / 1e9, 4), unit="G wedges/s", matches_gpu=None)
+        res["tc"]["cpu"] = rec
+        log(f"cpu tc: {rec}")
     del gl, lo, hu
     from apspie_amd.serving import mt19937_stream
 
@@ -268,7 +315,15 @@ def bench_matrix_ops(ctx):
         "workload": "mis_rmat_s21_ef16_undirected_mt19937_prio", "in_set": int(mis.sum()),
         "rounds": st.iterations, "ms": round(wall * 1e3, 3), "device_ms": round(st.kernel_ms, 3),
     }
-    log(f"mis: {res['mis']}")
+    if cpu:
+        import oracle as O
+
+        dt, ref, rec = cpu_port(lambda: O.mis_greedy(ocsr(hm), prio),
+                                "the whole graph, sequential greedy MIS in descending (priority, -id) order "
+                                "(SimpleReferenceMis, test/mis/main.cu:26-88, in priority order)")
+        rec.update(value=round(hm.m / dt / 1e6, 3), unit="M vertices/s", matches_gpu=bool(np.array_equal(ref, mis)))
+        res["mis"]["cpu"] = rec
+        log(f"cpu mis: {rec}")
     return res
 
 
@@ -441,13 +496,23 @@ def main():
             "roofline": roof_b,
         }
         log(f"bfs: {ms_b:.3f} ms/step, {out['bfs']['gteps_expanded']} GTEPS (expanded)")
+        if not args.no_cpu:
+            import oracle as O
+
+            dt, ref, rec = cpu_port(lambda: O.bfs(ocsr(hb), 0),
+                                    f"one full FIFO BFS from vertex 0 over the same s{BFS_SCALE} graph "
+                                    "(SimpleReferenceBfs, src/testBfs.cpp:6-72)")
+            rec.update(value=round(ref[2] / dt / 1e9, 4), unit="GTEPS (expanded edges)",
+                       matches_gpu=bool(np.array_equal(ref[0], depth)))
+            out["bfs"]["cpu"] = rec
+            log(f"cpu bfs: {rec}")
 
     if not args.no_batch:
         out["batched"] = bench_batched(ctx, not args.no_tune)
     if not args.no_serve:
         out["served"] = bench_served(ctx, args.serve_requests)
     if not args.no_matrix:
-        out.update(bench_matrix_ops(ctx))
+        out.update(bench_matrix_ops(ctx, cpu=not args.no_cpu))
 
     if not args.no_cpu:
         import oracle as O

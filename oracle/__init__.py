@@ -85,6 +85,8 @@ def lib():
         L.orc_strict_lower.argtypes = [P(OrcCsr), P(OrcCsr)]
         L.orc_tc.argtypes = [P(OrcCsr), P(C.c_int64)]
         L.orc_tc.restype = C.c_int64
+        L.orc_tc_rows.argtypes = [P(OrcCsr), C.c_int32, C.c_int32, P(C.c_int64)]
+        L.orc_tc_rows.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -237,4 +239,12 @@ def tc(l: Csr):
     s = _to_c(l)
     w = C.c_int64()
     t = lib().orc_tc(C.byref(s), C.byref(w))
+    return int(t), int(w.value)
+
+
+def tc_rows(l: Csr, r0: int, r1: int):
+    """The part of tc() whose mask rows are r0..r1-1: (triangles, wedges)."""
+    s = _to_c(l)
+    w = C.c_int64()
+    t = lib().orc_tc_rows(C.byref(s), r0, r1, C.byref(w))
     return int(t), int(w.value)

@@ -563,9 +563,13 @@ int orc_strict_lower(const orc_csr *a, orc_csr *l) {
     return 0;
 }
 
-int64_t orc_tc(const orc_csr *l, int64_t *wedges) {
+int64_t orc_tc(const orc_csr *l, int64_t *wedges) { return orc_tc_rows(l, 0, l->m, wedges); }
+
+/* the part of the count whose mask row i lies in [r0, r1) (a bounded sample
+   of the same work; the parts over a split of the rows add up to orc_tc) */
+int64_t orc_tc_rows(const orc_csr *l, int32_t r0, int32_t r1, int64_t *wedges) {
     int64_t tri = 0, w = 0;
-    for (int32_t i = 0; i < l->m; ++i) {
+    for (int32_t i = r0; i < r1; ++i) {
         int32_t ib = l->row_ptr[i], ie = l->row_ptr[i + 1];
         for (int32_t p = ib; p < ie; ++p) {
             int32_t k = l->col_ind[p];

@@ -99,6 +99,7 @@ int orc_strict_lower(const orc_csr *a, orc_csr *l);
  * count of src/tc.cuh:168-199 + test/tc/main.cu:238-249.  wedges = sum over
  * (i,k) in L of nnz(L(k,:)). */
 int64_t orc_tc(const orc_csr *l, int64_t *wedges);
+int64_t orc_tc_rows(const orc_csr *l, int32_t r0, int32_t r1, int64_t *wedges);
 
 #ifdef __cplusplus
 }

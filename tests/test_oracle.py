@@ -146,6 +146,24 @@ def test_tc_rmat_vs_trace():
     assert O.tc(O.strict_lower(a))[0] == int(np.trace(A @ A @ A)) // 6
 
 
+def test_tc_rows_split_adds_up():
+    """The row-range count (the CPU baseline's bounded sample) over a split
+    of the rows adds up to the whole count and wedges; a range of one row
+    counts the triangles whose largest vertex is that row (pairs of its
+    smaller neighbours that are adjacent), by brute force."""
+    a = O.rmat(10, 8, 78, True)
+    lo = O.strict_lower(a)
+    t, w = O.tc(lo)
+    cuts = [0, 100, 101, 517, lo.m]
+    parts = [O.tc_rows(lo, x, y) for x, y in zip(cuts, cuts[1:])]
+    assert sum(p[0] for p in parts) == t and sum(p[1] for p in parts) == w
+    A = dense(a)
+    i = int(np.argmax(np.diff(lo.row_ptr)))
+    nb = lo.col_ind[lo.row_ptr[i]:lo.row_ptr[i + 1]]
+    brute = sum(int(A[k, j]) for x, k in enumerate(nb) for j in nb[:x])
+    assert O.tc_rows(lo, i, i + 1)[0] == brute
+
+
 # --- MIS ---------------------------------------------------------------------
 
 @pytest.mark.parametrize("name", SMALL)
