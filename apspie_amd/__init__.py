@@ -24,6 +24,7 @@ RMAT_UNDIRECTED = 1
 RMAT_WEIGHTED = 2
 LOAD_UNDIRECTED = 1
 LOAD_SORT_COLUMNS = 2
+LOAD_MM_BANNER = 4
 
 ERRORS = {
     -1: "EINVAL",
@@ -135,6 +136,8 @@ def lib() -> C.CDLL:
         "apspie_memcpy_h2d": [vp, vp, vp, C.c_size_t],
         "apspie_memcpy_d2h": [vp, vp, vp, C.c_size_t],
         "apspie_load_mtx": [C.c_char_p, u32, P(_Csr)],
+        "apspie_csr_save": [C.c_char_p, P(_Csr)],
+        "apspie_csr_load": [C.c_char_p, P(_Csr)],
         "apspie_host_csr_free": [P(_Csr)],
         "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
         "apspie_ctx_set_tuning": [vp, P(Tuning)],
@@ -234,11 +237,25 @@ def _host_from_c(s: _Csr) -> HostCsr:
     return HostCsr(m, n, rp, ci, val)
 
 
-def load_mtx(path: str, undirected: bool = False, sort_columns: bool = False) -> HostCsr:
-    """readMtx + [makeSymmetric] + buildMatrix (src/util.cuh:355-643, src/matrix.cpp:55-111)."""
+def load_mtx(path: str, undirected: bool = False, sort_columns: bool = False, banner: bool = False) -> HostCsr:
+    """readMtx + [makeSymmetric] + buildMatrix (src/util.cuh:355-643, src/matrix.cpp:55-111);
+    banner: take field and symmetry from the %%MatrixMarket line."""
     s = _Csr()
-    flags = (LOAD_UNDIRECTED if undirected else 0) | (LOAD_SORT_COLUMNS if sort_columns else 0)
+    flags = ((LOAD_UNDIRECTED if undirected else 0) | (LOAD_SORT_COLUMNS if sort_columns else 0)
+             | (LOAD_MM_BANNER if banner else 0))
     _check(lib().apspie_load_mtx(os.fsencode(path), flags, C.byref(s)), None, f"load_mtx({path})")
+    return _host_from_c(s)
+
+
+def save_csr(path: str, h: HostCsr) -> None:
+    """Binary CSR cache file (apspie_csr_save)."""
+    _check(lib().apspie_csr_save(os.fsencode(path), C.byref(h._as_c())), None, f"save_csr({path})")
+
+
+def load_csr(path: str) -> HostCsr:
+    """Read a binary CSR cache file (apspie_csr_load: check word and structure verified)."""
+    s = _Csr()
+    _check(lib().apspie_csr_load(os.fsencode(path), C.byref(s)), None, f"load_csr({path})")
     return _host_from_c(s)
 
 

@@ -2,6 +2,7 @@
 // the reference's semantics and the seeded RMAT generator.
 #include <algorithm>
 #include <atomic>
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -176,6 +177,31 @@ int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
     return guarded(nullptr, [&] {
         Scanner sc;
         APSPIE_CHECK(sc.load(path), APSPIE_EIO, std::string("cannot open ") + path);
+        // %%MatrixMarket banner: field (real / integer / pattern) and symmetry
+        // (general / symmetric / skew-symmetric), when asked for
+        int banner_weighted = -1;  // -1: the reference's space-after rule
+        bool mirror = false, skew = false;
+        if (flags & APSPIE_LOAD_MM_BANNER) {
+            std::string line;
+            for (size_t i = 0; i + 1 < sc.buf.size() && sc.buf[i] != '\n'; ++i) line += char(std::tolower((unsigned char)sc.buf[i]));
+            std::vector<std::string> tok;
+            for (size_t i = 0; i < line.size();) {
+                while (i < line.size() && std::isspace((unsigned char)line[i])) ++i;
+                size_t j = i;
+                while (j < line.size() && !std::isspace((unsigned char)line[j])) ++j;
+                if (j > i) tok.push_back(line.substr(i, j - i));
+                i = j;
+            }
+            APSPIE_CHECK(tok.size() == 5 && tok[0] == "%%matrixmarket" && tok[1] == "matrix", APSPIE_EIO,
+                         "no %%MatrixMarket matrix banner");
+            APSPIE_CHECK(tok[2] == "coordinate", APSPIE_EIO, "only coordinate MatrixMarket files are read");
+            if (tok[3] == "pattern") banner_weighted = 0;
+            else if (tok[3] == "real" || tok[3] == "integer" || tok[3] == "double") banner_weighted = 1;
+            else throw Error(APSPIE_EIO, "unsupported MatrixMarket field " + tok[3]);
+            if (tok[4] == "symmetric") mirror = true;
+            else if (tok[4] == "skew-symmetric") mirror = skew = true;
+            else APSPIE_CHECK(tok[4] == "general", APSPIE_EIO, "unsupported MatrixMarket symmetry " + tok[4]);
+        }
         int c = sc.get(), old = 0;
         while (c != EOF) {
             if ((old == '\n' || old == 0) && c != '%') {
@@ -188,7 +214,7 @@ int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
         int m = 0, n = 0, nnz = 0;
         APSPIE_CHECK(sc.read_int(m) && sc.read_int(n) && sc.read_int(nnz), APSPIE_EIO, "bad mtx header");
         APSPIE_CHECK(m >= 0 && n >= 0 && nnz >= 0, APSPIE_EIO, "negative mtx size");
-        const bool undirected = flags & APSPIE_LOAD_UNDIRECTED;
+        const bool undirected = (flags & APSPIE_LOAD_UNDIRECTED) || mirror;
         const int64_t cap = undirected ? 2 * int64_t(nnz) : nnz;
         std::vector<int32_t> row(cap), col(cap);
         std::vector<float> val(cap);
@@ -199,7 +225,8 @@ int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
             APSPIE_CHECK(sc.read_int(a) && sc.read_int(b), APSPIE_EIO, "not enough rows in mtx file");
             if (j == 0) {
                 ch = sc.get();
-                weighted = ch == ' ';
+                weighted = banner_weighted >= 0 ? banner_weighted == 1 : ch == ' ';
+                if (banner_weighted >= 0) sc.pos--;  // the character may start the value
             }
             float w = 1.0f;
             if (weighted) APSPIE_CHECK(sc.read_float(w), APSPIE_EIO, "bad weight");
@@ -214,7 +241,7 @@ int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
                 if (col[i] != row[i]) {
                     row[k] = col[i];
                     col[k] = row[i];
-                    val[k] = val[i];
+                    val[k] = skew ? -val[i] : val[i];
                     ++k;
                 }
             e = k;
@@ -347,6 +374,128 @@ int apspie_rmat(int scale, int edge_factor, uint32_t seed, uint32_t flags, uint3
 // Each worker owns a range of columns and walks the rows in order, so the
 // entries of every output row come out in increasing source-row order
 // (the stable counting sort csr2csc performs).
+// ---- binary CSR cache ------------------------------------------------------
+// File: "APSPCSR1", u32 version (1), u32 flags (bit 0: values), i64 m, n, nnz,
+// int32 row_ptr[m+1], int32 col_ind[nnz], [float val[nnz]], then u64 check =
+// the word hash below over every byte before it.  Little-endian.
+
+namespace {
+
+constexpr char CSR_MAGIC[8] = {'A', 'P', 'S', 'P', 'C', 'S', 'R', '1'};
+
+struct CsrFileHeader {
+    char magic[8];
+    uint32_t version, flags;
+    int64_t m, n, nnz;
+};
+
+// 64-bit hash of a byte stream fed in pieces: 8-byte words mixed by
+// multiply-xorshift (a tail shorter than a word is zero-padded)
+struct WordHash {
+    uint64_t h = 0x9e3779b97f4a7c15ull, carry = 0;
+    int nc = 0;
+    void word(uint64_t w) {
+        h ^= w;
+        h *= 0xff51afd7ed558ccdull;
+        h ^= h >> 33;
+    }
+    void bytes(const void *p, size_t n) {
+        const unsigned char *b = (const unsigned char *)p;
+        size_t i = 0;
+        while (nc && i < n) {
+            carry |= uint64_t(b[i++]) << (8 * nc);
+            if (++nc == 8) {
+                word(carry);
+                carry = 0;
+                nc = 0;
+            }
+        }
+        for (; i + 8 <= n; i += 8) {
+            uint64_t w;
+            memcpy(&w, b + i, 8);
+            word(w);
+        }
+        for (; i < n; ++i) {
+            carry |= uint64_t(b[i]) << (8 * nc);
+            ++nc;
+        }
+    }
+    uint64_t done() {
+        if (nc) word(carry);
+        word(0x5bd1e995ull);
+        return h;
+    }
+};
+
+}  // namespace
+
+int apspie_csr_save(const char *path, const apspie_host_csr *h) {
+    if (!path || !h || h->m < 0 || h->n < 0 || h->nnz < 0 || !h->row_ptr || (h->nnz && !h->col_ind))
+        return APSPIE_EINVAL;
+    return guarded(nullptr, [&] {
+        FILE *f = fopen(path, "wb");
+        APSPIE_CHECK(f, APSPIE_EIO, std::string("cannot create ") + path);
+        CsrFileHeader hd;
+        memcpy(hd.magic, CSR_MAGIC, 8);
+        hd.version = 1;
+        hd.flags = h->val ? 1u : 0u;
+        hd.m = h->m;
+        hd.n = h->n;
+        hd.nnz = h->nnz;
+        WordHash wh;
+        bool ok = true;
+        auto put = [&](const void *p, size_t n) {
+            wh.bytes(p, n);
+            ok = ok && fwrite(p, 1, n, f) == n;
+        };
+        put(&hd, sizeof(hd));
+        put(h->row_ptr, (size_t(h->m) + 1) * 4);
+        put(h->col_ind, size_t(h->nnz) * 4);
+        if (h->val) put(h->val, size_t(h->nnz) * 4);
+        const uint64_t chk = wh.done();
+        ok = ok && fwrite(&chk, 1, 8, f) == 8;
+        ok = (fclose(f) == 0) && ok;
+        APSPIE_CHECK(ok, APSPIE_EIO, std::string("write failed: ") + path);
+    });
+}
+
+int apspie_csr_load(const char *path, apspie_host_csr *out) {
+    if (!path || !out) return APSPIE_EINVAL;
+    memset(out, 0, sizeof(*out));
+    return guarded(nullptr, [&] {
+        FILE *f = fopen(path, "rb");
+        APSPIE_CHECK(f, APSPIE_EIO, std::string("cannot open ") + path);
+        CsrFileHeader hd;
+        WordHash wh;
+        auto get = [&](void *p, size_t n) {
+            const bool ok = fread(p, 1, n, f) == n;
+            if (ok) wh.bytes(p, n);
+            return ok;
+        };
+        bool ok = get(&hd, sizeof(hd)) && memcmp(hd.magic, CSR_MAGIC, 8) == 0 && hd.version == 1 &&
+                  hd.m >= 0 && hd.m < (int64_t(1) << 31) && hd.n >= 0 && hd.n < (int64_t(1) << 31) && hd.nnz >= 0 &&
+                  hd.nnz < (int64_t(1) << 31) && (hd.flags & ~1u) == 0;
+        if (!ok) {
+            fclose(f);
+            throw Error(APSPIE_EIO, std::string("not a CSR cache file: ") + path);
+        }
+        host_csr_alloc(out, int32_t(hd.m), int32_t(hd.n), hd.nnz, hd.flags & 1u);
+        ok = get(out->row_ptr, (size_t(hd.m) + 1) * 4) && get(out->col_ind, size_t(hd.nnz) * 4) &&
+             (!(hd.flags & 1u) || get(out->val, size_t(hd.nnz) * 4));
+        uint64_t chk = 0;
+        ok = ok && fread(&chk, 1, 8, f) == 8 && chk == wh.done();
+        fclose(f);
+        // structure: offsets from 0 to nnz, non-decreasing, columns in range
+        for (int64_t i = 0; ok && i < hd.m; ++i) ok = out->row_ptr[i] <= out->row_ptr[i + 1];
+        ok = ok && out->row_ptr[0] == 0 && out->row_ptr[hd.m] == hd.nnz;
+        for (int64_t e = 0; ok && e < hd.nnz; ++e) ok = out->col_ind[e] >= 0 && out->col_ind[e] < hd.n;
+        if (!ok) {
+            apspie_host_csr_free(out);
+            throw Error(APSPIE_EIO, std::string("truncated or corrupt CSR cache file: ") + path);
+        }
+    });
+}
+
 int apspie_host_csr_transpose(const apspie_host_csr *a, apspie_host_csr *at) {
     if (!a || !at) return APSPIE_EINVAL;
     memset(at, 0, sizeof(*at));

@@ -178,12 +178,26 @@ int apspie_hbm_copy_bandwidth(apspie_ctx *ctx, size_t bytes, int iters, double *
 /* ---- graph loading -------------------------------------------------------- */
 #define APSPIE_LOAD_UNDIRECTED 1u   /* makeSymmetric (src/util.cuh:572-643) */
 #define APSPIE_LOAD_SORT_COLUMNS 2u /* sort each row's columns (not in the reference) */
+/* Read the %%MatrixMarket banner (the reference skips every % line): field
+ * pattern -> no values, real / integer -> a value on every line; symmetry
+ * symmetric -> off-diagonal entries mirrored as makeSymmetric does,
+ * skew-symmetric -> mirrored with negated values; array format, complex and
+ * hermitian are refused (APSPIE_EIO).  Entries keep readMtx's reading
+ * (line "a b" is column a-1 of row b-1). */
+#define APSPIE_LOAD_MM_BANNER 4u
 
 /* MatrixMarket reader: readEdge + readMtx + [makeSymmetric] + buildMatrix,
  * src/util.cuh:355-430, :522-643, src/matrix.cpp:55-111.  Line "a b [w]"
  * puts column a-1 into row b-1, as readMtx does. */
 int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out);
 void apspie_host_csr_free(apspie_host_csr *h);
+
+/* Binary CSR cache (no counterpart in the reference, which re-parses its .mtx
+ * text on every run): header ("APSPCSR1", version, flags, m, n, nnz), the
+ * arrays, a 64-bit check word.  Loading verifies the check word and the
+ * structure (offsets, column range); APSPIE_EIO otherwise. */
+int apspie_csr_save(const char *path, const apspie_host_csr *h);
+int apspie_csr_load(const char *path, apspie_host_csr *out);
 
 #define APSPIE_RMAT_UNDIRECTED 1u
 #define APSPIE_RMAT_WEIGHTED 2u

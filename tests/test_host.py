@@ -124,3 +124,77 @@ def test_transpose_duplicates_and_bad_columns():
     bad = A.HostCsr(2, 2, np.array([0, 1, 2], np.int32), np.array([0, 2], np.int32))
     with pytest.raises(A.ApspieError):
         A.transpose(bad)
+
+
+BANNER = os.path.join(ROOT, "tests", "golden", "mtx_banner")
+
+
+def dense_of(h):
+    d = np.zeros((h.m, h.n), np.float32)
+    for r in range(h.m):
+        for p in range(h.row_ptr[r], h.row_ptr[r + 1]):
+            d[r, h.col_ind[p]] += 1.0 if h.val is None else h.val[p]
+    return d
+
+
+def test_banner_symmetric_and_skew():
+    """Banner symmetry mirrors the stored triangle (skew: negated); entries
+    keep readMtx's reading (line "a b" is column a-1 of row b-1)."""
+    h = A.load_mtx(os.path.join(BANNER, "sym_real.mtx"), banner=True)
+    want = np.array([[2.5, 1.5, -2, 0], [1.5, 0, 0, 0.25], [-2, 0, 0, 3], [0, 0.25, 3, 0]], np.float32)
+    assert np.array_equal(dense_of(h), want) and h.nnz == 9
+    k = A.load_mtx(os.path.join(BANNER, "skew_int.mtx"), banner=True)
+    assert np.array_equal(dense_of(k), np.array([[0, 4, 0], [-4, 0, -7], [0, 7, 0]], np.float32))
+    # without the flag: the reference's reading, no mirroring
+    plain = A.load_mtx(os.path.join(BANNER, "sym_real.mtx"))
+    assert plain.nnz == 5 and plain.val is not None
+
+
+def test_banner_field():
+    p = A.load_mtx(os.path.join(BANNER, "pattern_general.mtx"), banner=True)
+    assert p.val is None and np.array_equal(dense_of(p), np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]], np.float32))
+    # a tab after the first pair: the banner reads the values; the reference's
+    # rule (values iff a space follows) reads none and then trips on them
+    t = A.load_mtx(os.path.join(BANNER, "real_general_tab.mtx"), banner=True)
+    assert np.array_equal(dense_of(t), np.array([[0, 0, -4], [0.5, 0, 0], [0, 1.25, 0]], np.float32))
+    with pytest.raises(A.ApspieError):
+        A.load_mtx(os.path.join(BANNER, "real_general_tab.mtx"))
+    with pytest.raises(A.ApspieError):
+        A.load_mtx(os.path.join(BANNER, "array.mtx"), banner=True)
+    with pytest.raises(A.ApspieError):  # no banner line
+        A.load_mtx(os.path.join(MTX, "test_cc.mtx"), banner=True)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_csr_cache_round_trip(tmp_path, weighted):
+    h = A.rmat(10, 8, 91, undirected=False, weight_seed=92 if weighted else None)
+    f = str(tmp_path / "g.csr")
+    A.save_csr(f, h)
+    g = A.load_csr(f)
+    assert (g.m, g.n) == (h.m, h.n)
+    assert np.array_equal(g.row_ptr, h.row_ptr) and np.array_equal(g.col_ind, h.col_ind)
+    assert (g.val is None) == (not weighted)
+    if weighted:
+        assert np.array_equal(g.val, h.val)
+    e = A.HostCsr(5, 3, np.zeros(6, np.int32), np.zeros(0, np.int32))
+    A.save_csr(f, e)
+    z = A.load_csr(f)
+    assert z.m == 5 and z.n == 3 and z.nnz == 0
+
+
+def test_csr_cache_rejects_damage(tmp_path):
+    h = A.rmat(9, 8, 93, weight_seed=94)
+    f = str(tmp_path / "g.csr")
+    A.save_csr(f, h)
+    raw = bytearray(open(f, "rb").read())
+    for pos in (3, 40, len(raw) // 2, len(raw) - 3):  # magic, header, payload, check word
+        bad = bytearray(raw)
+        bad[pos] ^= 0x10
+        open(f, "wb").write(bytes(bad))
+        with pytest.raises(A.ApspieError):
+            A.load_csr(f)
+    open(f, "wb").write(bytes(raw[:-20]))  # truncated
+    with pytest.raises(A.ApspieError):
+        A.load_csr(f)
+    with pytest.raises(A.ApspieError):
+        A.load_csr(str(tmp_path / "missing.csr"))
