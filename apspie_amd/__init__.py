@@ -137,6 +137,9 @@ def lib() -> C.CDLL:
         "apspie_memcpy_d2h": [vp, vp, vp, C.c_size_t],
         "apspie_load_mtx": [C.c_char_p, u32, P(_Csr)],
         "apspie_csr_save": [C.c_char_p, P(_Csr)],
+        "apspie_csr_attach_csc_device": [vp, P(_DCsr)],
+        "apspie_coo_to_csr": [vp, vp, i64, i32, vp],
+        "apspie_csr2csc": [vp, P(_DCsr), vp, vp, vp],
         "apspie_csr_load": [C.c_char_p, P(_Csr)],
         "apspie_host_csr_free": [P(_Csr)],
         "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
@@ -332,8 +335,9 @@ class DeviceBuffer:
 class DeviceCsr:
     """CSR resident in HBM (library-owned), optionally with its CSC.
 
-    csc: None (CSR only), "symmetric" (the CSR is its own transpose), or the
-    host transpose (transpose(h)) to upload beside it.
+    csc: None (CSR only), "symmetric" (the CSR is its own transpose),
+    "device" (built on the device, apspie_csr2csc), or the host transpose
+    (transpose(h)) to upload beside it.
     """
 
     def __init__(self, ctx: "Context", h: HostCsr, csc=None):
@@ -345,8 +349,11 @@ class DeviceCsr:
         _check(lib().apspie_csr_upload(ctx.h, C.byref(hc), C.byref(self._c)), ctx.h, "csr_upload")
         if csc is not None:
             if isinstance(csc, str):
-                assert csc == "symmetric", csc
-                rc = lib().apspie_csr_attach_csc(ctx.h, C.byref(self._c), None)
+                assert csc in ("symmetric", "device"), csc
+                if csc == "device":
+                    rc = lib().apspie_csr_attach_csc_device(ctx.h, C.byref(self._c))
+                else:
+                    rc = lib().apspie_csr_attach_csc(ctx.h, C.byref(self._c), None)
             else:
                 tc_ = csc._as_c()
                 rc = lib().apspie_csr_attach_csc(ctx.h, C.byref(self._c), C.byref(tc_))
@@ -559,6 +566,31 @@ class Context:
         ds.free()
         dd.free()
         return out, st
+
+    def csr2csc(self, g: DeviceCsr, values: bool = True) -> HostCsr:
+        """apspie_csr2csc on the device; returns the CSC as a host CSR of A^T."""
+        cp = self.alloc(4 * (g.n + 1))
+        ri = self.alloc(4 * max(g.nnz, 1))
+        cv = self.alloc(4 * max(g.nnz, 1)) if (values and g.weighted) else None
+        _check(lib().apspie_csr2csc(self.h, C.byref(g.c), cp.ptr, ri.ptr, cv.ptr if cv else None), self.h, "csr2csc")
+        out = HostCsr(g.n, g.m, cp.download(np.int32, g.n + 1), ri.download(np.int32, g.nnz),
+                      cv.download(np.float32, g.nnz) if cv else None)
+        for b in (cp, ri, cv):
+            if b:
+                b.free()
+        return out
+
+    def coo_to_csr(self, rows: np.ndarray, m: int) -> np.ndarray:
+        """apspie_coo_to_csr: row offsets of sorted COO row indices."""
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        dr = self.alloc(4 * max(len(rows), 1)).upload(rows) if len(rows) else None
+        rp = self.alloc(4 * (m + 1))
+        _check(lib().apspie_coo_to_csr(self.h, dr.ptr if dr else None, len(rows), m, rp.ptr), self.h, "coo_to_csr")
+        out = rp.download(np.int32, m + 1)
+        rp.free()
+        if dr:
+            dr.free()
+        return out
 
     def tc(self, l: DeviceCsr):
         """Triangles from the strict lower triangle L; returns (int count, Stats)."""

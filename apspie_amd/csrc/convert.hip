@@ -1,0 +1,302 @@
+// convert.hip -- sparse format conversions on the device.
+//
+//   apspie_coo_to_csr   row offsets of row-sorted COO row indices
+//                       (cusparseXcoo2csr in coo2csr, src/util.cuh:456-471)
+//   apspie_csr2csc      the transpose (CSC) of a CSR, rows ascending inside
+//                       each column, values moved with them
+//                       (cusparseScsr2csc in csr2csc, src/util.cuh:473-511)
+//
+// csr2csc is a stable least-significant-digit radix sort of the entries by
+// column, 8 bits per pass: entries enter in CSR order (rows ascending), so a
+// stable sort by column leaves each column's rows ascending.  A pass is a
+// per-tile digit histogram, one exclusive scan of the histograms (digit-major,
+// so every tile's range for a digit follows the earlier tiles'), and a
+// scatter in which each wave ranks its lanes among equal digits with eight
+// ballots and the waves of a tile add up their counts in LDS, round by round
+// in item order, so equal digits keep their order.  The sorted columns give
+// the column offsets the same way coo2csr gives row offsets; row indices and
+// values are gathered through the original entry positions.
+#include <algorithm>
+#include <cstring>
+
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace apspie {
+namespace cvt {
+
+using namespace dev;
+
+constexpr int T = 256, ITEMS = 16, TILE_N = T * ITEMS, RADIX = 256;
+
+// off[r] = first position e with key[e] >= r, r in [0, m]; keys must be
+// non-decreasing and in [0, m) (else *bad = 1: unsorted 1, out of range 2)
+__global__ __launch_bounds__(256) void k_compress(const int32_t *__restrict__ key, int64_t n, int m, int32_t *off,
+                                                  int32_t *bad) {
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e <= n; e += int64_t(gridDim.x) * 256) {
+        int prev = e > 0 ? key[e - 1] : -1;
+        int cur = e < n ? key[e] : m;
+        if (e < n && (cur < 0 || cur >= m)) {
+            *bad = 2;
+            continue;
+        }
+        if (e > 0 && (prev < 0 || prev >= m)) continue;  // reported by its own thread
+        if (cur < prev) {
+            *bad = 1;
+            continue;
+        }
+        for (int r = prev + 1; r <= cur; ++r) off[r] = int32_t(e);
+    }
+}
+
+// exclusive scan of n int32 into out[0..n] (out[n] = total), one pass
+__global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, int64_t n, uint64_t *status,
+                                              unsigned int *counter) {
+    __shared__ int s_bid;
+    __shared__ uint64_t s_w[4];
+    __shared__ uint64_t s_pre;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) s_bid = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int bid = s_bid;
+    const int64_t b0 = int64_t(bid) * TILE_N + int64_t(tid) * ITEMS;
+    uint32_t v[ITEMS];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        v[k] = b0 + k < n ? uint32_t(in[b0 + k]) : 0u;
+        sum += v[k];
+    }
+    const uint64_t incl = wave_incl_scan(sum, lane);
+    if (lane == 63) s_w[wid] = incl;
+    __syncthreads();
+    uint64_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        if (w < wid) wpre += s_w[w];
+        tot += s_w[w];
+    }
+    if (wid == 0) {
+        const uint64_t p = lookback_exclusive(status, bid, tot, lane);
+        if (lane == 0) s_pre = p;
+    }
+    __syncthreads();
+    uint64_t run = s_pre + wpre + incl - sum;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+        if (b0 + k < n) out[b0 + k] = int32_t(run);
+        run += v[k];
+        if (b0 + k == n - 1) out[n] = int32_t(run);
+    }
+    if (n == 0 && bid == 0 && tid == 0) out[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero64(uint64_t *p, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = 0;
+}
+
+// key[e] = col[e], idx[e] = e
+__global__ __launch_bounds__(256) void k_seed(const int32_t *__restrict__ col, int64_t n, int32_t *key, int32_t *idx) {
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += int64_t(gridDim.x) * 256) {
+        key[e] = col[e];
+        idx[e] = int32_t(e);
+    }
+}
+
+// hist[d * ntiles + tile] = items of the tile with digit d
+__global__ __launch_bounds__(T) void k_hist(const int32_t *__restrict__ key, int64_t n, int shift, int ntiles,
+                                            int32_t *hist) {
+    __shared__ int32_t s_h[RADIX];
+    const int tid = threadIdx.x;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        s_h[tid] = 0;
+        __syncthreads();
+        const int64_t t0 = int64_t(tile) * TILE_N;
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int64_t t = t0 + j * T + tid;
+            if (t < n) atomicAdd(&s_h[(uint32_t(key[t]) >> shift) & (RADIX - 1)], 1);
+        }
+        __syncthreads();
+        hist[int64_t(tid) * ntiles + tile] = s_h[tid];
+        __syncthreads();
+    }
+}
+
+// stable scatter of one pass: off[d * ntiles + tile] = where the tile's
+// first item of digit d goes
+__global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, const int32_t *__restrict__ idx,
+                                               int64_t n, int shift, int ntiles, const int32_t *__restrict__ off,
+                                               int32_t *key_out, int32_t *idx_out) {
+    constexpr int NW = T / WAVE;
+    __shared__ int32_t s_base[RADIX];
+    __shared__ int32_t s_wcnt[NW][RADIX];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        s_base[tid] = off[int64_t(tid) * ntiles + tile];
+        const int64_t t0 = int64_t(tile) * TILE_N;
+        for (int j = 0; j < ITEMS; ++j) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) s_wcnt[w][tid] = 0;
+            __syncthreads();
+            const int64_t t = t0 + j * T + tid;
+            const bool valid = t < n;
+            int k = 0, x = 0, d = 0;
+            if (valid) {
+                k = key[t];
+                x = idx[t];
+                d = (uint32_t(k) >> shift) & (RADIX - 1);
+            }
+            // lanes holding the same digit
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t ones = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? ones : ~ones;
+            }
+            const int rank = __popcll(peers & lt);
+            if (valid && rank == 0) s_wcnt[wid][d] = __popcll(peers);
+            __syncthreads();
+            if (valid) {
+                int pos = s_base[d] + rank;
+                for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
+                key_out[pos] = k;
+                idx_out[pos] = x;
+            }
+            __syncthreads();
+            int add = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) add += s_wcnt[w][tid];
+            s_base[tid] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// rowid[e] = the row holding entry e (one wave per row)
+__global__ __launch_bounds__(256) void k_row_ids(int m, const int32_t *__restrict__ rp, int32_t *rowid) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int64_t nw = int64_t(gridDim.x) * (256 / WAVE);
+    for (int64_t r = int64_t(blockIdx.x) * (256 / WAVE) + threadIdx.x / WAVE; r < m; r += nw)
+        for (int e = rp[r] + lane; e < rp[r + 1]; e += WAVE) rowid[e] = int32_t(r);
+}
+
+__global__ __launch_bounds__(256) void k_gather(const int32_t *__restrict__ idx, int64_t n,
+                                                const int32_t *__restrict__ rowid, const float *__restrict__ val,
+                                                int32_t *row_ind, float *val_out) {
+    for (int64_t p = int64_t(blockIdx.x) * 256 + threadIdx.x; p < n; p += int64_t(gridDim.x) * 256) {
+        const int32_t e = idx[p];
+        row_ind[p] = rowid[e];
+        if (val_out) val_out[p] = val[e];
+    }
+}
+
+unsigned grid_for(apspie_ctx *ctx, int64_t n, unsigned per) {
+    return std::max(1u, std::min(ceil_div(std::max<int64_t>(n, 1), per), unsigned(ctx->cu_count) * 32));
+}
+
+void compress(apspie_ctx *ctx, const int32_t *key, int64_t n, int m, int32_t *off) {
+    int32_t *bad = (int32_t *)ctx->slot(S_AUX7, 256);
+    APSPIE_HIP(hipMemsetAsync(bad, 0, 4, ctx->stream));
+    k_compress<<<grid_for(ctx, n + 1, 256), 256, 0, ctx->stream>>>(key, n, m, off, bad);
+    APSPIE_HIP(hipGetLastError());
+    int32_t h = 0;
+    APSPIE_HIP(hipMemcpyAsync(&h, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
+    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    APSPIE_CHECK(h != 1, APSPIE_EINVAL, "row indices are not sorted");
+    APSPIE_CHECK(h != 2, APSPIE_EINVAL, "row index out of range");
+}
+
+}  // namespace cvt
+}  // namespace apspie
+
+using namespace apspie;
+
+extern "C" {
+
+int apspie_coo_to_csr(apspie_ctx *ctx, const int32_t *d_rows, int64_t nnz, int32_t m, int32_t *d_row_ptr) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        APSPIE_CHECK(m >= 0 && nnz >= 0 && nnz < (int64_t(1) << 31), APSPIE_EINVAL, "bad COO shape");
+        APSPIE_CHECK(d_row_ptr && (nnz == 0 || d_rows), APSPIE_EINVAL, "null COO arrays");
+        cvt::compress(ctx, d_rows, nnz, m, d_row_ptr);
+    });
+}
+
+int apspie_csr2csc(apspie_ctx *ctx, const apspie_csr *a, int32_t *d_col_ptr, int32_t *d_row_ind, float *d_val) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        using namespace cvt;
+        validate_csr(a, false);
+        APSPIE_CHECK(d_col_ptr && (a->nnz == 0 || d_row_ind), APSPIE_EINVAL, "null CSC arrays");
+        APSPIE_CHECK(!d_val || a->val || a->nnz == 0, APSPIE_EINVAL, "values asked of a matrix without values");
+        const int64_t n = a->nnz;
+        if (n == 0) {
+            APSPIE_HIP(hipMemsetAsync(d_col_ptr, 0, (size_t(a->n) + 1) * 4, ctx->stream));
+            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            return;
+        }
+        int32_t *key[2] = {(int32_t *)ctx->slot(S_AUX0, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX1, size_t(n) * 4)};
+        int32_t *idx[2] = {(int32_t *)ctx->slot(S_AUX2, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX3, size_t(n) * 4)};
+        const int ntiles = (int)ceil_div(n, TILE_N);
+        const int64_t nh = int64_t(RADIX) * ntiles;
+        int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
+        int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
+        const int nscan = (int)ceil_div(nh, TILE_N);
+        uint64_t *status = (uint64_t *)ctx->slot(S_STATUS, size_t(nscan + 2) * 8);
+        const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
+        k_seed<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(a->col_ind, n, key[0], idx[0]);
+        int bits = 1;
+        while (bits < 31 && (int64_t(1) << bits) < int64_t(a->n)) ++bits;
+        int cur = 0;
+        for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
+            k_hist<<<gt, T, 0, ctx->stream>>>(key[cur], n, shift, ntiles, hist);
+            k_zero64<<<grid_for(ctx, nscan + 2, 256), 256, 0, ctx->stream>>>(status, nscan + 2);
+            k_scan<<<nscan, 256, 0, ctx->stream>>>(hist, offs, nh, status, (unsigned *)(status + nscan));
+            k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], n, shift, ntiles, offs, key[cur ^ 1],
+                                                 idx[cur ^ 1]);
+        }
+        APSPIE_HIP(hipGetLastError());
+        compress(ctx, key[cur], n, a->n, d_col_ptr);
+        int32_t *rowid = key[cur ^ 1];  // free after the last pass
+        k_row_ids<<<grid_for(ctx, int64_t(a->m) * WAVE, 256), 256, 0, ctx->stream>>>(a->m, a->row_ptr, rowid);
+        k_gather<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(idx[cur], n, rowid, a->val, d_row_ind, d_val);
+        APSPIE_HIP(hipGetLastError());
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int apspie_csr_attach_csc_device(apspie_ctx *ctx, apspie_csr_owned *c) {
+    if (!ctx || !c) return APSPIE_EINVAL;
+    int32_t *cp = nullptr, *ri = nullptr;
+    float *cv = nullptr;
+    const int rc = guarded(ctx, [&] {
+        APSPIE_CHECK(c->csc_col_ptr == nullptr, APSPIE_EINVAL, "the matrix already has a CSC");
+        APSPIE_HIP(hipMalloc(&cp, (size_t(c->n) + 1) * 4));
+        APSPIE_HIP(hipMalloc(&ri, size_t(std::max<int64_t>(c->nnz, 1)) * 4));
+        if (c->val) APSPIE_HIP(hipMalloc(&cv, size_t(std::max<int64_t>(c->nnz, 1)) * 4));
+    });
+    if (rc == APSPIE_OK) {
+        const apspie_csr v = apspie_csr_view(c);
+        const int r2 = apspie_csr2csc(ctx, &v, cp, ri, cv);
+        if (r2 == APSPIE_OK) {
+            c->csc_col_ptr = cp;
+            c->csc_row_ind = ri;
+            c->csc_val = cv;
+            return APSPIE_OK;
+        }
+        (void)hipFree(cp);
+        (void)hipFree(ri);
+        if (cv) (void)hipFree(cv);
+        return r2;
+    }
+    if (cp) (void)hipFree(cp);
+    if (ri) (void)hipFree(ri);
+    if (cv) (void)hipFree(cv);
+    return rc;
+}
+
+}  // extern "C"

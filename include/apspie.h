@@ -230,6 +230,18 @@ int apspie_csr_upload(apspie_ctx *ctx, const apspie_host_csr *h, apspie_csr_owne
  * matrix symmetric, values included, and aliases the CSR arrays.  The d_cscColPtrA /
  * d_cscRowIndA / d_cscValA set of test/sssp/main.cu:69-110. */
 int apspie_csr_attach_csc(apspie_ctx *ctx, apspie_csr_owned *c, const apspie_host_csr *csc);
+/* apspie_csr_attach_csc with the CSC built on the device (apspie_csr2csc). */
+int apspie_csr_attach_csc_device(apspie_ctx *ctx, apspie_csr_owned *c);
+/* Row offsets d_row_ptr[m+1] of row-sorted COO row indices; APSPIE_EINVAL if
+ * the rows are not sorted or out of [0, m).  Replaces coo2csr
+ * (src/util.cuh:456-471, cusparseXcoo2csr). */
+int apspie_coo_to_csr(apspie_ctx *ctx, const int32_t *d_rows, int64_t nnz, int32_t m, int32_t *d_row_ptr);
+/* The CSC of a device CSR into caller-allocated d_col_ptr[n+1],
+ * d_row_ind[nnz] and d_val[nnz] (NULL: structure only, as
+ * CUSPARSE_ACTION_SYMBOLIC): rows ascending inside each column, values moved
+ * with them (the order apspie_host_csr_transpose gives).  Replaces csr2csc
+ * (src/util.cuh:473-511, cusparseScsr2csc). */
+int apspie_csr2csc(apspie_ctx *ctx, const apspie_csr *a, int32_t *d_col_ptr, int32_t *d_row_ind, float *d_val);
 int apspie_csr_free(apspie_ctx *ctx, apspie_csr_owned *c);
 /* View an owned CSR as a borrowed one. */
 apspie_csr apspie_csr_view(const apspie_csr_owned *c);
