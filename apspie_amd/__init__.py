@@ -23,6 +23,7 @@ SSSP_INF = np.float32(1.70141e38)
 RMAT_UNDIRECTED = 1
 RMAT_WEIGHTED = 2
 LOAD_UNDIRECTED = 1
+SR_PLUS_TIMES, SR_MIN_PLUS, SR_MAX_TIMES, SR_OR_AND = 0, 1, 2, 3
 LOAD_SORT_COLUMNS = 2
 LOAD_MM_BANNER = 4
 
@@ -140,6 +141,8 @@ def lib() -> C.CDLL:
         "apspie_csr_attach_csc_device": [vp, P(_DCsr)],
         "apspie_coo_to_csr": [vp, vp, i64, i32, vp],
         "apspie_csr2csc": [vp, P(_DCsr), vp, vp, vp],
+        "apspie_mxv": [vp, P(_DCsr), C.c_int, vp, vp, P(Stats)],
+        "apspie_mxv_sparse": [vp, P(_DCsr), C.c_int, vp, vp, i32, vp, vp, P(i32), P(Stats)],
         "apspie_csr_load": [C.c_char_p, P(_Csr)],
         "apspie_host_csr_free": [P(_Csr)],
         "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
@@ -566,6 +569,37 @@ class Context:
         ds.free()
         dd.free()
         return out, st
+
+    def mxv(self, g: DeviceCsr, x: np.ndarray, semiring: int = SR_PLUS_TIMES):
+        """apspie_mxv: dense x[m] -> dense y[n] over the semiring; returns (y, Stats)."""
+        dx = self.alloc(4 * max(g.m, 1)).upload(np.asarray(x, dtype=np.float32))
+        dy = self.alloc(4 * max(g.n, 1))
+        st = Stats()
+        _check(lib().apspie_mxv(self.h, C.byref(g.c), semiring, dx.ptr, dy.ptr, C.byref(st)), self.h, "mxv")
+        y = dy.download(np.float32, g.n)
+        dx.free()
+        dy.free()
+        return y, st
+
+    def mxv_sparse(self, g: DeviceCsr, ind: np.ndarray, val: np.ndarray, semiring: int = SR_PLUS_TIMES):
+        """apspie_mxv_sparse: (index, value) pairs -> (reached columns ascending, their values)."""
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        val = np.ascontiguousarray(val, dtype=np.float32)
+        nx = len(ind)
+        di = self.alloc(4 * max(nx, 1)).upload(ind) if nx else None
+        dv = self.alloc(4 * max(nx, 1)).upload(val) if nx else None
+        yi = self.alloc(4 * max(g.n, 1))
+        yv = self.alloc(4 * max(g.n, 1))
+        ny = C.c_int32()
+        st = Stats()
+        _check(lib().apspie_mxv_sparse(self.h, C.byref(g.c), semiring, di.ptr if di else None,
+                                       dv.ptr if dv else None, nx, yi.ptr, yv.ptr, C.byref(ny), C.byref(st)),
+               self.h, "mxv_sparse")
+        out = yi.download(np.int32, ny.value), yv.download(np.float32, ny.value)
+        for b in (di, dv, yi, yv):
+            if b:
+                b.free()
+        return out
 
     def csr2csc(self, g: DeviceCsr, values: bool = True) -> HostCsr:
         """apspie_csr2csc on the device; returns the CSC as a host CSR of A^T."""

@@ -316,6 +316,27 @@ int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, aps
  * (cuSPARSE L*L^T, ewiseMultTc src/tc.cuh:168-199, float mgpu::Reduce). */
 int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, apspie_stats *stats);
 
+/* ---- general-semiring SpMSpV ----------------------------------------------
+ * The reference's mXv (src/mXv.cuh:110-389, dense-in and sparse-in forms):
+ * y[c] = (+)_{u present, (u,c) in A} x[u] (*) A(u,c), y = the semiring's
+ * identity where nothing reaches.  PLUS_TIMES sums in FP64 and rounds once
+ * (identity 0); MIN_PLUS is apspie_sssp_step's min-plus (identity and
+ * presence rule the reference's: 1.70141e38, -1 < x < 1e38); MAX_TIMES has
+ * identity -inf; OR_AND works on 0 / 1 (identity 0).  Dense form: x[m], u
+ * present when x[u] differs from the identity; y[n]. */
+#define APSPIE_SR_PLUS_TIMES 0
+#define APSPIE_SR_MIN_PLUS 1
+#define APSPIE_SR_MAX_TIMES 2
+#define APSPIE_SR_OR_AND 3
+int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, const float *d_x, float *d_y,
+               apspie_stats *stats);
+/* Sparse form: the nx listed indices of x are present (any value); the
+ * reached columns come back in ascending order (d_yind, d_yval, room for n)
+ * with *ny their count. */
+int apspie_mxv_sparse(apspie_ctx *ctx, const apspie_csr *a, int semiring, const int32_t *d_xind,
+                      const float *d_xval, int32_t nx, int32_t *d_yind, float *d_yval, int32_t *ny,
+                      apspie_stats *stats);
+
 /* ---- batched path ----------------------------------------------------------
  * No counterpart in the reference (it runs one source per call); each result
  * row equals the plain entry point's output for that source. */

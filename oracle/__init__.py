@@ -87,6 +87,8 @@ def lib():
         L.orc_tc.restype = C.c_int64
         L.orc_tc_rows.argtypes = [P(OrcCsr), C.c_int32, C.c_int32, P(C.c_int64)]
         L.orc_tc_rows.restype = C.c_int64
+        L.orc_mxv.argtypes = [P(OrcCsr), C.c_int, P(C.c_float), P(C.c_uint8), P(C.c_float), P(C.c_uint8)]
+        L.orc_mxv.restype = None
         _lib = L
     return _lib
 
@@ -248,3 +250,14 @@ def tc_rows(l: Csr, r0: int, r1: int):
     w = C.c_int64()
     t = lib().orc_tc_rows(C.byref(s), r0, r1, C.byref(w))
     return int(t), int(w.value)
+
+
+def mxv(a: Csr, sr: int, x: np.ndarray, listed: Optional[np.ndarray] = None):
+    """General-semiring SpMSpV: returns (y float32[n], reached uint8[n])."""
+    s = _to_c(a)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty(a.n, dtype=np.float32)
+    r = np.empty(a.n, dtype=np.uint8)
+    ls = None if listed is None else np.ascontiguousarray(listed, dtype=np.uint8)
+    lib().orc_mxv(C.byref(s), sr, _ptr(x, C.c_float), _ptr(ls, C.c_uint8), _ptr(y, C.c_float), _ptr(r, C.c_uint8))
+    return y, r

@@ -9,6 +9,7 @@
  */
 #include "oracle.h"
 
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -587,3 +588,41 @@ int64_t orc_tc_rows(const orc_csr *l, int32_t r0, int32_t r1, int64_t *wedges) {
     if (wedges) *wedges = w;
     return tri;
 }
+
+/* General-semiring SpMSpV, restating the reference's mXv (src/mXv.cuh:110-389)
+   with the semirings the library offers: y[c] = (+) over present u with
+   (u, c) in A of x[u] (*) A(u, c).  sr: 0 plus-times (double accumulation,
+   identity 0), 1 min-plus (identity 1.70141e38, present -1 < x < 1e38), 2
+   max-times (identity -inf), 3 or-and (0 / 1).  listed (nullable) marks the
+   present entries of a sparse input; reached (nullable) receives the columns
+   any present entry reaches. */
+void orc_mxv(const orc_csr *a, int sr, const float *x, const uint8_t *listed, float *y, uint8_t *reached) {
+    const float inf_sentinel = 1.70141e38f;
+    double *acc = (double *)calloc((size_t)(a->n > 0 ? a->n : 1), sizeof(double));
+    for (int32_t c = 0; c < a->n; ++c) {
+        y[c] = sr == 1 ? inf_sentinel : (sr == 2 ? -INFINITY : 0.0f);
+        if (reached) reached[c] = 0;
+    }
+    for (int32_t u = 0; u < a->m; ++u) {
+        const float xu = x[u];
+        int present;
+        if (listed) present = listed[u] != 0;
+        else if (sr == 1) present = xu > -1.0f && xu < 1e38f;
+        else if (sr == 2) present = xu != -INFINITY;
+        else present = xu != 0.0f;
+        if (!present) continue;
+        for (int32_t e = a->row_ptr[u]; e < a->row_ptr[u + 1]; ++e) {
+            const int32_t c = a->col_ind[e];
+            const float w = a->val ? a->val[e] : 1.0f;
+            if (reached) reached[c] = 1;
+            if (sr == 0) acc[c] += (double)xu * (double)w;
+            else if (sr == 1) { const float t = xu + w; if (t < y[c]) y[c] = t; }
+            else if (sr == 2) { const float t = xu * w; if (t > y[c]) y[c] = t; }
+            else if (xu != 0.0f && w != 0.0f) y[c] = 1.0f;
+        }
+    }
+    if (sr == 0)
+        for (int32_t c = 0; c < a->n; ++c) y[c] = (float)acc[c];
+    free(acc);
+}
+

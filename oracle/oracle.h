@@ -100,6 +100,7 @@ int orc_strict_lower(const orc_csr *a, orc_csr *l);
  * (i,k) in L of nnz(L(k,:)). */
 int64_t orc_tc(const orc_csr *l, int64_t *wedges);
 int64_t orc_tc_rows(const orc_csr *l, int32_t r0, int32_t r1, int64_t *wedges);
+void orc_mxv(const orc_csr *a, int sr, const float *x, const uint8_t *listed, float *y, uint8_t *reached);
 
 #ifdef __cplusplus
 }

@@ -386,3 +386,33 @@ def test_rmat_quadrant_rule_brute_force():
     a = O.rmat(scale, ef, seed, False)
     got = {(r, int(c)) for r in range(a.m) for c in a.col_ind[a.row_ptr[r] : a.row_ptr[r + 1]]}
     assert got == edges
+
+
+# --- general-semiring SpMSpV ---------------------------------------------------
+
+def test_mxv_semirings_pinned():
+    """orc_mxv against independent computations: plus-times = dense A^T x in
+    double; min-plus = the min-plus step oracle; max-times and or-and by
+    brute force over the edge list; listed (sparse) presence by reach."""
+    a = O.rmat(9, 8, 79, False, 80)
+    rs = np.random.RandomState(6)
+    x = np.where(rs.rand(a.m) < 0.3, rs.randint(1, 9, a.m), 0).astype(np.float32)
+    D = np.zeros((a.m, a.n))
+    rows = np.repeat(np.arange(a.m), np.diff(a.row_ptr))
+    np.add.at(D, (rows, a.col_ind), a.val.astype(np.float64))
+    y, r = O.mxv(a, 0, x)
+    assert np.array_equal(y, (D.T @ x.astype(np.float64)).astype(np.float32))
+    xs = np.where(x > 0, x, np.float32(1.70141e38)).astype(np.float32)
+    assert np.array_equal(O.mxv(a, 1, xs)[0], O.sssp_step(a, xs))
+    want_max = np.full(a.n, -np.inf, np.float32)
+    want_or = np.zeros(a.n, np.float32)
+    for u, c, w in zip(rows, a.col_ind, a.val):
+        if x[u] != 0:
+            want_max[c] = max(want_max[c], x[u] * w)
+            want_or[c] = 1.0
+    xm = np.where(x != 0, x, -np.inf).astype(np.float32)
+    assert np.array_equal(O.mxv(a, 2, xm)[0], want_max)
+    assert np.array_equal(O.mxv(a, 3, x)[0], want_or)
+    listed = (rs.rand(a.m) < 0.05).astype(np.uint8)
+    _, reach = O.mxv(a, 0, np.zeros(a.m, np.float32), listed)
+    assert np.array_equal(reach.astype(bool), (D[listed.astype(bool)] != 0).any(axis=0))
