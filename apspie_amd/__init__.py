@@ -141,7 +141,7 @@ def lib() -> C.CDLL:
         "apspie_csr_attach_csc_device": [vp, P(_DCsr)],
         "apspie_coo_to_csr": [vp, vp, i64, i32, vp],
         "apspie_csr2csc": [vp, P(_DCsr), vp, vp, vp],
-        "apspie_mxv": [vp, P(_DCsr), C.c_int, vp, vp, P(Stats)],
+        "apspie_mxv": [vp, P(_DCsr), C.c_int, C.c_int, vp, vp, P(Stats)],
         "apspie_mxv_sparse": [vp, P(_DCsr), C.c_int, vp, vp, i32, vp, vp, P(i32), P(Stats)],
         "apspie_csr_load": [C.c_char_p, P(_Csr)],
         "apspie_host_csr_free": [P(_Csr)],
@@ -570,12 +570,14 @@ class Context:
         dd.free()
         return out, st
 
-    def mxv(self, g: DeviceCsr, x: np.ndarray, semiring: int = SR_PLUS_TIMES):
-        """apspie_mxv: dense x[m] -> dense y[n] over the semiring; returns (y, Stats)."""
+    def mxv(self, g: DeviceCsr, x: np.ndarray, semiring: int = SR_PLUS_TIMES, above_half: bool = False):
+        """apspie_mxv: dense x[m] -> dense y[n] over the semiring (present: x not the
+        identity, or x > 0.5 when above_half); returns (y, Stats)."""
         dx = self.alloc(4 * max(g.m, 1)).upload(np.asarray(x, dtype=np.float32))
         dy = self.alloc(4 * max(g.n, 1))
         st = Stats()
-        _check(lib().apspie_mxv(self.h, C.byref(g.c), semiring, dx.ptr, dy.ptr, C.byref(st)), self.h, "mxv")
+        _check(lib().apspie_mxv(self.h, C.byref(g.c), semiring, 1 if above_half else 0, dx.ptr, dy.ptr,
+                                C.byref(st)), self.h, "mxv")
         y = dy.download(np.float32, g.n)
         dx.free()
         dy.free()

@@ -12,8 +12,10 @@
 //   APSPIE_SR_MAX_TIMES   (max, *), identity -inf
 //   APSPIE_SR_OR_AND      (or, and) on 0 / 1, identity 0
 //
-// The dense form takes x[m] (present: differs from the identity; MIN_PLUS by
-// the bitify rule) and writes y[n] with the identity where nothing reaches.
+// The dense form takes x[m] (present: differs from the identity, MIN_PLUS by
+// spmspvSssp's bitify rule; or, asked for, x > 0.5, the bitify of mXv and
+// bfs, src/mXv.cuh:50-55) and writes y[n] with the identity where nothing
+// reaches.
 // The sparse form takes (index, value) pairs -- every listed index is present
 // -- and returns the reached columns in ascending order with their values,
 // as the reference's sparse mXv returns its reduced keys.
@@ -58,12 +60,13 @@ __device__ __forceinline__ float identity(int sr) {
 struct PickPresent {
     const float *x;
     const uint8_t *listed;  // sparse form: the listed indices (x holds their values)
-    int m, sr;
+    int m, sr, presence;
     __device__ bool pick(int v, uint32_t &payload) const {
         if (v >= m) return false;
         const float xv = x[v];
         payload = __float_as_uint(xv);
         if (listed) return listed[v] != 0;
+        if (presence == APSPIE_MXV_ABOVE_HALF) return xv > 0.5f;
         switch (sr) {
         case APSPIE_SR_MIN_PLUS: return xv > -1.0f && xv < 1e38f;
         case APSPIE_SR_MAX_TIMES: return xv != -INFINITY;
@@ -202,15 +205,15 @@ void check_sr(int sr) {
 
 // y = A^T x over the semiring; x dense in HBM (listed: the sparse form's
 // index marks, else presence by value); fills `reached` when asked
-void mxv(apspie_ctx *ctx, const apspie_csr *a, int sr, const float *x, const uint8_t *listed, float *y,
-         uint8_t *reached, apspie_stats *stats) {
+void mxv(apspie_ctx *ctx, const apspie_csr *a, int sr, int presence, const float *x, const uint8_t *listed,
+         float *y, uint8_t *reached, apspie_stats *stats) {
     const Bufs b = trav::get_bufs(ctx, a->m, a->nnz);
     double *ysum = (double *)ctx->slot(S_AUX4, size_t(std::max(a->n, 1)) * 8);
     uint32_t *yord = (uint32_t *)ysum;  // one of the two is used
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned gi = std::max(1u, std::min(ceil_div(std::max(std::max(a->n, a->m), 2 * b.nblk), 256), 4096u));
     k_init<<<gi, 256, 0, ctx->stream>>>(sr, a->n, ysum, yord, reached, b.nblk, b.status, b.C);
-    trav::k_compact<PickPresent><<<b.nblk, WGC, 0, ctx->stream>>>(PickPresent{x, listed, a->m, sr}, 0, a->m,
+    trav::k_compact<PickPresent><<<b.nblk, WGC, 0, ctx->stream>>>(PickPresent{x, listed, a->m, sr, presence}, 0, a->m,
                                                                     b.nblk, 1, a->row_ptr, trav::queue_of(b),
                                                                     b.status, b.C);
     trav::k_expand<OpSemiring><<<b.expand_grid, WG, 0, ctx->stream>>>(OpSemiring{a->val, sr, ysum, yord, reached},
@@ -228,7 +231,7 @@ using namespace apspie;
 
 extern "C" {
 
-int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, const float *d_x, float *d_y,
+int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, int presence, const float *d_x, float *d_y,
                apspie_stats *stats) {
     if (!ctx) return APSPIE_EINVAL;
     std::lock_guard<std::mutex> lk(ctx->mu);
@@ -236,7 +239,9 @@ int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, const float *
         validate_csr(a, false);
         srg::check_sr(semiring);
         APSPIE_CHECK(d_x && d_y, APSPIE_EINVAL, "null vectors");
-        srg::mxv(ctx, a, semiring, d_x, nullptr, d_y, nullptr, stats);
+        APSPIE_CHECK(presence == APSPIE_MXV_NOT_IDENTITY || presence == APSPIE_MXV_ABOVE_HALF, APSPIE_EINVAL,
+                     "unknown presence rule");
+        srg::mxv(ctx, a, semiring, presence, d_x, nullptr, d_y, nullptr, stats);
     });
 }
 
@@ -268,7 +273,7 @@ int apspie_mxv_sparse(apspie_ctx *ctx, const apspie_csr *a, int semiring, const 
         APSPIE_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         APSPIE_CHECK(hbad == 0, APSPIE_EINVAL, "vector index out of range");
-        mxv(ctx, a, semiring, x, listed, y, reached, stats);
+        mxv(ctx, a, semiring, APSPIE_MXV_NOT_IDENTITY, x, listed, y, reached, stats);
         const int nc = std::max(1, (int)ceil_div(n, 256 * 16));
         uint64_t *status = (uint64_t *)ctx->slot(S_AUX6, size_t(nc + 2) * 8);
         int32_t *dny = bad + 1;

@@ -276,6 +276,49 @@ inline long long tc(const int m, const int edgeL, const int *d_rowPtrL, const in
     return (long long)count;
 }
 
+// mXv, src/mXv.cuh:111 (dense form): y = sum over u with x[u] > 0.5 (bitify,
+// src/mXv.cuh:50-55) of x[u] * M(u, :), M given by the "csc" arrays read as
+// rows, as the reference gathers them; y = 0 where nothing reaches.  Returns
+// the expanded edges (the reference's `total`).  The scratch argument of the
+// reference is not needed.
+template <typename T>
+inline int mXv(const T *d_randVec, const int edge, const int m, const T *d_cscVal, const int *d_cscColPtr,
+               const int *d_cscRowInd, T *d_mmResult, void * /*d_scratch*/, mgpu::CudaContext &context) {
+    static_assert(sizeof(T) == sizeof(float), "FP32 vectors");
+    const apspie_csr g = apspie_ref::csr_of(m, edge, d_cscColPtr, d_cscRowInd, (const float *)d_cscVal);
+    apspie_stats st{};
+    apspie_ref::check(apspie_mxv(context.get(), &g, APSPIE_SR_PLUS_TIMES, APSPIE_MXV_ABOVE_HALF,
+                                 (const float *)d_randVec, (float *)d_mmResult, &st),
+                      context.get(), "mXv");
+    return int(st.edges);
+}
+
+// mXm, src/tc.cuh:314 (column-by-column SpGEMM over mXv): C(:, i) = A B(:, i)
+// for every column i, all three matrices in CSC (m x m).  In CSC terms that
+// is C^T = B^T A^T by rows, one row-wise SpGEMM of B's and A's arrays read as
+// rows.  C's arrays are allocated here and handed back through the
+// references (the reference allocates them into by-value pointers); returns
+// nnz(C).  h_cscColPtrB is not needed.
+template <typename T>
+inline int mXm(const int edge, const int m, const T *d_cscValA, const int *d_cscColPtrA, const int *d_cscRowIndA,
+               const T *d_cscValB, const int * /*h_cscColPtrB*/, const int *d_cscColPtrB, const int *d_cscRowIndB,
+               T *&d_cscValC, int *&d_cscColPtrC, int *&d_cscRowIndC, mgpu::CudaContext &context,
+               int edgeB = -1) {
+    static_assert(sizeof(T) == sizeof(float), "FP32 values");
+    apspie_ctx *c = context.get();
+    int nnzb = edgeB;
+    if (nnzb < 0)
+        apspie_ref::check(apspie_memcpy_d2h(c, &nnzb, d_cscColPtrB + m, sizeof(int)), c, "mXm nnz(B)");
+    const apspie_csr a = apspie_ref::csr_of(m, edge, d_cscColPtrA, d_cscRowIndA, (const float *)d_cscValA);
+    const apspie_csr b = apspie_ref::csr_of(m, nnzb, d_cscColPtrB, d_cscRowIndB, (const float *)d_cscValB);
+    apspie_csr_owned out{};
+    apspie_ref::check(apspie_spgemm(c, &b, &a, &out, nullptr), c, "mXm");
+    d_cscColPtrC = out.row_ptr;
+    d_cscRowIndC = out.col_ind;
+    d_cscValC = (T *)out.val;
+    return int(out.nnz);
+}
+
 // ---------------------------------------------------------------------------
 // loader: readMtx / makeSymmetric (src/util.cuh:370-643) as one call returning
 // host CSR arrays with the reference's conventions.

@@ -328,7 +328,11 @@ int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, apspie_stats
 #define APSPIE_SR_MIN_PLUS 1
 #define APSPIE_SR_MAX_TIMES 2
 #define APSPIE_SR_OR_AND 3
-int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, const float *d_x, float *d_y,
+/* presence of a dense x entry: it differs from the semiring's identity, or
+ * (the reference's mXv / bfs frontier, bitify in src/mXv.cuh:50-55) x > 0.5 */
+#define APSPIE_MXV_NOT_IDENTITY 0
+#define APSPIE_MXV_ABOVE_HALF 1
+int apspie_mxv(apspie_ctx *ctx, const apspie_csr *a, int semiring, int presence, const float *d_x, float *d_y,
                apspie_stats *stats);
 /* Sparse form: the nx listed indices of x are present (any value); the
  * reached columns come back in ascending order (d_yind, d_yval, room for n)

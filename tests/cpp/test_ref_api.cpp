@@ -1,6 +1,6 @@
 // Drives the reference-named C++ host API (apspie_amd/host/apspie_ref.hpp)
 // the way the reference's test drivers do (test/bfs, sssp, mis, spgemm, tc
-// main.cu) and checks every result against the CPU oracle.  Prints
+// main.cu, plus mXv and mXm) and checks every result against the CPU oracle.  Prints
 // "REF_API OK" and exits 0 when all checks pass.
 #include <hip/hip_runtime.h>
 
@@ -143,6 +143,44 @@ int main(int argc, char **argv) {
         const long long t = tc(m, int(l.nnz), d_lrp, d_lci, *context);
         EXPECT(t == orc_tc(&ol, nullptr), "tc");
         apspie_host_csr_free(&l);
+        // mXv (src/mXv.cuh:111): entries above 0.5 are the frontier
+        std::vector<float> hx(m), xh(m), yo(m);
+        long long want_tot = 0;
+        for (int v = 0; v < m; ++v) {
+            hx[v] = v % 3 == 0 ? 1.0f + float(v % 4) : (v % 3 == 1 ? 0.25f : 0.0f);
+            xh[v] = hx[v] > 0.5f ? hx[v] : 0.0f;
+            if (hx[v] > 0.5f) want_tot += h.row_ptr[v + 1] - h.row_ptr[v];
+        }
+        float *d_x = dev_copy(hx.data(), size_t(m)), *d_av = dev_copy(cv.data(), size_t(edge)), *d_y = nullptr;
+        (void)hipMalloc(&d_y, size_t(m) * 4);
+        const int tot = mXv(d_x, edge, m, d_av, d_rp, d_ci, d_y, nullptr, *context);
+        orc_csr ov;
+        ov.m = m;
+        ov.n = m;
+        ov.nnz = edge;
+        ov.row_ptr = h.row_ptr;
+        ov.col_ind = h.col_ind;
+        ov.val = cv.data();
+        orc_mxv(&ov, 0, xh.data(), nullptr, yo.data(), nullptr);
+        EXPECT(host_copy(d_y, size_t(m)) == yo && tot == want_tot, "mXv");
+        // mXm (src/tc.cuh:314): A's arrays read as CSC twice; the result's
+        // arrays read as rows are A * A
+        float *d_cval = nullptr;
+        int *d_ccp = nullptr, *d_cri = nullptr;
+        const int nnzc = mXm(edge, m, d_av, d_rp, d_ci, d_av, h.row_ptr, d_rp, d_ci, d_cval, d_ccp, d_cri, *context);
+        orc_csr om;
+        int64_t pm = 0;
+        orc_spgemm(&ov, &ov, &om, &pm);
+        const std::vector<int> gcp = host_copy(d_ccp, size_t(m) + 1), gri = host_copy(d_cri, size_t(nnzc));
+        const std::vector<float> gcv = host_copy(d_cval, size_t(nnzc));
+        bool mm = nnzc == om.nnz;
+        for (int i = 0; mm && i <= m; ++i) mm &= gcp[i] == om.row_ptr[i];
+        for (int q = 0; mm && q < nnzc; ++q)
+            mm &= gri[q] == om.col_ind[q] && std::fabs(gcv[q] - om.val[q]) <= 1e-5f * std::fabs(om.val[q]);
+        EXPECT(mm, "mXm");
+        orc_csr_free(&om);
+        for (void *q : {(void *)d_x, (void *)d_av, (void *)d_y, (void *)d_cval, (void *)d_ccp, (void *)d_cri})
+            (void)hipFree(q);
         matrix_delete(&A);
         matrix_delete(&D);
         matrix_delete(&E);

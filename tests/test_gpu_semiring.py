@@ -75,3 +75,16 @@ def test_mxv_empty_inputs(ctx):
         ctx.mxv_sparse(g, np.array([h.m], np.int32), np.array([1.0], np.float32))
     with pytest.raises(A.ApspieError):
         ctx.mxv(g, np.zeros(h.m, np.float32), 7)
+
+
+def test_mxv_reference_frontier_rule(ctx):
+    """The reference mXv's frontier is x > 0.5 (bitify, src/mXv.cuh:50-55):
+    entries of 0.25 are left out, as if they were 0."""
+    h = A.rmat(11, 8, 6511, weight_seed=7511)
+    g = ctx.upload(h)
+    rs = np.random.RandomState(8)
+    x = rs.choice(np.array([0.0, 0.25, 0.5, 1.0, 3.0], np.float32), h.m)
+    y, st = ctx.mxv(g, x, A.SR_PLUS_TIMES, above_half=True)
+    want, _ = O.mxv(ocsr(h), A.SR_PLUS_TIMES, np.where(x > 0.5, x, 0).astype(np.float32))
+    assert np.array_equal(y, want)
+    assert st.edges == int(np.diff(h.row_ptr)[x > 0.5].sum())
