@@ -37,7 +37,8 @@ constexpr int BIN_DEG[NB - 1] = {256, 4096};
 
 __global__ __launch_bounds__(256) void k_setup(int64_t nnz, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ col, int32_t *ew, int nscan,
-                                               uint64_t *status, int32_t *counts, unsigned long long *acc) {
+                                               uint64_t *status, int32_t *counts, unsigned long long *acc,
+                                               int32_t *bad) {
     const int64_t stride = int64_t(gridDim.x) * 256;
     for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nnz; e += stride) {
         const int k = col[e];
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(256) void k_setup(int64_t nnz, const int32_t *__res
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < nscan + 2; i += stride) status[i] = 0;
     if (blockIdx.x == 0 && threadIdx.x < 16) counts[threadIdx.x] = 0;
     if (blockIdx.x == 0 && threadIdx.x < 4) acc[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *bad = 0;  // the slot keeps earlier calls' bytes
 }
 
 // exclusive scan of n int32 into int64 out[0..n] (out[n] = total), one pass
@@ -297,7 +299,7 @@ extern "C" int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, a
         const int nscan = std::max(1, (int)ceil_div(nnz, 256 * 16));
         uint64_t *status = (uint64_t *)ctx->slot(S_STATUS, size_t(nscan + 2) * 8);
         const unsigned g = std::max(1u, std::min(ceil_div(std::max<int64_t>(nnz, m) + 2, 256), 8192u));
-        k_setup<<<g, 256, 0, ctx->stream>>>(nnz, l->row_ptr, l->col_ind, ew, nscan, status, counts, acc);
+        k_setup<<<g, 256, 0, ctx->stream>>>(nnz, l->row_ptr, l->col_ind, ew, nscan, status, counts, acc, bad);
         k_scan64<<<nscan, 256, 0, ctx->stream>>>(ew, gpre, nnz, status, (unsigned *)(status + nscan));
         const unsigned gm = std::max(1u, std::min(ceil_div(std::max(m, 1), 256), 4096u));
         k_items<<<gm, 256, 0, ctx->stream>>>(0, m, l->row_ptr, gpre, counts, nullptr, nullptr, bad);

@@ -74,3 +74,16 @@ def test_single_edge(ctx):
         want = O.sssp_dijkstra(o, s)
         want[want == -1.0] = INF
         assert np.array_equal(row, want)
+
+
+def test_tc_after_scratch_reuse(ctx):
+    """TC's row-limit flag lives in a scratch slot that other entry points fill
+    (the device CSR->CSC keeps row indices there): a count that follows such a
+    call must still see the flag cleared and return the oracle's count."""
+    h = A.rmat(10, 8, seed=5, undirected=True)
+    ctx.csr2csc(ctx.upload(h), values=True)  # leaves nonzero row indices in the shared slot
+    lo = A.strict_lower(h)
+    t, _ = ctx.tc(ctx.upload(lo))
+    assert t == O.tc(O.Csr(lo.m, lo.n, lo.row_ptr, lo.col_ind, None))[0]
+    t0, _ = ctx.tc(ctx.upload(empty(9)))
+    assert t0 == 0
