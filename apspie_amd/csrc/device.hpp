@@ -36,6 +36,53 @@ __device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t x, int lane) {
     return x;
 }
 
+// Inclusive max-scan of non-negative ints across a full wave, on the VALU:
+// Hillis-Steele steps inside each 16-lane row (DPP row_shr 1, 2, 4, 8; lanes
+// shifted in from before the row read 0), then lane 15 of rows 0 and 2 into
+// rows 1 and 3 (row_bcast:15), then lane 31 into rows 2 and 3
+// (row_bcast:31).  Six DPP moves instead of six LDS-routed shuffles, each of
+// which waits a full LDS round trip.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_incl_max_nonneg(int x) {
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true));  // row_shr:1
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true));  // row_shr:2
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true));  // row_shr:4
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true));  // row_shr:8
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return x;
+}
+
+// The same scan for sums of uint32 (0 is the identity of both).
+__device__ __forceinline__ uint32_t wave_incl_sum_dpp(uint32_t x) {
+    int v = int(x);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);
+    return uint32_t(v);
+}
+
+// Lane 63's value, as a wave-uniform scalar.
+__device__ __forceinline__ int wave_last(int x) { return __builtin_amdgcn_readlane(x, WAVE - 1); }
+
+// Wave totals through the DPP scans (all 64 lanes active); the result is
+// wave-uniform.
+__device__ __forceinline__ uint32_t wave_total32(uint32_t x) { return uint32_t(wave_last(int(wave_incl_sum_dpp(x)))); }
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
+    // lanes shifted in from outside a row, and rows outside the mask, read
+    // the identity ~0u (bound_ctrl off keeps the `old` operand)
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x111, 0xf, 0xf, false)));
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x112, 0xf, 0xf, false)));
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x114, 0xf, 0xf, false)));
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x118, 0xf, 0xf, false)));
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x142, 0xa, 0xf, false)));
+    x = min(x, uint32_t(__builtin_amdgcn_update_dpp(-1, int(x), 0x143, 0xc, 0xf, false)));
+    return uint32_t(wave_last(int(x)));
+}
+
 // Sum across the wave; every lane receives the total.
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
 #pragma unroll

@@ -190,11 +190,10 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
             lane_es += d;
         }
     }
-    const uint32_t wcnt = wave_sum32(lane_cnt), wes = wave_sum32(lane_es), wnew = wave_sum32(lane_new);
+    const uint32_t wcnt = wave_total32(lane_cnt), wes = wave_total32(lane_es), wnew = wave_total32(lane_new);
     if constexpr (has_defer<Pick>::value) {
-        ndef = wave_sum32(ndef);
-#pragma unroll
-        for (int d = WAVE / 2; d >= 1; d >>= 1) dmin = min(dmin, (uint32_t)__shfl_xor((int)dmin, d, WAVE));
+        ndef = wave_total32(ndef);
+        dmin = wave_min_u32(dmin);
     }
     if (lane == 0) {
         s_wsum[wid] = (uint64_t(wcnt) << 31) | wes;
@@ -244,7 +243,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         const int rank =
             __builtin_amdgcn_mbcnt_hi(uint32_t(qball >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qball), 0));
         const uint32_t d = qd ? uint32_t(dg[r]) : 0u;
-        const uint32_t dincl = wave_incl_scan32(d, lane);
+        const uint32_t dincl = wave_incl_sum_dpp(d);
         if (qd) {
             const int p = pos + rank;
             const int o = off + int(dincl - d);
@@ -256,7 +255,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
             for (int t = (o + TILE - 1) / TILE; t * TILE < o + int(d); ++t) q.tiles[t] = p;
         }
         pos += __popcll(qball);
-        off += int(__shfl(dincl, WAVE - 1, WAVE));
+        off += wave_last(int(dincl));
     }
     if (tid == 0) {
         if (bnew) {
@@ -299,8 +298,7 @@ __device__ __forceinline__ void slot_max_scan(int (&x)[IPT], int32_t *s_wmax) {
     int mx = 0;
 #pragma unroll
     for (int r = 0; r < IPT; ++r) mx = max(mx, x[r]);
-#pragma unroll
-    for (int d = WAVE / 2; d >= 1; d >>= 1) mx = max(mx, __shfl_xor(mx, d, WAVE));
+    mx = wave_last(wave_incl_max_nonneg(mx));
     if (lane == 0) s_wmax[wid] = mx;
     __syncthreads();
     int carry = 0;
@@ -309,14 +307,9 @@ __device__ __forceinline__ void slot_max_scan(int (&x)[IPT], int32_t *s_wmax) {
         if (w < wid) carry = max(carry, s_wmax[w]);
 #pragma unroll
     for (int r = 0; r < IPT; ++r) {
-        int xo = max(x[r], carry);
-#pragma unroll
-        for (int d = 1; d < WAVE; d <<= 1) {
-            const int y = __shfl_up(xo, d, WAVE);
-            if (lane >= d) xo = max(xo, y);
-        }
+        const int xo = wave_incl_max_nonneg(max(x[r], carry));
         x[r] = xo;
-        carry = __shfl(xo, WAVE - 1, WAVE);
+        carry = wave_last(xo);
     }
 }
 
