@@ -161,6 +161,7 @@ def lib() -> C.CDLL:
         "apspie_spmspv_or": [vp, P(_DCsr), vp, vp],
         "apspie_powerlaw_csr": [i32, i32, C.c_double, i32, u32, P(_Csr)],
         "apspie_spgemm": [vp, P(_DCsr), P(_DCsr), P(_DCsr), P(Stats)],
+        "apspie_spgemm_outer": [vp, P(_DCsr), P(_DCsr), C.c_int32, P(_DCsr), P(Stats)],
         "apspie_strict_lower": [P(_Csr), P(_Csr)],
         "apspie_tc": [vp, P(_DCsr), P(i64), P(Stats)],
         "apspie_bfs_batch": [vp, P(_DCsr), vp, i32, vp, P(Stats)],
@@ -539,6 +540,23 @@ class Context:
     def spgemm(self, a: DeviceCsr, b: DeviceCsr):
         """C = A * B; returns (HostCsr C, Stats)."""
         c, st = self.spgemm_device(a, b)
+        try:
+            return self.download_csr(c), st
+        finally:
+            self.free_owned(c)
+
+    def spgemm_outer_device(self, a: DeviceCsr, b: DeviceCsr, part_size: int = 65536):
+        """C = A * B^T by the strip-pair outer products (apspie_spgemm_outer),
+        left on the device: returns (_DCsr owned by the library, Stats)."""
+        c = _DCsr()
+        st = Stats()
+        _check(lib().apspie_spgemm_outer(self.h, C.byref(a.c), C.byref(b.c), int(part_size), C.byref(c),
+                                         C.byref(st)), self.h, "spgemm_outer")
+        return c, st
+
+    def spgemm_outer(self, a: DeviceCsr, b: DeviceCsr, part_size: int = 65536):
+        """C = A * B^T by the strip-pair outer products; returns (HostCsr C, Stats)."""
+        c, st = self.spgemm_outer_device(a, b, part_size)
         try:
             return self.download_csr(c), st
         finally:

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "convert.hpp"
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -208,6 +209,52 @@ void compress(apspie_ctx *ctx, const int32_t *key, int64_t n, int m, int32_t *of
     APSPIE_CHECK(h != 2, APSPIE_EINVAL, "row index out of range");
 }
 
+void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) {
+    const int nscan = std::max(1, (int)ceil_div(n, TILE_N));
+    uint64_t *status = (uint64_t *)ctx->slot(S_STATUS, size_t(nscan + 2) * 8);
+    k_zero64<<<grid_for(ctx, nscan + 2, 256), 256, 0, ctx->stream>>>(status, nscan + 2);
+    k_scan<<<nscan, 256, 0, ctx->stream>>>(in, out, n, status, (unsigned *)(status + nscan));
+    APSPIE_HIP(hipGetLastError());
+}
+
+int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits) {
+    if (n == 0) return 0;
+    const int ntiles = (int)ceil_div(n, TILE_N);
+    const int64_t nh = int64_t(RADIX) * ntiles;
+    int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
+    int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
+    const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
+        k_hist<<<gt, T, 0, ctx->stream>>>(key[cur], n, shift, ntiles, hist);
+        scan32(ctx, hist, offs, nh);
+        k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], n, shift, ntiles, offs, key[cur ^ 1], idx[cur ^ 1]);
+    }
+    APSPIE_HIP(hipGetLastError());
+    return cur;
+}
+
+void csr2csc_into(apspie_ctx *ctx, const apspie_csr *a, int32_t *d_col_ptr, int32_t *d_row_ind, float *d_val) {
+    const int64_t n = a->nnz;
+    if (n == 0) {
+        APSPIE_HIP(hipMemsetAsync(d_col_ptr, 0, (size_t(a->n) + 1) * 4, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        return;
+    }
+    int32_t *key[2] = {(int32_t *)ctx->slot(S_AUX0, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX1, size_t(n) * 4)};
+    int32_t *idx[2] = {(int32_t *)ctx->slot(S_AUX2, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX3, size_t(n) * 4)};
+    k_seed<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(a->col_ind, n, key[0], idx[0]);
+    int bits = 1;
+    while (bits < 31 && (int64_t(1) << bits) < int64_t(a->n)) ++bits;
+    const int cur = radix_pairs(ctx, key, idx, n, bits);
+    compress(ctx, key[cur], n, a->n, d_col_ptr);
+    int32_t *rowid = key[cur ^ 1];  // free after the last pass
+    k_row_ids<<<grid_for(ctx, int64_t(a->m) * WAVE, 256), 256, 0, ctx->stream>>>(a->m, a->row_ptr, rowid);
+    k_gather<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(idx[cur], n, rowid, a->val, d_row_ind, d_val);
+    APSPIE_HIP(hipGetLastError());
+    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+}
+
 }  // namespace cvt
 }  // namespace apspie
 
@@ -233,39 +280,7 @@ int apspie_csr2csc(apspie_ctx *ctx, const apspie_csr *a, int32_t *d_col_ptr, int
         validate_csr(a, false);
         APSPIE_CHECK(d_col_ptr && (a->nnz == 0 || d_row_ind), APSPIE_EINVAL, "null CSC arrays");
         APSPIE_CHECK(!d_val || a->val || a->nnz == 0, APSPIE_EINVAL, "values asked of a matrix without values");
-        const int64_t n = a->nnz;
-        if (n == 0) {
-            APSPIE_HIP(hipMemsetAsync(d_col_ptr, 0, (size_t(a->n) + 1) * 4, ctx->stream));
-            APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-            return;
-        }
-        int32_t *key[2] = {(int32_t *)ctx->slot(S_AUX0, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX1, size_t(n) * 4)};
-        int32_t *idx[2] = {(int32_t *)ctx->slot(S_AUX2, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX3, size_t(n) * 4)};
-        const int ntiles = (int)ceil_div(n, TILE_N);
-        const int64_t nh = int64_t(RADIX) * ntiles;
-        int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
-        int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
-        const int nscan = (int)ceil_div(nh, TILE_N);
-        uint64_t *status = (uint64_t *)ctx->slot(S_STATUS, size_t(nscan + 2) * 8);
-        const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
-        k_seed<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(a->col_ind, n, key[0], idx[0]);
-        int bits = 1;
-        while (bits < 31 && (int64_t(1) << bits) < int64_t(a->n)) ++bits;
-        int cur = 0;
-        for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
-            k_hist<<<gt, T, 0, ctx->stream>>>(key[cur], n, shift, ntiles, hist);
-            k_zero64<<<grid_for(ctx, nscan + 2, 256), 256, 0, ctx->stream>>>(status, nscan + 2);
-            k_scan<<<nscan, 256, 0, ctx->stream>>>(hist, offs, nh, status, (unsigned *)(status + nscan));
-            k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], n, shift, ntiles, offs, key[cur ^ 1],
-                                                 idx[cur ^ 1]);
-        }
-        APSPIE_HIP(hipGetLastError());
-        compress(ctx, key[cur], n, a->n, d_col_ptr);
-        int32_t *rowid = key[cur ^ 1];  // free after the last pass
-        k_row_ids<<<grid_for(ctx, int64_t(a->m) * WAVE, 256), 256, 0, ctx->stream>>>(a->m, a->row_ptr, rowid);
-        k_gather<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(idx[cur], n, rowid, a->val, d_row_ind, d_val);
-        APSPIE_HIP(hipGetLastError());
-        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        csr2csc_into(ctx, a, d_col_ptr, d_row_ind, d_val);
     });
 }
 

@@ -1,0 +1,36 @@
+// convert.hpp -- host helpers of the device format conversions (convert.hip)
+// shared with the other sparse kernels.
+#pragma once
+
+#include <cstdint>
+
+#include "internal.hpp"
+
+namespace apspie {
+namespace cvt {
+
+// Workgroups for a grid-stride loop over n items, `per` items per workgroup
+// per step, capped at 32 per CU.
+unsigned grid_for(apspie_ctx *ctx, int64_t n, unsigned per);
+
+// off[r] = first position e with key[e] >= r, r in [0, m] (keys sorted, in
+// [0, m); APSPIE_EINVAL otherwise).  Synchronises the stream.
+void compress(apspie_ctx *ctx, const int32_t *key, int64_t n, int m, int32_t *off);
+
+// out[0..n] = exclusive scan of n non-negative int32 (out[n] = total); the
+// caller guarantees the total stays below 2^31.  Uses the S_STATUS slot.
+void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);
+
+// Stable least-significant-digit radix sort of the pairs (key[0][e],
+// idx[0][e]) by the low `bits` bits of the key (non-negative keys), 8 bits a
+// pass, ping-ponging between the two buffers of each array.  Returns which
+// buffer (0 or 1) holds the result.  Uses the S_AUX4, S_AUX5 and S_STATUS
+// slots.
+int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits);
+
+// The CSC of a (rows ascending in each column) into caller memory; cv may be
+// null.  Uses the S_AUX0..S_AUX5 and S_STATUS slots.
+void csr2csc_into(apspie_ctx *ctx, const apspie_csr *a, int32_t *cp, int32_t *ri, float *cv);
+
+}  // namespace cvt
+}  // namespace apspie

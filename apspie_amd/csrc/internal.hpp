@@ -64,7 +64,8 @@ enum Slot : int {
     S_AUX5,
     S_AUX6,
     S_AUX7,
-    S_NSLOTS
+    S_OUTER0,                   // outer-product SpGEMM working arrays: S_OUTER0 + k, k < 40
+    S_NSLOTS = S_OUTER0 + 40
 };
 
 }  // namespace apspie

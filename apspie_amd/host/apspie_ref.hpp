@@ -266,6 +266,25 @@ inline void spgemm(d_matrix *C, d_matrix *A, d_matrix *B, const int /*partSize*/
     if (!C->h_cscColPtr) C->h_cscColPtr = (int *)calloc(size_t(C->m) + 1, sizeof(int));
 }
 
+// The reference's spgemm algorithm itself (src/spgemm.cuh:131-342): the same
+// C = A * B^T by outer products of matching columns over row strips of
+// partSize rows, for comparison with spgemm above.  partNum follows from
+// partSize and the row counts.
+inline void spgemmOuter(d_matrix *C, d_matrix *A, d_matrix *B, const int partSize, const int /*partNum*/,
+                        mgpu::CudaContext &context) {
+    apspie_ctx *c = context.get();
+    const apspie_csr a = apspie_csr_view(&A->owned), b = apspie_csr_view(&B->owned);
+    if (C->owned.row_ptr) apspie_csr_free(c, &C->owned);
+    apspie_ref::check(apspie_spgemm_outer(c, &a, &b, partSize, &C->owned, nullptr), c, "spgemmOuter");
+    C->m = int(C->owned.m);
+    C->n = int(C->owned.n);
+    C->nnz = int(C->owned.nnz);
+    C->d_cscColPtr = C->owned.row_ptr;
+    C->d_cscRowInd = C->owned.col_ind;
+    C->d_cscVal = C->owned.val;
+    if (!C->h_cscColPtr) C->h_cscColPtr = (int *)calloc(size_t(C->m) + 1, sizeof(int));
+}
+
 // Triangle counting, the pipeline of test/tc/main.cu:148-261 on the strict
 // lower triangle L (CSR, sorted columns): exact int64 count.
 inline long long tc(const int m, const int edgeL, const int *d_rowPtrL, const int *d_colIndL,

@@ -270,6 +270,31 @@ def bench_matrix_ops(ctx, cpu=True):
         "bytes_alg": st.bytes_alg,
         "hbm_frac": round(st.bytes_alg / (st.kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    # the reference's own scheme on the same product: outer products of A's
+    # and (A^T)^T's matching columns over strips of 128000 / (mean row
+    # length) rows (the strip size rule of test/spgemm/main.cu:362-375)
+    part = max(1, int(128000 / (h.nnz / h.m)))
+    at = ctx.upload(A.transpose(h))
+
+    def spgemm_outer():
+        c, st = ctx.spgemm_outer_device(a, at, part)
+        ctx.free_owned(c)
+        return st
+
+    wall_o, st_o = timed(spgemm_outer)
+    c_row, _ = ctx.spgemm(a, a)
+    c_out, _ = ctx.spgemm_outer(a, at, part)
+    same_o = bool(np.array_equal(c_row.row_ptr, c_out.row_ptr) and np.array_equal(c_row.col_ind, c_out.col_ind)
+                  and np.allclose(c_row.val, c_out.val, rtol=1e-5, atol=0))
+    del c_row, c_out, at
+    res["spgemm_outer"] = {
+        "workload": "spgemm_powerlaw_1M_mean8_max4096_AxA", "algorithm": "strip-pair outer products (DCSC)",
+        "part_size": part, "column_matches": st_o.improved, "products": st_o.edges, "nnz_c": st_o.vertices,
+        "ms": round(wall_o * 1e3, 3), "device_ms": round(st_o.kernel_ms, 3),
+        "gflops": round(2 * st_o.edges / st_o.kernel_ms / 1e6, 2),
+        "rowwise_speedup": round(st_o.kernel_ms / st.kernel_ms, 2), "matches_rowwise": same_o,
+    }
+    log(f"spgemm outer: {res['spgemm_outer']}")
     if cpu:
         import oracle as O
 

@@ -309,6 +309,21 @@ int apspie_spmspv_or(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_in, 
 int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
                   apspie_stats *stats);
 
+/* C = A * B^T, C(r, c) = sum_k A(r, k) * B(c, k), by the reference's own
+ * scheme: row strips of part_size rows of A and of B in DCSC, the column
+ * intersection of every strip pair, and the outer products of the matched
+ * columns (spgemm, src/spgemm.cuh:131-342; csr_to_dcsc,
+ * src/matrix.cpp:286-468).  Kept for comparison with apspie_spgemm; the
+ * products are sorted and summed in FP64 in increasing k (deterministic, the
+ * same sums as a row-wise product in double) where the reference hashes them
+ * with FP32 atomics.  A and B must have the same number of columns; C (m(A) x
+ * m(B), sorted columns, structural zeros kept) is allocated by the library.
+ * part_size >= 1 and ceil(m(A)/part_size) * ceil(m(B)/part_size) < 2^31.
+ * stats: edges = products, vertices = nnz(C), improved = strip-pair column
+ * matches.  APSPIE_ELIMIT when the products or matches reach 2^31. */
+int apspie_spgemm_outer(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, int32_t part_size,
+                        apspie_csr_owned *c, apspie_stats *stats);
+
 /* Triangle count of an undirected graph from its strict lower triangle L
  * (row i holds the sorted columns j < i): *count = sum over (i,k) in L of
  * |L(i,:) & L(k,:)|, exact int64.  stats: edges = wedges checked,

@@ -134,6 +134,16 @@ int main(int argc, char **argv) {
             sg &= E.h_cscRowInd[p] == oc.col_ind[p] &&
                   std::fabs(E.h_cscVal[p] - oc.val[p]) <= 1e-5f * std::fabs(oc.val[p]);
         EXPECT(sg, "spgemm");
+        // the reference's own algorithm, strips of 3 rows: the same product
+        d_matrix E2;
+        matrix_new(&E2, m, m);
+        spgemmOuter(&E2, &A, &D, 3, (m + 2) / 3, *context);
+        copy_matrix_device(&E2);
+        bool so = E2.nnz == oc.nnz;
+        for (int i = 0; so && i <= m; ++i) so &= E2.h_cscColPtr[i] == oc.row_ptr[i];
+        for (int p = 0; so && p < E2.nnz; ++p)
+            so &= E2.h_cscRowInd[p] == oc.col_ind[p] && E2.h_cscVal[p] == oc.val[p];
+        EXPECT(so, "spgemmOuter");
         orc_csr_free(&oc);
         // TC (test/tc/main.cu) on the strict lower triangle
         apspie_host_csr l{};
