@@ -411,7 +411,7 @@ class Context:
         _check(lib().apspie_ctx_synchronize(self.h), self.h, "synchronize")
 
     # -- per-kernel-class event timing ---------------------------------------
-    KCLASS = {"expand": 0, "compact": 1, "other": 2, "pull": 3, "bin": 4, "apply": 5}
+    KCLASS = {"expand": 0, "compact": 1, "other": 2, "pull": 3, "bin": 4, "apply": 5, "round": 6}
 
     def set_profiling(self, on: bool) -> None:
         _check(lib().apspie_ctx_set_profiling(self.h, int(on)), self.h, "set_profiling")

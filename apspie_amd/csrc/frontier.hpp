@@ -364,20 +364,19 @@ struct op_batch<T, std::void_t<decltype(T::kBatch)>> {
     static constexpr int value = T::kBatch;
 };
 
+// The tiles t = bid, bid + nblk, ... of the frontier `cur` (block-uniform
+// arguments; every thread of the workgroup calls it).
 template <class Op>
-__global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, const Control *C, Queue q,
-                                               const int32_t *__restrict__ col_ind) {
+__device__ __forceinline__ void expand_tiles(const Op &op, int m, const Ctl &cur, const Queue &q,
+                                             const int32_t *__restrict__ col_ind, int bid, int nblk) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];  // start - (off - e0): col_ind index of slot 0
     __shared__ uint32_t s_val[Op::kVal ? TILE + 1 : 1];
     __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
     __shared__ int32_t s_wmax[NW];
-    const int level = level_arg >= 0 ? level_arg : C->cur_level;  // round of the next compaction
-    const Ctl cur = C->ctl[(level - 1) & 1];
-    if (cur.pull) return;  // this round runs as a pull step
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-    for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
+    for (int t = bid; t < cur.nTiles; t += nblk) {
         int own[IPT];
         const int elim = tile_owners<Op::kVal, Op::kVert>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
         int pos[IPT], col[IPT];
@@ -406,6 +405,15 @@ __global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, cons
         }
         __syncthreads();
     }
+}
+
+template <class Op>
+__global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, const Control *C, Queue q,
+                                               const int32_t *__restrict__ col_ind) {
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;  // round of the next compaction
+    const Ctl cur = C->ctl[(level - 1) & 1];
+    if (cur.pull) return;  // this round runs as a pull step
+    expand_tiles(op, m, cur, q, col_ind, blockIdx.x, gridDim.x);
 }
 
 }  // namespace trav

@@ -175,13 +175,10 @@ struct OpRelax {
 // the frontier bitmap and stops at the first one.  On RMAT graphs nearly all
 // vertices find one at their first in-edge (the hubs, lowest ids, come
 // first), so a thread per vertex keeps the wave converged.
-__global__ __launch_bounds__(WG) void k_pull_bfs(int level_arg, int m, const Control *C,
-                                                 const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
-                                                 const uint32_t *__restrict__ visited,
-                                                 const uint32_t *__restrict__ fbits, uint8_t *flags) {
-    const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    if (C->ctl[(level - 1) & 1].pull != DIR_PULL) return;
-    for (int v = blockIdx.x * WG + threadIdx.x; v < m; v += gridDim.x * WG) {
+__device__ __forceinline__ void pull_bfs_rows(int m, const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
+                                              const uint32_t *__restrict__ visited,
+                                              const uint32_t *__restrict__ fbits, uint8_t *flags, int bid, int nblk) {
+    for (int v = bid * WG + threadIdx.x; v < m; v += nblk * WG) {
         if ((visited[v >> 5] >> (v & 31)) & 1u) continue;
         const int a = cp[v], b = cp[v + 1];
         for (int e = a; e < b; ++e) {
@@ -192,6 +189,25 @@ __global__ __launch_bounds__(WG) void k_pull_bfs(int level_arg, int m, const Con
             }
         }
     }
+}
+
+// One BFS round in one launch: the bottom-up step over every block when the
+// compaction chose it, else the push expansion over the first expand_blocks
+// blocks (the rest exit).  Separate expansion and bottom-up launches would
+// leave one of the two with nothing to do every round, and a launch of a few
+// thousand workgroups that exit at once still costs several microseconds.
+__global__ __launch_bounds__(WG) void k_round_bfs(OpBfs op, int level_arg, int m, int expand_blocks,
+                                                  const Control *C, Queue q, const int32_t *__restrict__ col_ind,
+                                                  const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
+                                                  const uint32_t *__restrict__ fbits) {
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    const Ctl cur = C->ctl[(level - 1) & 1];
+    if (cur.pull == DIR_PULL) {
+        pull_bfs_rows(m, cp, ri, op.visited, fbits, op.flags, blockIdx.x, gridDim.x);
+        return;
+    }
+    if (cur.pull || int(blockIdx.x) >= expand_blocks) return;
+    expand_tiles(op, m, cur, q, col_ind, blockIdx.x, expand_blocks);
 }
 
 // Row holding the first edge of every TILE-edge tile of the CSC.
@@ -448,6 +464,14 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                            level, level == 0);
     };
     auto expand = [&](int level) {
+        if (MODE == MODE_BFS && pull) {
+            ctx->prof_begin(APSPIE_KCLASS_ROUND);
+            k_round_bfs<<<std::max(b.expand_grid, vert_grid), WG, 0, ctx->stream>>>(
+                OpBfs{b.visited, b.flags}, level, g->m, int(b.expand_grid), b.C, queue_of(b), g->col_ind,
+                g->csc_col_ptr, g->csc_row_ind, b.fbits);
+            ctx->prof_end();
+            return;
+        }
         if (MODE == MODE_BFS)
             launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
         else
@@ -464,12 +488,8 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         }
         if (!pull) return;
         ctx->prof_begin(APSPIE_KCLASS_PULL);
-        if (MODE == MODE_BFS)
-            k_pull_bfs<<<vert_grid, WG, 0, ctx->stream>>>(level, g->m, b.C, g->csc_col_ptr, g->csc_row_ind,
-                                                          b.visited, b.fbits, b.flags);
-        else
-            k_pull_relax<<<pull_grid, WG, 0, ctx->stream>>>(level, g->m, nnz32, b.C, g->csc_col_ptr,
-                                                            g->csc_row_ind, g->csc_val, b.trows, fval, dist);
+        k_pull_relax<<<pull_grid, WG, 0, ctx->stream>>>(level, g->m, nnz32, b.C, g->csc_col_ptr, g->csc_row_ind,
+                                                        g->csc_val, b.trows, fval, dist);
         ctx->prof_end();
     };
     compact(0, src);

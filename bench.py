@@ -92,6 +92,8 @@ def bfs_class_bytes(rounds, m):
             out["pull"] += m / 8.0 + 12.0 * (m - seen) + nxt
         else:
             out["expand"] += 8.0 * e + 8.0 * nf + nxt
+    # with a CSC, one launch per round runs either form (k_round_bfs)
+    out["round"] = out["expand"] + out["pull"]
     return out
 
 
@@ -99,7 +101,7 @@ KERNEL_NAMES = {
     ("sssp", "expand"): "k_expand<OpRelax>",
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",
-    ("bfs", "pull"): "k_pull_bfs",
+    ("bfs", "round"): "k_round_bfs",
     ("sssp", "bin"): "k_bin_relax",
     ("sssp", "apply"): "k_apply_relax",
 }

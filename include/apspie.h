@@ -124,6 +124,7 @@ int apspie_ctx_cu_count(const apspie_ctx *ctx);
 #define APSPIE_KCLASS_PULL 3     /* pull steps over in-edges (BFS bottom-up, SSSP pull) */
 #define APSPIE_KCLASS_BIN 4      /* SSSP binned relaxation: candidates written by bucket */
 #define APSPIE_KCLASS_APPLY 5    /* SSSP binned relaxation: buckets folded into the distances */
+#define APSPIE_KCLASS_ROUND 6    /* BFS with a CSC: one launch per round, push or bottom-up */
 #define APSPIE_KCLASS_COUNT 8
 int apspie_ctx_set_profiling(apspie_ctx *ctx, int enable);
 /* Totals since the last reset: device milliseconds and launch count. */

@@ -191,14 +191,24 @@ def test_push_and_pull_agree_and_thresholds_round_trip(tuned):
 
 
 def test_pull_rounds_are_timed_separately(tuned):
+    """BFS with a CSC runs one launch per round (class "round": the push tiles
+    or the bottom-up step); SSSP with pull rounds launches the expansion and
+    the pull step side by side, each timed in its own class."""
     ctx = tuned
-    h = A.rmat(14, 16, 77, undirected=True)
+    ctx.set_tuning(**ALWAYS)
+    h = A.rmat(14, 16, 77, undirected=True, weight_seed=5)
     g = with_csc(ctx, h, True)
     ctx.set_profiling(True)
     ctx.profile_reset()
     try:
-        ctx.bfs(g, 0)
-        prof = ctx.profile()
+        _, st = ctx.bfs(g, 0)
+        prof_bfs = ctx.profile()
+        ctx.profile_reset()
+        ctx.sssp(g, 0)
+        prof_sssp = ctx.profile()
     finally:
         ctx.set_profiling(False)
-    assert prof["pull"][1] > 0 and prof["expand"][1] == prof["pull"][1]
+    assert prof_bfs["round"][1] > 0 and prof_bfs["expand"][1] == 0 and prof_bfs["pull"][1] == 0
+    assert prof_bfs["round"][1] == prof_bfs["compact"][1] - 1  # one per round after the seed's compaction
+    assert prof_sssp["pull"][1] > 0 and prof_sssp["expand"][1] == prof_sssp["pull"][1]
+    assert prof_sssp["round"][1] == 0
