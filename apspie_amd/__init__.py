@@ -82,6 +82,7 @@ class Tuning(C.Structure):
         ("batch_chunk", C.c_int32),
         ("sssp_batch_lanes", C.c_int32),
         ("sssp_delta", C.c_int32),
+        ("spgemm_long_rows", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -29,12 +29,16 @@
 
 #include "convert.hpp"
 #include "device.hpp"
+#include "esc.hpp"
 #include "internal.hpp"
 
 namespace apspie {
 namespace outer {
 
 using namespace dev;
+using esc::bits_for;
+using esc::k_rows_by;
+using esc::k_runflag;
 
 constexpr int TS = 1024;  // product slots per tile (256 threads x 4)
 
@@ -188,24 +192,6 @@ __global__ __launch_bounds__(256) void k_products(int64_t ptot, int64_t ntr, int
     }
 }
 
-// key[s] = the row of the product perm[s], idx[s] = perm[s] (perm may be idx)
-__global__ __launch_bounds__(256) void k_rows_by(int64_t n, const int32_t *perm, const int32_t *__restrict__ prow,
-                                                 int32_t *key, int32_t *idx) {
-    for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
-        const int p = perm[s];
-        key[s] = prow[p];
-        idx[s] = p;
-    }
-}
-
-// flag[s] = 1 where a new (row, column) starts in the sorted products
-__global__ __launch_bounds__(256) void k_runflag(int64_t n, const int32_t *__restrict__ rows,
-                                                 const int32_t *__restrict__ perm, const int32_t *__restrict__ pcol,
-                                                 int32_t *flag) {
-    for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256)
-        flag[s] = (s == 0 || rows[s] != rows[s - 1] || pcol[perm[s]] != pcol[perm[s - 1]]) ? 1 : 0;
-}
-
 // one output entry per run: its products summed in FP64 in sorted order
 __global__ __launch_bounds__(256) void k_runs(int64_t n, const int32_t *__restrict__ flag,
                                               const int32_t *__restrict__ pos, const int32_t *__restrict__ rows,
@@ -221,12 +207,6 @@ __global__ __launch_bounds__(256) void k_runs(int64_t n, const int32_t *__restri
         ccol[j] = pcol[perm[s]];
         cval[j] = float(acc);
     }
-}
-
-int bits_for(int64_t x) {  // smallest b with 2^b >= x
-    int b = 0;
-    while ((int64_t(1) << b) < x) ++b;
-    return b;
 }
 
 struct Dcsc {

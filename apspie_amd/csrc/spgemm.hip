@@ -28,6 +28,8 @@
 #include <cstring>
 #include <vector>
 
+#include "convert.hpp"
+#include "esc.hpp"
 #include "frontier.hpp"
 #include "internal.hpp"
 
@@ -461,6 +463,158 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
     }
 }
 
+// The rows with the most products, by expand-sort-compress instead of
+// global hash tables (whose inserts are device-scope atomics, a few G/s):
+// row r of the list writes its products at poff[r] in the order of A's
+// entries, then of B's row: (r, column, a * b in FP64); key = column, idx =
+// position.  A product whose column lies outside B gets row nrows, which
+// sorts after every real row and is dropped, as the hash tables skip it.
+// Stable sorts by column and then by row leave each output entry's products
+// in increasing k, so the FP64 run sums are the tables' and Gustavson's.
+template <int T>
+__global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
+                                                   const int32_t *__restrict__ poff, int32_t *prow, int32_t *pcol,
+                                                   double *pval, int32_t *key, int32_t *idx) {
+    __shared__ int32_t s_bs[T];
+    __shared__ int32_t s_pre[T];
+    __shared__ float s_a[T];
+    __shared__ int32_t s_tmp[T / WAVE + 1];
+    const int tid = threadIdx.x;
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int i = rows[r];
+        int base = poff[r];
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        for (int jb = ab; jb < ae; jb += T) {
+            const int j = jb + tid;
+            int L = 0, bs = 0;
+            float a = 0.f;
+            if (j < ae) {
+                const int k = A.col[j];
+                if ((unsigned)k < (unsigned)B.m) {
+                    a = A.val ? A.val[j] : 1.0f;
+                    bs = B.rp[k];
+                    L = B.rp[k + 1] - bs;
+                }
+            }
+            int tot;
+            const int pre = block_excl_scan<T>(L, s_tmp, tot);
+            s_bs[tid] = bs;
+            s_pre[tid] = pre;
+            s_a[tid] = a;
+            const int ne = min(T, ae - jb);
+            __syncthreads();
+            int lo = 0;
+            for (int t = tid; t < tot; t += T) {
+                int hi = ne - 1;  // last entry whose prefix <= t (t only grows)
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int q = s_bs[lo] + (t - s_pre[lo]);
+                const int c = B.col[q];
+                const bool ok = (unsigned)c < (unsigned)B.n;
+                const int o = base + t;
+                prow[o] = ok ? r : nrows;
+                pcol[o] = ok ? c : 0;
+                key[o] = ok ? c : 0;
+                idx[o] = o;
+                pval[o] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
+            }
+            base += tot;
+            __syncthreads();
+        }
+    }
+}
+
+// nnz(C(i,:)) of the long rows: the runs between their sorted row starts
+__global__ __launch_bounds__(256) void k_long_nnz(int nrows, const int32_t *__restrict__ rows,
+                                                  const int32_t *__restrict__ rstart,
+                                                  const int32_t *__restrict__ runpos, int32_t *nnzc) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrows; r += gridDim.x * 256)
+        nnzc[rows[r]] = runpos[rstart[r + 1]] - runpos[rstart[r]];
+}
+
+// one output entry per run of a long row: its products summed in FP64 in
+// sorted order, written at the row's offset in C
+__global__ __launch_bounds__(256) void k_long_runs(int64_t n, int nrows, const int32_t *__restrict__ rows,
+                                                   const int32_t *__restrict__ rstart,
+                                                   const int32_t *__restrict__ flag,
+                                                   const int32_t *__restrict__ runpos,
+                                                   const int32_t *__restrict__ srow, const int32_t *__restrict__ perm,
+                                                   const int32_t *__restrict__ pcol,
+                                                   const double *__restrict__ pval, const int32_t *__restrict__ crp,
+                                                   int32_t *ccol, float *cval) {
+    for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
+        if (!flag[s]) continue;
+        const int r = srow[s];
+        if (r >= nrows) continue;  // products outside B's columns
+        double acc = pval[perm[s]];
+        for (int64_t x = s + 1; x < n && !flag[x]; ++x) acc += pval[perm[x]];
+        const int j = crp[rows[r]] + runpos[s] - runpos[rstart[r]];
+        ccol[j] = pcol[perm[s]];
+        cval[j] = float(acc);
+    }
+}
+
+// Device arrays of the long rows' expand-sort-compress, kept from the
+// symbolic to the numeric phase.
+struct LongRows {
+    int nrows = 0;
+    int64_t n = 0;  // products
+    const int32_t *rows = nullptr;
+    int32_t *rstart = nullptr, *flag = nullptr, *runpos = nullptr, *srow = nullptr, *perm = nullptr, *pcol = nullptr;
+    double *pval = nullptr;
+};
+
+// Symbolic phase of the long rows: products, sorts, runs, nnz(C(i,:)).
+// gub[r] = the products of row r of the list (their sum < 2^31).
+LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const CsrD &A,
+                       const CsrD &B, int32_t *nnzc) {
+    LongRows L;
+    const int ng = int(gub.size());
+    std::vector<int32_t> po(size_t(ng) + 1, 0);
+    for (int r = 0; r < ng; ++r) po[r + 1] = po[r] + gub[r];
+    const int64_t n = po[ng];
+    L.nrows = ng;
+    L.n = n;
+    L.rows = rows;
+    auto S = [&](int k, int64_t bytes) { return ctx->slot(S_OUTER0 + k, size_t(std::max<int64_t>(bytes, 16))); };
+    int32_t *poff = (int32_t *)S(0, (int64_t(ng) + 1) * 4);
+    int32_t *prow = (int32_t *)S(1, n * 4);
+    L.pcol = (int32_t *)S(2, n * 4);
+    L.pval = (double *)S(3, n * 8);
+    int32_t *key[2] = {(int32_t *)S(4, (n + 1) * 4), (int32_t *)S(5, (n + 1) * 4)};
+    int32_t *idx[2] = {(int32_t *)S(6, (n + 1) * 4), (int32_t *)S(7, (n + 1) * 4)};
+    L.rstart = (int32_t *)S(8, (int64_t(ng) + 2) * 4);
+    APSPIE_HIP(hipMemcpyAsync(poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
+                          ctx->stream>>>(rows, ng, A, B, poff, prow, L.pcol, L.pval, key[0], idx[0]);
+    APSPIE_HIP(hipGetLastError());
+    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(B.n));
+    const unsigned g = cvt::grid_for(ctx, n, 256);
+    esc::k_rows_by<<<g, 256, 0, ctx->stream>>>(n, idx[c1], prow, key[0], idx[0]);
+    const int c2 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(ng) + 1));
+    L.srow = key[c2];
+    L.perm = idx[c2];
+    L.flag = key[c2 ^ 1];
+    L.runpos = idx[c2 ^ 1];
+    esc::k_runflag<<<g, 256, 0, ctx->stream>>>(n, L.srow, L.perm, L.pcol, L.flag);
+    cvt::scan32(ctx, L.flag, L.runpos, n);
+    cvt::compress(ctx, L.srow, n, ng + 1, L.rstart);  // sorted row starts, rows 0 .. ng (the dropped ones)
+    k_long_nnz<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(ng, rows, L.rstart,
+                                                                                         L.runpos, nnzc);
+    APSPIE_HIP(hipGetLastError());
+    return L;
+}
+
+void long_numeric(apspie_ctx *ctx, const LongRows &L, const int32_t *crp, int32_t *ccol, float *cval) {
+    k_long_runs<<<cvt::grid_for(ctx, L.n, 256), 256, 0, ctx->stream>>>(L.n, L.nrows, L.rows, L.rstart, L.flag,
+                                                                       L.runpos, L.srow, L.perm, L.pcol, L.pval,
+                                                                       crp, ccol, cval);
+    APSPIE_HIP(hipGetLastError());
+}
+
 struct Plan {
     int counts[NBINS] = {};
     int32_t *lists = nullptr;
@@ -545,6 +699,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         int64_t *goff = nullptr;
         int32_t *gslots = nullptr, *gkeys = nullptr;
         double *gvals = nullptr;
+        std::vector<int32_t> gub;
+        bool esc_rows = false;
         if (P.counts[4]) {
             const int ng = P.counts[4];
             // the product bounds of these rows only (gathered into the slot
@@ -552,9 +708,16 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
             k_gather<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(
                 lists + 4 * int64_t(m), ng, ub, gslots);
-            std::vector<int32_t> gub(ng);
+            gub.resize(ng);
             APSPIE_HIP(hipMemcpyAsync(gub.data(), gslots, size_t(ng) * 4, hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            int64_t gtot = 0;
+            for (int r = 0; r < ng; ++r) gtot += gub[r];
+            // expand-sort-compress while the products index with int32
+            esc_rows = ctx->tune.spgemm_long_rows == 1 && gtot < (int64_t(1) << 31) - 1;
+        }
+        if (P.counts[4] && !esc_rows) {
+            const int ng = P.counts[4];
             std::vector<int64_t> off(ng);
             std::vector<int32_t> sl(ng);
             int64_t tot = 0;
@@ -572,8 +735,15 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             APSPIE_HIP(hipMemcpyAsync(goff, off.data(), size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
             APSPIE_HIP(hipMemcpyAsync(gslots, sl.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
         }
-        // 3. symbolic
-        launch_bins<false>(ctx, P, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
+        // 3. symbolic (the longest rows by expand-sort-compress when their
+        // products fit int32 indices, else in global hash tables)
+        Plan Pb = P;
+        LongRows lr;
+        if (esc_rows) {
+            Pb.counts[4] = 0;
+            lr = long_symbolic(ctx, lists + 4 * int64_t(m), gub, A, B, nnzc);
+        }
+        launch_bins<false>(ctx, Pb, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
         // 4. row pointers
         APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
         c->m = m;
@@ -597,7 +767,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             throw Error(APSPIE_ENOMEM, "output allocation failed");
         }
         // 5. numeric
-        launch_bins<true>(ctx, P, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
+        launch_bins<true>(ctx, Pb, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
+        if (esc_rows) long_numeric(ctx, lr, c->row_ptr, c->col_ind, c->val);
         APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         if (stats) {

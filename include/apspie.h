@@ -148,6 +148,9 @@ typedef struct {
     int32_t sssp_delta;       /* SSSP near/far step: a changed vertex waits while its distance is at or
                                  past a threshold that rises by this much when the frontier below it
                                  runs dry; 0 = off (plain frontier Bellman-Ford), 0..2^24 */
+    int32_t spgemm_long_rows; /* SpGEMM rows beyond the LDS tables (over 2048 products): 1 = expand-
+                                 sort-compress (default; global hash tables when their products reach
+                                 2^31), 0 = global hash tables */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

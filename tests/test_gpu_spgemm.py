@@ -87,3 +87,49 @@ def test_spgemm_full_config(ctx):
     """1M x 1M, power-law rows (mean 8, max 4096), uniform columns, C = A * A."""
     h = A.powerlaw(1 << 20, 1 << 20, 8.0, 4096, 3003)
     run(ctx, h, h)
+
+
+def long_row_products(ha, hb):
+    """Products of each row of A * B (the bound the row bins use)."""
+    deg = np.diff(hb.row_ptr).astype(np.int64)
+    per = deg[ha.col_ind]
+    return np.add.reduceat(per, ha.row_ptr[:-1]) * (np.diff(ha.row_ptr) > 0)
+
+
+@pytest.mark.parametrize("m,maxlen,seed", [(30000, 4096, 2), (200000, 4096, 7)])
+def test_spgemm_long_rows_both_ways(ctx, m, maxlen, seed):
+    """Rows beyond the LDS tables (over 2048 products) by expand-sort-compress
+    (default) and by global hash tables (spgemm_long_rows=0): both match the
+    oracle, with the same structure."""
+    h = A.powerlaw(m, m, 8.0, maxlen, seed)
+    assert (long_row_products(h, h) > 2048).sum() > 10  # the long-row bin is exercised
+    saved = ctx.tuning()
+    try:
+        got = {}
+        for mode in (0, 1):
+            ctx.set_tuning(spgemm_long_rows=mode)
+            got[mode], _ = run(ctx, h, h)
+    finally:
+        ctx.set_tuning(**saved)
+    assert np.array_equal(got[0].row_ptr, got[1].row_ptr) and np.array_equal(got[0].col_ind, got[1].col_ind)
+
+
+def test_spgemm_long_rows_with_columns_outside_b(ctx):
+    """A long row whose B rows hold columns outside B's range: those products
+    are dropped by both paths (validate_csr does not check column ranges)."""
+    h = A.powerlaw(3000, 3000, 40.0, 2000, 9)
+    assert (long_row_products(h, h) > 2048).any()
+    bad = h.col_ind.copy()
+    bad[::97] = 3000 + (np.arange(bad[::97].size) % 5)  # columns n .. n+4
+    hb = A.HostCsr(h.m, h.n, h.row_ptr, bad, h.val)
+    saved = ctx.tuning()
+    try:
+        out = {}
+        for mode in (0, 1):
+            ctx.set_tuning(spgemm_long_rows=mode)
+            out[mode], _ = ctx.spgemm(ctx.upload(h), ctx.upload(hb))
+    finally:
+        ctx.set_tuning(**saved)
+    assert np.array_equal(out[0].row_ptr, out[1].row_ptr) and np.array_equal(out[0].col_ind, out[1].col_ind)
+    assert np.allclose(out[0].val, out[1].val, rtol=1e-5, atol=0)
+    assert out[1].col_ind.size == 0 or out[1].col_ind.max() < 3000
