@@ -125,31 +125,51 @@ __global__ __launch_bounds__(T) void k_hist(const int32_t *__restrict__ key, int
 }
 
 // stable scatter of one pass: off[d * ntiles + tile] = where the tile's
-// first item of digit d goes
+// first item of digit d goes.  Wave w of a workgroup owns the tile's items
+// [w * 1024, (w + 1) * 1024) and keeps them in registers: it counts their
+// digits into its own LDS row, one thread per digit turns the rows into each
+// wave's starting position per digit, and the wave walks its items in order,
+// ranking lanes of equal digit with eight ballots and advancing its row (the
+// LDS operations of one wave execute in order, so no barrier is needed
+// between rounds).  Equal digits keep their input order.
 __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, const int32_t *__restrict__ idx,
                                                int64_t n, int shift, int ntiles, const int32_t *__restrict__ off,
                                                int32_t *key_out, int32_t *idx_out) {
-    constexpr int NW = T / WAVE;
-    __shared__ int32_t s_base[RADIX];
-    __shared__ int32_t s_wcnt[NW][RADIX];
+    constexpr int NW = T / WAVE, PER = TILE_N / NW, R = PER / WAVE;
+    static_assert(RADIX == T, "one thread per digit");
+    __shared__ int32_t s_cnt[NW][RADIX];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const uint64_t lt = (1ull << lane) - 1;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        s_base[tid] = off[int64_t(tid) * ntiles + tile];
-        const int64_t t0 = int64_t(tile) * TILE_N;
-        for (int j = 0; j < ITEMS; ++j) {
 #pragma unroll
-            for (int w = 0; w < NW; ++w) s_wcnt[w][tid] = 0;
-            __syncthreads();
-            const int64_t t = t0 + j * T + tid;
-            const bool valid = t < n;
-            int k = 0, x = 0, d = 0;
-            if (valid) {
-                k = key[t];
-                x = idx[t];
-                d = (uint32_t(k) >> shift) & (RADIX - 1);
+        for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
+        __syncthreads();
+        const int64_t wb = int64_t(tile) * TILE_N + int64_t(wid) * PER;
+        int k[R], x[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int64_t t = wb + r * WAVE + lane;
+            k[r] = t < n ? key[t] : -1;
+            x[r] = t < n ? idx[t] : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (k[r] >= 0) atomicAdd(&s_cnt[wid][(uint32_t(k[r]) >> shift) & (RADIX - 1)], 1);
+        __syncthreads();
+        {  // thread d: the waves' starting positions for digit d
+            int run = off[int64_t(tid) * ntiles + tile];
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const int c = s_cnt[w][tid];
+                s_cnt[w][tid] = run;
+                run += c;
             }
-            // lanes holding the same digit
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool valid = k[r] >= 0;
+            const int d = valid ? (uint32_t(k[r]) >> shift) & (RADIX - 1) : 0;
             uint64_t peers = __ballot(valid);
 #pragma unroll
             for (int b = 0; b < 8; ++b) {
@@ -157,21 +177,14 @@ __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, 
                 peers &= ((d >> b) & 1) ? ones : ~ones;
             }
             const int rank = __popcll(peers & lt);
-            if (valid && rank == 0) s_wcnt[wid][d] = __popcll(peers);
-            __syncthreads();
+            const int pos = valid ? s_cnt[wid][d] + rank : 0;
+            if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
             if (valid) {
-                int pos = s_base[d] + rank;
-                for (int w = 0; w < wid; ++w) pos += s_wcnt[w][d];
-                key_out[pos] = k;
-                idx_out[pos] = x;
+                key_out[pos] = k[r];
+                idx_out[pos] = x[r];
             }
-            __syncthreads();
-            int add = 0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) add += s_wcnt[w][tid];
-            s_base[tid] += add;
         }
-        __syncthreads();
+        __syncthreads();  // the next tile clears the rows
     }
 }
 
@@ -217,14 +230,14 @@ void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) {
     APSPIE_HIP(hipGetLastError());
 }
 
-int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits) {
-    if (n == 0) return 0;
+int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start) {
+    if (n == 0) return start;
     const int ntiles = (int)ceil_div(n, TILE_N);
     const int64_t nh = int64_t(RADIX) * ntiles;
     int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
     int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
     const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
-    int cur = 0;
+    int cur = start;
     for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
         k_hist<<<gt, T, 0, ctx->stream>>>(key[cur], n, shift, ntiles, hist);
         scan32(ctx, hist, offs, nh);

@@ -466,15 +466,15 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
 // The rows with the most products, by expand-sort-compress instead of
 // global hash tables (whose inserts are device-scope atomics, a few G/s):
 // row r of the list writes its products at poff[r] in the order of A's
-// entries, then of B's row: (r, column, a * b in FP64); key = column, idx =
-// position.  A product whose column lies outside B gets row nrows, which
-// sorts after every real row and is dropped, as the hash tables skip it.
-// Stable sorts by column and then by row leave each output entry's products
-// in increasing k, so the FP64 run sums are the tables' and Gustavson's.
+// entries, then of B's row: (column, a * b in FP64); key = column (B.n for a
+// column outside B, dropped later as the hash tables skip it), idx =
+// position.  Stable sorts by column and then by row leave each output
+// entry's products in increasing k, so the FP64 run sums are the tables' and
+// Gustavson's.
 template <int T>
 __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
-                                                   const int32_t *__restrict__ poff, int32_t *prow, int32_t *pcol,
-                                                   double *pval, int32_t *key, int32_t *idx) {
+                                                   const int32_t *__restrict__ poff, int32_t *pcol, double *pval,
+                                                   int32_t *key, int32_t *idx) {
     __shared__ int32_t s_bs[T];
     __shared__ int32_t s_pre[T];
     __shared__ float s_a[T];
@@ -515,15 +515,45 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
                 const int c = B.col[q];
                 const bool ok = (unsigned)c < (unsigned)B.n;
                 const int o = base + t;
-                prow[o] = ok ? r : nrows;
                 pcol[o] = ok ? c : 0;
-                key[o] = ok ? c : 0;
+                key[o] = ok ? c : B.n;
                 idx[o] = o;
                 pval[o] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
             }
             base += tot;
             __syncthreads();
         }
+    }
+}
+
+// Sort keys of the row pass: the list row of product perm[s] (the last r
+// with poff[r] <= perm[s], searched in LDS when the offsets fit), or nrows
+// for a product outside B's columns (column key B.n), which sorts after
+// every row and is dropped.
+__global__ __launch_bounds__(256) void k_long_rowkey(int64_t n, int nrows, int ncol, const int32_t *__restrict__ poff,
+                                                     const int32_t *__restrict__ ckey,
+                                                     const int32_t *__restrict__ perm, int32_t *key, int32_t *idx) {
+    constexpr int MAXL = 16384;
+    __shared__ int32_t s_off[MAXL];
+    const bool lds = nrows + 1 <= MAXL;
+    if (lds)
+        for (int r = threadIdx.x; r <= nrows; r += 256) s_off[r] = poff[r];
+    __syncthreads();
+    const int32_t *off = lds ? s_off : poff;
+    for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
+        const int p = perm[s];
+        int r = nrows;
+        if (ckey[s] != ncol) {
+            int lo = 0, hi = nrows - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            r = lo;
+        }
+        key[s] = r;
+        idx[s] = p;
     }
 }
 
@@ -581,7 +611,6 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     L.rows = rows;
     auto S = [&](int k, int64_t bytes) { return ctx->slot(S_OUTER0 + k, size_t(std::max<int64_t>(bytes, 16))); };
     int32_t *poff = (int32_t *)S(0, (int64_t(ng) + 1) * 4);
-    int32_t *prow = (int32_t *)S(1, n * 4);
     L.pcol = (int32_t *)S(2, n * 4);
     L.pval = (double *)S(3, n * 8);
     int32_t *key[2] = {(int32_t *)S(4, (n + 1) * 4), (int32_t *)S(5, (n + 1) * 4)};
@@ -589,12 +618,17 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     L.rstart = (int32_t *)S(8, (int64_t(ng) + 2) * 4);
     APSPIE_HIP(hipMemcpyAsync(poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
-                          ctx->stream>>>(rows, ng, A, B, poff, prow, L.pcol, L.pval, key[0], idx[0]);
+                          ctx->stream>>>(rows, ng, A, B, poff, L.pcol, L.pval, key[0], idx[0]);
     APSPIE_HIP(hipGetLastError());
-    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(B.n));
+    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(B.n) + 1));
     const unsigned g = cvt::grid_for(ctx, n, 256);
-    esc::k_rows_by<<<g, 256, 0, ctx->stream>>>(n, idx[c1], prow, key[0], idx[0]);
-    const int c2 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(ng) + 1));
+    // the row keys and the permutation go to the other buffer pair, where
+    // the row pass starts
+    // (each workgroup stages the row offsets in LDS once: a few per CU)
+    k_long_rowkey<<<std::min(g, unsigned(ctx->cu_count) * 4), 256, 0, ctx->stream>>>(n, ng, B.n, poff, key[c1],
+                                                                                     idx[c1], key[c1 ^ 1],
+                                                                                     idx[c1 ^ 1]);
+    const int c2 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(ng) + 1), c1 ^ 1);
     L.srow = key[c2];
     L.perm = idx[c2];
     L.flag = key[c2 ^ 1];
