@@ -83,6 +83,7 @@ class Tuning(C.Structure):
         ("sssp_batch_lanes", C.c_int32),
         ("sssp_delta", C.c_int32),
         ("spgemm_long_rows", C.c_int32),
+        ("spgemm_passes", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -97,6 +97,7 @@ struct apspie_ctx {
         int expand_wg_per_cu = 8, round_chunk = 4, batch_chunk = 4, sssp_batch_lanes = 0;
         int sssp_delta = 0;
         int spgemm_long_rows = 1;
+        int spgemm_passes = 1;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

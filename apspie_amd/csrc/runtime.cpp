@@ -147,7 +147,8 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     if (t->expand_wg_per_cu < 1 || t->expand_wg_per_cu > 32 || t->round_chunk < 1 || t->round_chunk > 32 ||
         t->batch_chunk < 1 || t->batch_chunk > 16 ||
         (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64) ||
-        t->sssp_delta < 0 || t->sssp_delta > (1 << 24) || (t->spgemm_long_rows != 0 && t->spgemm_long_rows != 1)) {
+        t->sssp_delta < 0 || t->sssp_delta > (1 << 24) || (t->spgemm_long_rows != 0 && t->spgemm_long_rows != 1) ||
+        (t->spgemm_passes != 1 && t->spgemm_passes != 2)) {
         ctx->err = "tuning knob out of range";
         return APSPIE_EINVAL;
     }
@@ -162,6 +163,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.sssp_batch_lanes = t->sssp_batch_lanes;
     ctx->tune.sssp_delta = t->sssp_delta;
     ctx->tune.spgemm_long_rows = t->spgemm_long_rows;
+    ctx->tune.spgemm_passes = t->spgemm_passes;
     return APSPIE_OK;
 }
 
@@ -178,6 +180,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->sssp_batch_lanes = ctx->tune.sssp_batch_lanes;
     t->sssp_delta = ctx->tune.sssp_delta;
     t->spgemm_long_rows = ctx->tune.spgemm_long_rows;
+    t->spgemm_passes = ctx->tune.spgemm_passes;
     return APSPIE_OK;
 }
 

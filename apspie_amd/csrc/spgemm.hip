@@ -444,6 +444,7 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
         }
         int tot;
         int out = crp[i] + block_excl_scan<T>(cnt, s_tmp, tot);
+        if (tid == 0) nnzc[i] = tot;  // (a one-pass product learns nnz here)
         for (int q = p; q < se;) {
             if (key[q] == -1) {
                 ++q;
@@ -584,6 +585,30 @@ __global__ __launch_bounds__(256) void k_long_runs(int64_t n, int nrows, const i
         const int j = crp[rows[r]] + runpos[s] - runpos[rstart[r]];
         ccol[j] = pcol[perm[s]];
         cval[j] = float(acc);
+    }
+}
+
+// ubs[i] = the products of row i when the LDS tables take it, else 0: the
+// rows' room in the staging area of the one-pass product
+__global__ __launch_bounds__(256) void k_stage_ub(int m, const int32_t *__restrict__ ub, int32_t *ubs) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
+        ubs[i] = ub[i] <= BIN_UB[NBINS - 2] ? ub[i] : 0;
+}
+
+// C(i,:) of the LDS-table rows from the staging area (one wave per row)
+__global__ __launch_bounds__(256) void k_unstage(int m, const int32_t *__restrict__ ub,
+                                                 const int32_t *__restrict__ ubpre, const int32_t *__restrict__ nnzc,
+                                                 const int32_t *__restrict__ crp, const int32_t *__restrict__ scol,
+                                                 const float *__restrict__ sval, int32_t *ccol, float *cval) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nw = gridDim.x * (256 / WAVE);
+    for (int i = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; i < m; i += nw) {
+        if (ub[i] > BIN_UB[NBINS - 2]) continue;  // a long row, written by its own path
+        const int src = ubpre[i], dst = crp[i], n = nnzc[i];
+        for (int e = lane; e < n; e += WAVE) {
+            ccol[dst + e] = scol[src + e];
+            cval[dst + e] = sval[src + e];
+        }
     }
 }
 
@@ -735,6 +760,7 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         double *gvals = nullptr;
         std::vector<int32_t> gub;
         bool esc_rows = false;
+        int64_t gtot = 0;  // products of the long rows
         if (P.counts[4]) {
             const int ng = P.counts[4];
             // the product bounds of these rows only (gathered into the slot
@@ -745,7 +771,6 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             gub.resize(ng);
             APSPIE_HIP(hipMemcpyAsync(gub.data(), gslots, size_t(ng) * 4, hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-            int64_t gtot = 0;
             for (int r = 0; r < ng; ++r) gtot += gub[r];
             // expand-sort-compress while the products index with int32
             esc_rows = ctx->tune.spgemm_long_rows == 1 && gtot < (int64_t(1) << 31) - 1;
@@ -769,15 +794,35 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             APSPIE_HIP(hipMemcpyAsync(goff, off.data(), size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
             APSPIE_HIP(hipMemcpyAsync(gslots, sl.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
         }
-        // 3. symbolic (the longest rows by expand-sort-compress when their
-        // products fit int32 indices, else in global hash tables)
-        Plan Pb = P;
-        LongRows lr;
-        if (esc_rows) {
-            Pb.counts[4] = 0;
-            lr = long_symbolic(ctx, lists + 4 * int64_t(m), gub, A, B, nnzc);
+        // 3. the rows of the LDS tables in one pass while their products fit
+        // int32 offsets: each sorted row goes to a staging area at the prefix
+        // of the rows' products, with its nnz (else a counting pass now and
+        // the product pass after the row pointers); the long rows by
+        // expand-sort-compress when their products fit int32 indices, else in
+        // global hash tables
+        Plan P03 = P, P4 = P;
+        P03.counts[NBINS - 1] = 0;
+        for (int k = 0; k < NBINS - 1; ++k) P4.counts[k] = 0;
+        const int64_t staged = int64_t(prod_total) - gtot;
+        const bool one_pass = ctx->tune.spgemm_passes == 1 && staged < (int64_t(1) << 31) - 1;
+        int32_t *ubpre = nullptr, *scol = nullptr;
+        float *sval = nullptr;
+        if (one_pass) {
+            ubpre = (int32_t *)ctx->slot(S_OUTER0 + 12, (size_t(m) + 1) * 4);
+            int32_t *ubs = (int32_t *)ctx->slot(S_OUTER0 + 13, size_t(std::max(m, 1)) * 4);
+            scol = (int32_t *)ctx->slot(S_OUTER0 + 10, size_t(std::max<int64_t>(staged, 1)) * 4);
+            sval = (float *)ctx->slot(S_OUTER0 + 11, size_t(std::max<int64_t>(staged, 1)) * 4);
+            k_stage_ub<<<g256, 256, 0, ctx->stream>>>(m, ub, ubs);
+            cvt::scan32(ctx, ubs, ubpre, m);
+            launch_bins<true>(ctx, P03, A, B, nnzc, ubpre, scol, sval, nullptr, nullptr, nullptr, nullptr);
+        } else {
+            launch_bins<false>(ctx, P03, A, B, nnzc, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         }
-        launch_bins<false>(ctx, Pb, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
+        LongRows lr;
+        if (esc_rows)
+            lr = long_symbolic(ctx, lists + 4 * int64_t(m), gub, A, B, nnzc);
+        else if (P.counts[NBINS - 1])
+            launch_bins<false>(ctx, P4, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
         // 4. row pointers
         APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
         c->m = m;
@@ -801,8 +846,17 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             throw Error(APSPIE_ENOMEM, "output allocation failed");
         }
         // 5. numeric
-        launch_bins<true>(ctx, Pb, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
-        if (esc_rows) long_numeric(ctx, lr, c->row_ptr, c->col_ind, c->val);
+        if (one_pass)
+            k_unstage<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 256 / WAVE), (unsigned)ctx->cu_count * 32)),
+                        256, 0, ctx->stream>>>(m, ub, ubpre, nnzc, c->row_ptr, scol, sval, c->col_ind, c->val);
+        else
+            launch_bins<true>(ctx, P03, A, B, nnzc, c->row_ptr, c->col_ind, c->val, nullptr, nullptr, nullptr,
+                              nullptr);
+        if (esc_rows)
+            long_numeric(ctx, lr, c->row_ptr, c->col_ind, c->val);
+        else if (P.counts[NBINS - 1])
+            launch_bins<true>(ctx, P4, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
+        APSPIE_HIP(hipGetLastError());
         APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         if (stats) {

@@ -151,6 +151,9 @@ typedef struct {
     int32_t spgemm_long_rows; /* SpGEMM rows beyond the LDS tables (over 2048 products): 1 = expand-
                                  sort-compress (default; global hash tables when their products reach
                                  2^31), 0 = global hash tables */
+    int32_t spgemm_passes;    /* SpGEMM rows of the LDS tables: 1 = one pass into a staging area
+                                 (default; two when their products reach 2^31), 2 = a counting pass,
+                                 then the product pass */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

@@ -99,19 +99,23 @@ def long_row_products(ha, hb):
 @pytest.mark.parametrize("m,maxlen,seed", [(30000, 4096, 2), (200000, 4096, 7)])
 def test_spgemm_long_rows_both_ways(ctx, m, maxlen, seed):
     """Rows beyond the LDS tables (over 2048 products) by expand-sort-compress
-    (default) and by global hash tables (spgemm_long_rows=0): both match the
-    oracle, with the same structure."""
+    (default) and by global hash tables (spgemm_long_rows=0), the other rows in
+    one pass through a staging area (default) or in a counting and a product
+    pass (spgemm_passes=2): every combination matches the oracle."""
     h = A.powerlaw(m, m, 8.0, maxlen, seed)
     assert (long_row_products(h, h) > 2048).sum() > 10  # the long-row bin is exercised
     saved = ctx.tuning()
     try:
         got = {}
-        for mode in (0, 1):
-            ctx.set_tuning(spgemm_long_rows=mode)
-            got[mode], _ = run(ctx, h, h)
+        for long_rows in (0, 1):
+            for passes in (1, 2):  # the LDS-table rows in one pass (staging) or two
+                ctx.set_tuning(spgemm_long_rows=long_rows, spgemm_passes=passes)
+                got[long_rows, passes], _ = run(ctx, h, h)
     finally:
         ctx.set_tuning(**saved)
-    assert np.array_equal(got[0].row_ptr, got[1].row_ptr) and np.array_equal(got[0].col_ind, got[1].col_ind)
+    ref = got[1, 1]
+    for c in got.values():
+        assert np.array_equal(c.row_ptr, ref.row_ptr) and np.array_equal(c.col_ind, ref.col_ind)
 
 
 def test_spgemm_long_rows_with_columns_outside_b(ctx):
