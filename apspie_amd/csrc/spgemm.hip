@@ -1,17 +1,25 @@
-// spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned hash
-// tables (LDS for rows up to 2048 products, global memory beyond).
+// spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned LDS
+// hash tables for rows up to 2048 products, expand-sort-compress beyond.
 //
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
 //                edge-parallel over A with the frontier kernels.
 //   2. bins    rows by ub: <= 16, <= 128, <= 1024, <= 2048 (LDS tables of
 //              64 / 512 / 4096 / 8192 slots, one wave .. 512 threads per row)
-//              and > 2048 (global table of 4 * pow2ceil(ub) slots per row).
-//   3. symbolic: insert every product's column; nnz(C(i,:)) = distinct count.
+//              and > 2048 (the long rows).
+//   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
+//      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
+//      once) and write the sorted row to a staging area at the prefix of the
+//      rows' ub, with its nnz.  Long rows: their products are written once,
+//      sorted stably by column and then by row (radix), and each run of equal
+//      columns is one entry (FP64 sum in k order); nnz = the runs.
 //   4. row_ptr(C) = exclusive scan (decoupled look-back).
-//   5. numeric: insert (column, a * b) with FP64 accumulation (products of
-//      FP32 inputs are exact in FP64; the sum is rounded to FP32 once), then
-//      write the row out with sorted columns.
+//   5. the staged rows are copied into C; the long rows' runs are summed
+//      into C.
+// Tuning keeps the older forms selectable (and they take over when int32
+// offsets would overflow): a counting pass before the product pass for the
+// LDS rows (spgemm_passes = 2), global hash tables of 4 * pow2ceil(ub) slots
+// for the long rows (spgemm_long_rows = 0).
 // The table uses an order-preserving home slot, home(c) = c * H / n, with
 // linear probing and no wrap-around (2H slots for at most H / 2 keys): a key's
 // slot lies in the cluster (run of occupied slots) holding its home, so all
