@@ -59,8 +59,9 @@ struct CsrD {
 // A table of S slots has H = S / 2 home slots and takes rows of at most S / 4
 // products, so keys fill at most half of the home range (linear probing with
 // order-preserving homes clusters badly when that range is nearly full).
-constexpr int NBINS = 5;
-constexpr int BIN_UB[NBINS - 1] = {16, 128, 1024, 2048};
+constexpr int NBINS = 7;
+constexpr int BIN_UB[NBINS - 1] = {16, 128, 256, 512, 1024, 2048};
+constexpr int LONG = NBINS - 1;  // the bin of rows beyond the LDS tables
 
 // ub[i] += nnz(B(c,:)) for every edge (i -> c) of A
 struct OpUpper {
@@ -704,14 +705,20 @@ void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, i
         k_rows<128, 512, NUMERIC, false><<<grid(P.counts[1], 8), 128, 0, ctx->stream>>>(
             P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[2])
-        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[2], 4), 256, 0, ctx->stream>>>(
+        k_rows<128, 1024, NUMERIC, false><<<grid(P.counts[2], 8), 128, 0, ctx->stream>>>(
             P.lists + 2 * int64_t(m), P.counts[2], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[3])
-        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[3], 1), 512, 0, ctx->stream>>>(
+        k_rows<256, 2048, NUMERIC, false><<<grid(P.counts[3], 4), 256, 0, ctx->stream>>>(
             P.lists + 3 * int64_t(m), P.counts[3], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[4])
-        k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[4], 2), 1024, 0, ctx->stream>>>(
-            P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, gkeys, gvals, goff, gslots);
+        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[4], 4), 256, 0, ctx->stream>>>(
+            P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[5])
+        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[5], 1), 512, 0, ctx->stream>>>(
+            P.lists + 5 * int64_t(m), P.counts[5], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[LONG])
+        k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[LONG], 2), 1024, 0, ctx->stream>>>(
+            P.lists + LONG * int64_t(m), P.counts[LONG], A, B, nnzc, crp, ccol, cval, gkeys, gvals, goff, gslots);
     APSPIE_HIP(hipGetLastError());
 }
 
@@ -769,13 +776,13 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         std::vector<int32_t> gub;
         bool esc_rows = false;
         int64_t gtot = 0;  // products of the long rows
-        if (P.counts[4]) {
-            const int ng = P.counts[4];
+        if (P.counts[LONG]) {
+            const int ng = P.counts[LONG];
             // the product bounds of these rows only (gathered into the slot
             // that later holds their table sizes)
             gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
             k_gather<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(
-                lists + 4 * int64_t(m), ng, ub, gslots);
+                lists + LONG * int64_t(m), ng, ub, gslots);
             gub.resize(ng);
             APSPIE_HIP(hipMemcpyAsync(gub.data(), gslots, size_t(ng) * 4, hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
@@ -783,8 +790,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             // expand-sort-compress while the products index with int32
             esc_rows = ctx->tune.spgemm_long_rows == 1 && gtot < (int64_t(1) << 31) - 1;
         }
-        if (P.counts[4] && !esc_rows) {
-            const int ng = P.counts[4];
+        if (P.counts[LONG] && !esc_rows) {
+            const int ng = P.counts[LONG];
             std::vector<int64_t> off(ng);
             std::vector<int32_t> sl(ng);
             int64_t tot = 0;
@@ -809,8 +816,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         // expand-sort-compress when their products fit int32 indices, else in
         // global hash tables
         Plan P03 = P, P4 = P;
-        P03.counts[NBINS - 1] = 0;
-        for (int k = 0; k < NBINS - 1; ++k) P4.counts[k] = 0;
+        P03.counts[LONG] = 0;
+        for (int k = 0; k < LONG; ++k) P4.counts[k] = 0;
         const int64_t staged = int64_t(prod_total) - gtot;
         const bool one_pass = ctx->tune.spgemm_passes == 1 && staged < (int64_t(1) << 31) - 1;
         int32_t *ubpre = nullptr, *scol = nullptr;
@@ -828,8 +835,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         }
         LongRows lr;
         if (esc_rows)
-            lr = long_symbolic(ctx, lists + 4 * int64_t(m), gub, A, B, nnzc);
-        else if (P.counts[NBINS - 1])
+            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc);
+        else if (P.counts[LONG])
             launch_bins<false>(ctx, P4, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
         // 4. row pointers
         APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
@@ -862,7 +869,7 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
                               nullptr);
         if (esc_rows)
             long_numeric(ctx, lr, c->row_ptr, c->col_ind, c->val);
-        else if (P.counts[NBINS - 1])
+        else if (P.counts[LONG])
             launch_bins<true>(ctx, P4, A, B, nnzc, c->row_ptr, c->col_ind, c->val, gkeys, gvals, goff, gslots);
         APSPIE_HIP(hipGetLastError());
         APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
