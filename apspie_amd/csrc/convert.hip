@@ -50,8 +50,10 @@ __global__ __launch_bounds__(256) void k_compress(const int32_t *__restrict__ ke
     }
 }
 
-// exclusive scan of n int32 into out[0..n] (out[n] = total), one pass
-__global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, int64_t n, uint64_t *status,
+// exclusive scan of n int32 into out[0..n] (out[n] = total), one pass; O is
+// int32 (totals below 2^31) or int64
+template <class O>
+__global__ __launch_bounds__(256) void k_scan(const int32_t *in, O *out, int64_t n, uint64_t *status,
                                               unsigned int *counter) {
     __shared__ int s_bid;
     __shared__ uint64_t s_w[4];
@@ -85,9 +87,9 @@ __global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, i
     uint64_t run = s_pre + wpre + incl - sum;
 #pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        if (b0 + k < n) out[b0 + k] = int32_t(run);
+        if (b0 + k < n) out[b0 + k] = O(run);
         run += v[k];
-        if (b0 + k == n - 1) out[n] = int32_t(run);
+        if (b0 + k == n - 1) out[n] = O(run);
     }
     if (n == 0 && bid == 0 && tid == 0) out[0] = 0;
 }
@@ -222,13 +224,17 @@ void compress(apspie_ctx *ctx, const int32_t *key, int64_t n, int m, int32_t *of
     APSPIE_CHECK(h != 2, APSPIE_EINVAL, "row index out of range");
 }
 
-void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) {
+template <class O>
+void scan_to(apspie_ctx *ctx, const int32_t *in, O *out, int64_t n) {
     const int nscan = std::max(1, (int)ceil_div(n, TILE_N));
     uint64_t *status = (uint64_t *)ctx->slot(S_STATUS, size_t(nscan + 2) * 8);
     k_zero64<<<grid_for(ctx, nscan + 2, 256), 256, 0, ctx->stream>>>(status, nscan + 2);
-    k_scan<<<nscan, 256, 0, ctx->stream>>>(in, out, n, status, (unsigned *)(status + nscan));
+    k_scan<O><<<nscan, 256, 0, ctx->stream>>>(in, out, n, status, (unsigned *)(status + nscan));
     APSPIE_HIP(hipGetLastError());
 }
+
+void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) { scan_to(ctx, in, out, n); }
+void scan_wide(apspie_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) { scan_to(ctx, in, out, n); }
 
 int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start) {
     if (n == 0) return start;

@@ -21,6 +21,9 @@ void compress(apspie_ctx *ctx, const int32_t *key, int64_t n, int m, int32_t *of
 // caller guarantees the total stays below 2^31.  Uses the S_STATUS slot.
 void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n);
 
+// The same scan into int64 (any total).
+void scan_wide(apspie_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
+
 // Stable least-significant-digit radix sort of the pairs (key[start][e],
 // idx[start][e]) by the low `bits` bits of the key (non-negative keys), 8
 // bits a pass, ping-ponging between the two buffers of each array.  Returns

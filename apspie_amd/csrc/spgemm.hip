@@ -38,17 +38,13 @@
 
 #include "convert.hpp"
 #include "esc.hpp"
-#include "frontier.hpp"
+#include "device.hpp"
 #include "internal.hpp"
 
 namespace apspie {
 namespace sg {
 
 using namespace dev;
-using trav::Bufs;
-using trav::Control;
-using trav::WG;
-using trav::WGC;
 
 struct CsrD {
     const int32_t *rp, *col;
@@ -63,34 +59,29 @@ constexpr int NBINS = 7;
 constexpr int BIN_UB[NBINS - 1] = {16, 128, 256, 512, 1024, 2048};
 constexpr int LONG = NBINS - 1;  // the bin of rows beyond the LDS tables
 
-// ub[i] += nnz(B(c,:)) for every edge (i -> c) of A
-struct OpUpper {
-    static constexpr bool kVert = true, kVal = false;
-    using Pre = int32_t;
-    const int32_t *brp;
-    int32_t *ub;
-    __device__ Pre load(int c, int, uint32_t, int) const { return brp[c + 1] - brp[c]; }
-    __device__ void apply(int, int, uint32_t, int v, Pre len) const {
-        if (len) atomicAdd(&ub[v], len);
+// Product bounds ub[i] = sum over A(i,:) of nnz(B(k,:)): every entry's B
+// row length (0 for a column outside B), an exclusive scan in int64, and one
+// difference per row, so no atomics.
+__global__ __launch_bounds__(256) void k_entry_len(int64_t nnz, const int32_t *__restrict__ col,
+                                                   const int32_t *__restrict__ brp, int bm, int32_t *len) {
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nnz; e += int64_t(gridDim.x) * 256) {
+        const int k = col[e];
+        len[e] = (unsigned)k < (unsigned)bm ? brp[k + 1] - brp[k] : 0;
     }
-};
+}
 
-struct PickRows {
-    int m;
-    __device__ bool pick(int v, uint32_t &) const { return v < m; }
-    __device__ void on_new(int) const {}
-};
+// ub[i] from the scanned lengths; *bad = 1 for a row of 2^31 or more products
+__global__ __launch_bounds__(256) void k_row_ub(int m, const int32_t *__restrict__ rp,
+                                                const int64_t *__restrict__ psum, int32_t *ub, int32_t *bad) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
+        const int64_t u = psum[rp[i + 1]] - psum[rp[i]];
+        if (u >= (int64_t(1) << 31) - 1) *bad = 1;
+        ub[i] = int32_t(std::min<int64_t>(u, (int64_t(1) << 31) - 2));
+    }
+}
 
 __global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = 0;
-}
-
-__global__ __launch_bounds__(256) void k_init_ctl(int nblk, uint64_t *status, Control *C) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < 2 * nblk; i += gridDim.x * 256) status[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        Control z{};
-        *C = z;
-    }
 }
 
 // rows -> bin lists; rows without products get nnz 0 directly.  Each
@@ -741,22 +732,23 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         const CsrD A{a->row_ptr, a->col_ind, a->val, a->m, a->n};
         const CsrD B{b->row_ptr, b->col_ind, b->val, b->m, b->n};
         APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        // 1. product upper bounds, edge-parallel over A
-        const Bufs fb = trav::get_bufs(ctx, m, a->nnz);
+        // 1. product bounds (the products of each row of C)
         int32_t *ub = (int32_t *)ctx->slot(S_AUX0, (size_t(m) + 1) * 4);
         int32_t *nnzc = (int32_t *)ctx->slot(S_AUX1, (size_t(m) + 1) * 4);
         int32_t *lists = (int32_t *)ctx->slot(S_AUX2, size_t(NBINS) * (size_t(m) + 1) * 4);
         int32_t *counts = (int32_t *)ctx->slot(S_AUX3, 256);
         unsigned long long *products = (unsigned long long *)(counts + 16);
         const unsigned g256 = std::max(1u, std::min(ceil_div(std::max(m, 1), 256), 4096u));
-        k_zero_i32<<<g256, 256, 0, ctx->stream>>>(ub, m + 1);
+        int32_t *bad = counts + 10;
         k_zero_i32<<<1, 256, 0, ctx->stream>>>(counts, 64);
-        k_init_ctl<<<4, 256, 0, ctx->stream>>>(fb.nblk, fb.status, fb.C);
-        const apspie_csr ga{a->m, a->n, a->nnz, a->row_ptr, a->col_ind, a->val};
-        trav::k_compact<PickRows><<<fb.nblk, WGC, 0, ctx->stream>>>(PickRows{m}, 0, m, fb.nblk, 1, ga.row_ptr,
-                                                                      trav::queue_of(fb), fb.status, fb.C);
-        trav::k_expand<OpUpper><<<fb.expand_grid, WG, 0, ctx->stream>>>(OpUpper{b->row_ptr, ub}, 1, a->n, fb.C,
-                                                                          trav::queue_of(fb), a->col_ind);
+        {
+            int32_t *len = (int32_t *)ctx->slot(S_OUTER0 + 14, size_t(std::max<int64_t>(a->nnz, 1)) * 4);
+            int64_t *psum = (int64_t *)ctx->slot(S_OUTER0 + 15, (size_t(a->nnz) + 1) * 8);
+            k_entry_len<<<cvt::grid_for(ctx, a->nnz, 256), 256, 0, ctx->stream>>>(a->nnz, a->col_ind, b->row_ptr,
+                                                                                  b->m, len);
+            cvt::scan_wide(ctx, len, psum, a->nnz);
+            k_row_ub<<<g256, 256, 0, ctx->stream>>>(m, a->row_ptr, psum, ub, bad);
+        }
         // 2. bins
         k_bin<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 4096), (unsigned)ctx->cu_count * 2)), 256, 0,
                 ctx->stream>>>(m, ub, nnzc, lists, counts, products);
@@ -767,7 +759,10 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         int hcounts[NBINS];
         APSPIE_HIP(hipMemcpyAsync(hcounts, counts, sizeof(hcounts), hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipMemcpyAsync(&prod_total, products, 8, hipMemcpyDeviceToHost, ctx->stream));
+        int32_t hbad = 0;
+        APSPIE_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        APSPIE_CHECK(hbad == 0, APSPIE_ELIMIT, "a row of C has 2^31 or more products");
         for (int k = 0; k < NBINS; ++k) P.counts[k] = hcounts[k];
         // global tables for the largest rows: 2 * pow2ceil(ub) slots each
         int64_t *goff = nullptr;
