@@ -211,10 +211,22 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
 #pragma unroll
                 for (int x = 0; x < U; ++x) {
                     const int j = j0 + x * E + sub;
-                    cj[x] = __shfl(c, j, WAVE);
-                    uj[x] = __shfl(u, j, WAVE);
-                    wj[x] = __shfl(w, j, WAVE);
-                    const uint32_t lo = __shfl(fu_lo, j, WAVE), hi = __shfl(fu_hi, j, WAVE);
+                    uint32_t lo, hi;
+                    if constexpr (K == WAVE) {
+                        // one edge per wave step: its lane index is wave-uniform,
+                        // so a scalar read of that lane replaces the LDS shuffle
+                        cj[x] = __builtin_amdgcn_readlane(c, j);
+                        uj[x] = __builtin_amdgcn_readlane(u, j);
+                        wj[x] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w), j));
+                        lo = uint32_t(__builtin_amdgcn_readlane(int(fu_lo), j));
+                        hi = uint32_t(__builtin_amdgcn_readlane(int(fu_hi), j));
+                    } else {
+                        cj[x] = __shfl(c, j, WAVE);
+                        uj[x] = __shfl(u, j, WAVE);
+                        wj[x] = __shfl(w, j, WAVE);
+                        lo = __shfl(fu_lo, j, WAVE);
+                        hi = __shfl(fu_hi, j, WAVE);
+                    }
                     const uint64_t f = (uint64_t(hi) << 32) | lo;
                     act[x] = cj[x] >= 0 && ((f >> src) & 1ull);
                 }
