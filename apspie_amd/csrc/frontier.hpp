@@ -289,6 +289,46 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
     }
 }
 
+// Round 0 of a single-source traversal: what k_compact<Pick> computes when
+// only the seed is picked (the pick and its side effects, its bitmap words,
+// the queue entry, the tile table, the counters and the round descriptor
+// with its direction), without a pass over the vertex range.  State that
+// pass rewrites for every other vertex (frontier bitmap words, pull
+// snapshots) is cleared by the traversal's initialisation, and the control
+// block starts zeroed.  One workgroup.
+template <class Pick>
+__global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *__restrict__ row_ptr, Queue q,
+                                             Control *C) {
+    const int start = row_ptr[seed], deg = row_ptr[seed + 1] - start;
+    const int ntiles = (deg + TILE - 1) / TILE;
+    for (int t = threadIdx.x; t < ntiles; t += WG) q.tiles[t] = 0;
+    if (threadIdx.x != 0) return;
+    using PP = pick_phases<Pick>;
+    uint32_t payload = 0;
+    const typename PP::Pre pre = PP::load(pk, seed);
+    PP::pick(pk, seed, pre, payload);  // the seed is always picked
+    pk.on_new(seed);
+    if constexpr (has_on_balls<Pick>::value) pk.on_balls(1ull << (seed & (WAVE - 1)), seed & ~(WAVE - 1), 0, 1);
+    if (deg > 0) {
+        q.vert[0] = seed;
+        q.start[0] = start;
+        q.off[0] = 0;
+        q.val[0] = payload;
+    }
+    C->nnew[0][0] = 1;
+    C->acc.vertices[0] = 1;
+    C->acc.levels = 1;
+    C->acc.edges = (unsigned long long)deg;
+    Ctl c;
+    c.nF = deg > 0 ? 1 : 0;
+    c.E = deg;
+    c.nTiles = ntiles;
+    c.pull = 0;
+    if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, (unsigned long long)deg, C->ctl[1].pull);
+    C->ctl[0] = c;
+    C->log[0] = c;
+}
+
 // Inclusive max-scan of x over the TILE slots of a workgroup in slot order
 // (slot = wid*SEG + r*64 + lane); s_wmax holds one int per wave.  Contains a
 // workgroup barrier.

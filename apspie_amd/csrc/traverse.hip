@@ -306,22 +306,29 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
     }
 }
 
+// (fbits / fval: the frontier bitmap and pull snapshots of a traversal with
+// pull steps, written whole by every compaction but round 0's seed kernel)
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
-                                             uint64_t *status, Control *C, uint32_t *bin_totals, uint32_t thr0) {
+                                             uint64_t *status, Control *C, uint32_t *bin_totals, uint32_t thr0,
+                                             uint32_t *fbits, uint32_t *fval) {
     const int64_t gtid = int64_t(blockIdx.x) * WG + threadIdx.x;
     const int64_t stride = int64_t(gridDim.x) * WG;
     const int64_t mpad = int64_t(nblk) * CHUNK;
     if (mode == MODE_BFS) {
         for (int64_t i = gtid; i < mpad / 16; i += stride) {
             *(uint4 *)(flags + 16 * i) = make_uint4(0, 0, 0, 0);
-            if ((i & 1) == 0) visited[i >> 1] = 0u;  // one bitmap word per 32 vertices
+            if ((i & 1) == 0) {  // one bitmap word per 32 vertices
+                visited[i >> 1] = 0u;
+                if (fbits) fbits[i >> 1] = 0u;
+            }
         }
         for (int64_t i = gtid; i < m; i += stride) depth[i] = -1;
     } else {
         for (int64_t i = gtid; i < m; i += stride) {
             dist[i] = INF_BITS;
             if (prev) prev[i] = INF_BITS;
+            if (fval) fval[i] = NOT_IN_FRONTIER;
         }
     }
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
@@ -447,22 +454,20 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     uint32_t thr0 = ~0u;
     if (delta > 0.0f) memcpy(&thr0, &delta, 4);
     k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE, g->m, b.nblk, depth, dist, b.prev, b.flags,
-                                                       b.visited, b.status, b.C, bin_tot, thr0);
+                                                       b.visited, b.status, b.C, bin_tot, thr0,
+                                                       MODE == MODE_BFS ? b.fbits : nullptr, fval);
     if (pull && MODE == MODE_SSSP)
         k_tile_rows<<<vert_grid, WG, 0, ctx->stream>>>(g->m, g->csc_col_ptr, b.trows);
     ctx->prof_end();
-    auto compact = [&](int level, int seed) {
-        if (MODE == MODE_BFS)
-            launch_compact(ctx, b, g,
-                           PickBfs{b.flags, b.visited, depth, level, seed,
-                                   PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}},
-                           level, level == 0);
+    auto pick_for = [&](int level, int seed) {
+        if constexpr (MODE == MODE_BFS)
+            return PickBfs{b.flags, b.visited, depth, level, seed,
+                           PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}};
         else
-            launch_compact(ctx, b, g,
-                           PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
-                                    binned ? ctx->tune.sssp_bin_ratio : 0, &b.C->thr, delta},
-                           level, level == 0);
+            return PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
+                            binned ? ctx->tune.sssp_bin_ratio : 0, &b.C->thr, delta};
     };
+    auto compact = [&](int level) { launch_compact(ctx, b, g, pick_for(level, -1), level, 0); };
     auto expand = [&](int level) {
         if (MODE == MODE_BFS && pull) {
             ctx->prof_begin(APSPIE_KCLASS_ROUND);
@@ -492,7 +497,13 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                                                         g->csc_val, b.trows, fval, dist);
         ctx->prof_end();
     };
-    compact(0, src);
+    // round 0: the source alone, without a pass over the vertices
+    {
+        const auto pk0 = pick_for(0, src);
+        ctx->prof_begin(APSPIE_KCLASS_COMPACT);
+        k_seed<std::decay_t<decltype(pk0)>><<<1, WG, 0, ctx->stream>>>(pk0, src, g->row_ptr, queue_of(b), b.C);
+        ctx->prof_end();
+    }
     // chunks of 4, 2, 2, 4, 8, ... rounds between frontier checks: RMAT
     // traversals end within 6-12 rounds, and a round launched past the end
     // costs its kernels' launch
@@ -501,7 +512,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     for (;;) {
         for (int i = 0; i < chunk; ++i, ++level) {
             expand(level);
-            compact(level, -1);
+            compact(level);
         }
         APSPIE_HIP(hipGetLastError());
         APSPIE_HIP(hipMemcpyAsync((void *)host_ctl, &b.C->ctl[(level - 1) & 1], sizeof(Ctl), hipMemcpyDeviceToHost,
@@ -568,7 +579,8 @@ int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, flo
         APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         ctx->prof_begin(APSPIE_KCLASS_OTHER);
         k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE_STEP, g->m, b.nblk, nullptr, y, nullptr,
-                                                           b.flags, b.visited, b.status, b.C, nullptr, ~0u);
+                                                           b.flags, b.visited, b.status, b.C, nullptr, ~0u,
+                                                           nullptr, nullptr);
         ctx->prof_end();
         launch_compact(ctx, b, g, PickStep{d_x, g->m}, 0, 1);
         launch_expand(ctx, b, g, OpRelax{g->val, y}, 1);
