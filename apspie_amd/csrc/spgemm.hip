@@ -33,7 +33,9 @@
 // (products that cancel) are kept, as the reference's hash does
 // (src/spgemmKernel.cuh:194-198).
 #include <algorithm>
+#include <climits>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "convert.hpp"
@@ -464,6 +466,174 @@ __global__ __launch_bounds__(T) void k_rows(const int32_t *__restrict__ rows, in
     }
 }
 
+// Rows of at most 16 products, four to a wave: a 16-lane group per row, in
+// registers, no LDS.  Lane e takes A entry e of each chunk of 16, the group
+// scans the B row lengths, and product p of the row lands on lane p (the
+// row's products, ub <= 16).  A sorting network on (column, p) puts equal
+// columns side by side in increasing k; each run's first lane sums the run
+// in FP64 in that order and the run starts are written at their rank: at
+// crp[i] (NUMERIC) after nnzc[i] (always).  Every exchange inside a 16-lane
+// row is a DPP move (row shifts, row broadcasts, quad permutes, mirrors),
+// which never leaves the row, so the four groups stay apart.
+template <int CTRL>
+__device__ __forceinline__ int row_dpp(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xf, 0xf, true);
+}
+
+constexpr int DPP_XOR1 = 0xb1;         // quad_perm [1,0,3,2]
+constexpr int DPP_XOR2 = 0x4e;         // quad_perm [2,3,0,1]
+constexpr int DPP_XOR3 = 0x1b;         // quad_perm [3,2,1,0]
+constexpr int DPP_HALF_MIRROR = 0x141;  // lane i of each 8 <- lane 7 - i
+constexpr int DPP_MIRROR = 0x140;       // lane i of the row <- lane 15 - i
+constexpr int DPP_SHL0 = 0x100;         // + n: lane i <- lane i + n (0 past the row)
+constexpr int DPP_SHR0 = 0x110;         // + n: lane i <- lane i - n (0 before the row)
+constexpr int DPP_BCAST0 = 0x150;       // + n: every lane of the row <- lane n
+
+__device__ __forceinline__ int row16_incl_sum(int x) {
+    x += row_dpp<DPP_SHR0 + 1>(x);
+    x += row_dpp<DPP_SHR0 + 2>(x);
+    x += row_dpp<DPP_SHR0 + 4>(x);
+    x += row_dpp<DPP_SHR0 + 8>(x);
+    return x;
+}
+
+// The entry of the chunk that holds product t: the number of entries e < ne
+// whose inclusive end incl[e] <= t (the ends do not decrease, so these are a
+// prefix and the next entry, of length > 0, holds t).  ne is wave-uniform.
+template <int... E>
+__device__ __forceinline__ int row16_owner(int incl, int t, int ne, std::integer_sequence<int, E...>) {
+    int own = 0;
+    ((own += (E < ne && row_dpp<DPP_BCAST0 + E>(incl) <= t) ? 1 : 0), ...);
+    return own;
+}
+
+// One compare-exchange of the network: the lane takes its partner's key
+// when it keeps the smaller key (lower) and the partner's is smaller, or it
+// keeps the larger (upper) and the partner's is larger.  Keys are distinct.
+template <class Partner>
+__device__ __forceinline__ void row16_cx(int &col, int &prod, bool lower, Partner partner) {
+    const int oc = partner(col), op = partner(prod);
+    const bool smaller = oc < col || (oc == col && op < prod);
+    if (lower == smaller) {
+        col = oc;
+        prod = op;
+    }
+}
+
+// Sum of the run that starts on this lane, over the next lanes with the same
+// column, left to right; mp bounds every group's products (wave-uniform).
+// The moves run with every lane active, the sums under the lane's own flag.
+template <int... D>
+__device__ __forceinline__ double row16_run_sum(int col, double v, int lane, int mp, std::integer_sequence<int, D...>) {
+    double acc = v;
+    bool run = true;
+    auto step = [&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        if (d < mp) {
+            const int nc = row_dpp<DPP_SHL0 + d>(col);
+            const int lo = row_dpp<DPP_SHL0 + d>(__double2loint(v)), hi = row_dpp<DPP_SHL0 + d>(__double2hiint(v));
+            run = run && lane + d < 16 && nc == col;
+            if (run) acc += __hiloint2double(hi, lo);
+        }
+    };
+    (step(std::integral_constant<int, D + 1>{}), ...);
+    return acc;
+}
+
+template <bool NUMERIC>
+__global__ __launch_bounds__(256) void k_rows_tiny(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
+                                                   int32_t *nnzc, const int32_t *__restrict__ crp, int32_t *ccol,
+                                                   float *cval) {
+    const int lane = threadIdx.x & 15, grp = (threadIdx.x & (WAVE - 1)) >> 4;
+    const int wid = (blockIdx.x * 256 + threadIdx.x) / WAVE, nw = gridDim.x * (256 / WAVE);
+    // the loop bound is wave-uniform (four rows a step): every DPP move and
+    // shuffle below runs with all 64 lanes active
+    for (int rb = wid * 4; rb < nrows; rb += nw * 4) {
+        const int r = rb + grp;
+        const bool live = r < nrows;
+        const int i = live ? rows[r] : 0;
+        const int ab = live ? A.rp[i] : 0, ae = live ? A.rp[i + 1] : 0;
+        int col = INT_MAX, prod = lane;  // this lane's product: column, index in the row
+        double val = 0.0;
+        int pbase = 0;
+        for (int cb = ab;; cb += 16) {
+            const uint64_t have = __ballot(cb + lane < ae);
+            if (!have) break;
+            // entries in the wave's fullest group (each group's are a prefix)
+            const uint32_t g16 = uint32_t(have | (have >> 16) | (have >> 32) | (have >> 48)) & 0xffffu;
+            const int ne = 32 - __clz(g16);
+            const int j = cb + lane;
+            int L = 0, bs = 0;
+            float a = 0.f;
+            if (j < ae) {
+                const int k = A.col[j];
+                if ((unsigned)k < (unsigned)B.m) {
+                    a = A.val ? A.val[j] : 1.0f;
+                    bs = B.rp[k];
+                    L = B.rp[k + 1] - bs;
+                }
+            }
+            const int incl = row16_incl_sum(L), excl = incl - L;
+            const int tot = row_dpp<DPP_BCAST0 + 15>(incl);
+            const int t = lane - pbase;  // products pbase .. pbase + tot - 1 sit on those lanes
+            const int own = min(row16_owner(incl, t, ne, std::make_integer_sequence<int, 16>{}), 15);
+            const int obs = __shfl(bs, own, 16), oex = __shfl(excl, own, 16), ol = __shfl(L, own, 16);
+            const float oa = __shfl(a, own, 16);
+            if (t >= 0 && t < tot && t - oex >= 0 && t - oex < ol) {
+                const int q = obs + (t - oex);
+                const int cq = B.col[q];
+                if ((unsigned)cq < (unsigned)B.n) {
+                    col = cq;
+                    val = double(oa) * double(B.val ? B.val[q] : 1.0f);
+                }
+            }
+            pbase += tot;
+        }
+        // the most products of any group of the wave
+        const int mp = max(max(__builtin_amdgcn_readlane(pbase, 0), __builtin_amdgcn_readlane(pbase, 16)),
+                           max(__builtin_amdgcn_readlane(pbase, 32), __builtin_amdgcn_readlane(pbase, 48)));
+        // sorting network on (column, product): each merge of two sorted
+        // blocks of k/2 compares mirrored lanes, then halves; the stages
+        // for blocks of k are needed only when some group has more than
+        // k/2 products (the rest of the lanes hold (INT_MAX, lane), in order)
+        auto x1 = [](int x) { return row_dpp<DPP_XOR1>(x); };
+        auto x2 = [](int x) { return row_dpp<DPP_XOR2>(x); };
+        auto x4 = [lane](int x) {
+            const int up = row_dpp<DPP_SHR0 + 4>(x), down = row_dpp<DPP_SHL0 + 4>(x);
+            return (lane & 4) ? up : down;
+        };
+        if (mp > 1) row16_cx(col, prod, (lane & 1) == 0, x1);
+        if (mp > 2) {
+            row16_cx(col, prod, (lane & 2) == 0, [](int x) { return row_dpp<DPP_XOR3>(x); });
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 4) {
+            row16_cx(col, prod, (lane & 4) == 0, [](int x) { return row_dpp<DPP_HALF_MIRROR>(x); });
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 8) {
+            row16_cx(col, prod, (lane & 8) == 0, [](int x) { return row_dpp<DPP_MIRROR>(x); });
+            row16_cx(col, prod, (lane & 4) == 0, x4);
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        val = __shfl(val, prod, 16);  // the products stayed on their lanes
+        const int prev = row_dpp<DPP_SHR0 + 1>(col);
+        const bool start = col != INT_MAX && (lane == 0 || prev != col);
+        const double acc = row16_run_sum(col, val, lane, mp, std::make_integer_sequence<int, 15>{});
+        const uint64_t ball = __ballot(start);
+        const uint32_t mine = uint32_t(ball >> (grp * 16)) & 0xffffu;
+        const int rank = __popc(mine & ((1u << lane) - 1u));
+        if (live && lane == 0) nnzc[i] = __popc(mine);
+        if (NUMERIC && live && start && rank < pbase) {  // runs <= products <= ub
+            const int o = crp[i] + rank;
+            ccol[o] = col;
+            cval[o] = float(acc);
+        }
+    }
+}
+
 // The rows with the most products, by expand-sort-compress instead of
 // global hash tables (whose inserts are device-scope atomics, a few G/s):
 // row r of the list writes its products at poff[r] in the order of A's
@@ -690,8 +860,8 @@ void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, i
         return std::max(1u, std::min((unsigned)n, (unsigned)ctx->cu_count * per_cu));
     };
     if (P.counts[0])
-        k_rows<64, 64, NUMERIC, false><<<grid(P.counts[0], 16), 64, 0, ctx->stream>>>(
-            P.lists + 0 * int64_t(m), P.counts[0], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+        k_rows_tiny<NUMERIC><<<std::max(1u, std::min(ceil_div(P.counts[0], 16), (unsigned)ctx->cu_count * 16)), 256, 0,
+                               ctx->stream>>>(P.lists + 0 * int64_t(m), P.counts[0], A, B, nnzc, crp, ccol, cval);
     if (P.counts[1])
         k_rows<128, 512, NUMERIC, false><<<grid(P.counts[1], 8), 128, 0, ctx->stream>>>(
             P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
