@@ -4,9 +4,9 @@
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
 //                edge-parallel over A with the frontier kernels.
-//   2. bins    rows by ub: <= 16, <= 128, <= 1024, <= 2048 (LDS tables of
-//              64 / 512 / 4096 / 8192 slots, one wave .. 512 threads per row)
-//              and > 2048 (the long rows).
+//   2. bins    rows by ub: <= 16 (four rows a wave, in registers), <= 128,
+//              <= 256, <= 512, <= 1024, <= 2048 (LDS tables of 512 .. 8192
+//              slots, 128 .. 512 threads per row) and > 2048 (the long rows).
 //   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
 //      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
 //      once) and write the sorted row to a staging area at the prefix of the

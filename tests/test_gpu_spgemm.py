@@ -137,3 +137,85 @@ def test_spgemm_long_rows_with_columns_outside_b(ctx):
     assert np.array_equal(out[0].row_ptr, out[1].row_ptr) and np.array_equal(out[0].col_ind, out[1].col_ind)
     assert np.allclose(out[0].val, out[1].val, rtol=1e-5, atol=0)
     assert out[1].col_ind.size == 0 or out[1].col_ind.max() < 3000
+
+
+def _rows_of(rng, lens, draw):
+    """CSR arrays of rows with len[r] distinct columns each, drawn by draw(r, l)."""
+    rp = np.zeros(len(lens) + 1, np.int32)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(draw(r, l)) for r, l in enumerate(lens)]).astype(np.int32)
+    val = rng.uniform(-2.0, 2.0, col.size).astype(np.float32)
+    return rp, col, val
+
+
+def skewed_long_rows():
+    """A * B whose rows are all over 2048 products: rows over uniform B rows
+    (up to 21000 products), rows whose B rows crowd a band of 300 low columns,
+    and two rows with 11500 and 12000 products on column 7 alone."""
+    rng = np.random.default_rng(17)
+    n, mb = 60000, 20000
+    uni = lambda k: rng.choice(np.arange(300, n), k, replace=False)
+
+    def b_row(r, l):
+        if r < 4000:  # uniform
+            return rng.choice(n, l, replace=False)
+        if r < 8000:  # 20 in the band [0, 300), 20 beyond it
+            return np.concatenate([rng.choice(300, 20, replace=False), uni(20)])
+        return np.concatenate([[7], uni(4)])  # column 7 and 4 beyond the band
+    blens = np.where(np.arange(mb) < 4000, 30, np.where(np.arange(mb) < 8000, 40, 5))
+    brp, bcol, bval = _rows_of(rng, blens, b_row)
+    hb = A.HostCsr(mb, n, brp, bcol, bval)
+    alens = np.concatenate([rng.integers(80, 700, 20), rng.integers(200, 600, 10), [11500, 12000]])
+
+    def a_row(r, l):
+        if r < 20:
+            return rng.choice(4000, l, replace=False)
+        if r < 30:
+            return 4000 + rng.choice(4000, l, replace=False)
+        return 8000 + rng.choice(12000, l, replace=False)
+    arp, acol, aval = _rows_of(rng, alens, a_row)
+    ha = A.HostCsr(len(alens), mb, arp, acol, aval)
+    assert long_row_products(ha, hb).min() > 2048
+    return ha, hb
+
+
+def test_spgemm_long_rows_skewed_exact(ctx):
+    """Every row of C from the long-row paths, over skewed columns (a crowded
+    band, one column with 12000 products in a row).  Expand-sort-compress
+    (default) sums each entry in FP64 in k order, as the oracle does, so its
+    C is identical, values included; the global hash tables match within
+    1e-5."""
+    ha, hb = skewed_long_rows()
+    want, prods = O.spgemm(ocsr(ha), ocsr(hb))
+    saved = ctx.tuning()
+    try:
+        got = {}
+        for mode in (1, 0):
+            ctx.set_tuning(spgemm_long_rows=mode)
+            got[mode], st = ctx.spgemm(ctx.upload(ha), ctx.upload(hb))
+            assert st.edges == prods and st.vertices == want.nnz
+    finally:
+        ctx.set_tuning(**saved)
+    c = got[1]
+    assert np.array_equal(c.row_ptr, want.row_ptr) and np.array_equal(c.col_ind, want.col_ind)
+    assert np.array_equal(c.val, want.val)
+    check(got[0], want)
+
+
+@pytest.mark.skip(reason="the LDS-sorted long-row path (a workgroup per long row, column parts radix-sorted "
+                  "in LDS, spgemm_long_rows=2) faulted on its first GPU run; neither reading nor a CPU "
+                  "emulation of it showed a cause, so the path is not in the library")
+def test_spgemm_long_rows_sorted_in_lds_exact(ctx):
+    """The LDS-sorted long rows: C identical to the oracle's, values included,
+    on the skewed rows (column parts, a band halved until it fits LDS, and a
+    column with more products than LDS holds)."""
+    ha, hb = skewed_long_rows()
+    want, _ = O.spgemm(ocsr(ha), ocsr(hb))
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(spgemm_long_rows=2)
+        c, _ = ctx.spgemm(ctx.upload(ha), ctx.upload(hb))
+    finally:
+        ctx.set_tuning(**saved)
+    assert np.array_equal(c.row_ptr, want.row_ptr) and np.array_equal(c.col_ind, want.col_ind)
+    assert np.array_equal(c.val, want.val)
