@@ -4,9 +4,10 @@
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
 //                edge-parallel over A with the frontier kernels.
-//   2. bins    rows by ub: <= 16 (four rows a wave, in registers), <= 128,
-//              <= 256, <= 512, <= 1024, <= 2048 (LDS tables of 512 .. 8192
-//              slots, 128 .. 512 threads per row) and > 2048 (the long rows).
+//   2. bins    rows by ub: <= 16 (four rows a wave) and <= 64 (a wave per
+//              row), both in registers; <= 128, <= 256, <= 512, <= 1024,
+//              <= 2048 (LDS tables of 512 .. 8192 slots, 128 .. 512 threads
+//              per row); > 2048 (the long rows).
 //   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
 //      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
 //      once) and write the sorted row to a staging area at the prefix of the
@@ -57,8 +58,8 @@ struct CsrD {
 // A table of S slots has H = S / 2 home slots and takes rows of at most S / 4
 // products, so keys fill at most half of the home range (linear probing with
 // order-preserving homes clusters badly when that range is nearly full).
-constexpr int NBINS = 7;
-constexpr int BIN_UB[NBINS - 1] = {16, 128, 256, 512, 1024, 2048};
+constexpr int NBINS = 8;
+constexpr int BIN_UB[NBINS - 1] = {16, 64, 128, 256, 512, 1024, 2048};
 constexpr int LONG = NBINS - 1;  // the bin of rows beyond the LDS tables
 
 // Product bounds ub[i] = sum over A(i,:) of nnz(B(k,:)): every entry's B
@@ -634,6 +635,122 @@ __global__ __launch_bounds__(256) void k_rows_tiny(const int32_t *__restrict__ r
     }
 }
 
+// Rows of 17 .. 64 products, one to a wave, in registers: the same steps
+// as k_rows_tiny over 64 lanes.  The owner search reads the scan through
+// readlane (wave-uniform), the sorting network's steps inside a 16-lane row
+// are DPP moves and its three steps across rows (mirrors over 32 and 64
+// lanes, xor 16) are lane permutes; the run sums stop once no run goes on.
+template <bool NUMERIC>
+__global__ __launch_bounds__(256) void k_rows_wave(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
+                                                   int32_t *nnzc, const int32_t *__restrict__ crp, int32_t *ccol,
+                                                   float *cval) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wid = (blockIdx.x * 256 + threadIdx.x) / WAVE, nw = gridDim.x * (256 / WAVE);
+    for (int r = wid; r < nrows; r += nw) {  // wave-uniform
+        const int i = rows[r];
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        int col = INT_MAX, prod = lane;  // this lane's product: column, index in the row
+        double val = 0.0;
+        int pbase = 0;
+        for (int cb = ab;; cb += WAVE) {
+            const uint64_t have = __ballot(cb + lane < ae);
+            if (!have) break;
+            const int ne = __popcll(have);
+            const int j = cb + lane;
+            int L = 0, bs = 0;
+            float a = 0.f;
+            if (j < ae) {
+                const int k = A.col[j];
+                if ((unsigned)k < (unsigned)B.m) {
+                    a = A.val ? A.val[j] : 1.0f;
+                    bs = B.rp[k];
+                    L = B.rp[k + 1] - bs;
+                }
+            }
+            const int incl = int(wave_incl_sum_dpp(uint32_t(L))), excl = incl - L;
+            const int tot = wave_last(incl);
+            const int t = lane - pbase;  // products pbase .. pbase + tot - 1 sit on those lanes
+            int own = 0;
+            for (int e = 0; e < ne; ++e) own += __builtin_amdgcn_readlane(incl, e) <= t ? 1 : 0;
+            own = min(own, WAVE - 1);
+            const int obs = __shfl(bs, own), oex = __shfl(excl, own), ol = __shfl(L, own);
+            const float oa = __shfl(a, own);
+            if (t >= 0 && t < tot && t - oex >= 0 && t - oex < ol) {
+                const int q = obs + (t - oex);
+                const int cq = B.col[q];
+                if ((unsigned)cq < (unsigned)B.n) {
+                    col = cq;
+                    val = double(oa) * double(B.val ? B.val[q] : 1.0f);
+                }
+            }
+            pbase += tot;
+        }
+        const int mp = pbase;  // the row's products (wave-uniform)
+        auto x1 = [](int x) { return row_dpp<DPP_XOR1>(x); };
+        auto x2 = [](int x) { return row_dpp<DPP_XOR2>(x); };
+        auto x4 = [lane](int x) {
+            const int up = row_dpp<DPP_SHR0 + 4>(x), down = row_dpp<DPP_SHL0 + 4>(x);
+            return (lane & 4) ? up : down;
+        };
+        auto x8 = [lane](int x) {
+            const int up = row_dpp<DPP_SHR0 + 8>(x), down = row_dpp<DPP_SHL0 + 8>(x);
+            return (lane & 8) ? up : down;
+        };
+        auto perm = [lane](int mask) { return [lane, mask](int x) { return __shfl(x, lane ^ mask); }; };
+        if (mp > 1) row16_cx(col, prod, (lane & 1) == 0, x1);
+        if (mp > 2) {
+            row16_cx(col, prod, (lane & 2) == 0, [](int x) { return row_dpp<DPP_XOR3>(x); });
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 4) {
+            row16_cx(col, prod, (lane & 4) == 0, [](int x) { return row_dpp<DPP_HALF_MIRROR>(x); });
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 8) {
+            row16_cx(col, prod, (lane & 8) == 0, [](int x) { return row_dpp<DPP_MIRROR>(x); });
+            row16_cx(col, prod, (lane & 4) == 0, x4);
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 16) {
+            row16_cx(col, prod, (lane & 16) == 0, perm(31));
+            row16_cx(col, prod, (lane & 8) == 0, x8);
+            row16_cx(col, prod, (lane & 4) == 0, x4);
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        if (mp > 32) {
+            row16_cx(col, prod, (lane & 32) == 0, perm(63));
+            row16_cx(col, prod, (lane & 16) == 0, perm(16));
+            row16_cx(col, prod, (lane & 8) == 0, x8);
+            row16_cx(col, prod, (lane & 4) == 0, x4);
+            row16_cx(col, prod, (lane & 2) == 0, x2);
+            row16_cx(col, prod, (lane & 1) == 0, x1);
+        }
+        val = __shfl(val, prod);  // the products stayed on their lanes
+        const int prev = __shfl_up(col, 1);
+        const bool start = col != INT_MAX && (lane == 0 || prev != col);
+        double acc = val;
+        bool run = true;
+        for (int d = 1; d < mp; ++d) {  // wave-uniform: every lane shuffles
+            const int nc = __shfl_down(col, d);
+            const double nv = __shfl_down(val, d);
+            run = run && lane + d < WAVE && nc == col;
+            if (run) acc += nv;
+            if (!__ballot(run)) break;
+        }
+        const uint64_t ball = __ballot(start);
+        const int rank = __builtin_amdgcn_mbcnt_hi(uint32_t(ball >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(ball), 0));
+        if (lane == 0) nnzc[i] = __popcll(ball);
+        if (NUMERIC && start && rank < mp) {  // runs <= products <= ub
+            const int o = crp[i] + rank;
+            ccol[o] = col;
+            cval[o] = float(acc);
+        }
+    }
+}
+
 // The rows with the most products, by expand-sort-compress instead of
 // global hash tables (whose inserts are device-scope atomics, a few G/s):
 // row r of the list writes its products at poff[r] in the order of A's
@@ -863,20 +980,24 @@ void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, i
         k_rows_tiny<NUMERIC><<<std::max(1u, std::min(ceil_div(P.counts[0], 16), (unsigned)ctx->cu_count * 16)), 256, 0,
                                ctx->stream>>>(P.lists + 0 * int64_t(m), P.counts[0], A, B, nnzc, crp, ccol, cval);
     if (P.counts[1])
-        k_rows<128, 512, NUMERIC, false><<<grid(P.counts[1], 8), 128, 0, ctx->stream>>>(
-            P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+        k_rows_wave<NUMERIC><<<std::max(1u, std::min(ceil_div(P.counts[1], 256 / WAVE), (unsigned)ctx->cu_count * 16)),
+                               256, 0, ctx->stream>>>(P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol,
+                                                      cval);
     if (P.counts[2])
-        k_rows<128, 1024, NUMERIC, false><<<grid(P.counts[2], 8), 128, 0, ctx->stream>>>(
+        k_rows<128, 512, NUMERIC, false><<<grid(P.counts[2], 8), 128, 0, ctx->stream>>>(
             P.lists + 2 * int64_t(m), P.counts[2], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[3])
-        k_rows<256, 2048, NUMERIC, false><<<grid(P.counts[3], 4), 256, 0, ctx->stream>>>(
+        k_rows<128, 1024, NUMERIC, false><<<grid(P.counts[3], 8), 128, 0, ctx->stream>>>(
             P.lists + 3 * int64_t(m), P.counts[3], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[4])
-        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[4], 4), 256, 0, ctx->stream>>>(
+        k_rows<256, 2048, NUMERIC, false><<<grid(P.counts[4], 4), 256, 0, ctx->stream>>>(
             P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[5])
-        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[5], 1), 512, 0, ctx->stream>>>(
+        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[5], 4), 256, 0, ctx->stream>>>(
             P.lists + 5 * int64_t(m), P.counts[5], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    if (P.counts[6])
+        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[6], 1), 512, 0, ctx->stream>>>(
+            P.lists + 6 * int64_t(m), P.counts[6], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[LONG])
         k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[LONG], 2), 1024, 0, ctx->stream>>>(
             P.lists + LONG * int64_t(m), P.counts[LONG], A, B, nnzc, crp, ccol, cval, gkeys, gvals, goff, gslots);
