@@ -882,19 +882,43 @@ __global__ __launch_bounds__(256) void k_stage_ub(int m, const int32_t *__restri
         ubs[i] = ub[i] <= BIN_UB[NBINS - 2] ? ub[i] : 0;
 }
 
-// C(i,:) of the LDS-table rows from the staging area (one wave per row)
+// C(i,:) of the rows of the LDS and register bins from the staging area.
+// A workgroup takes 256 consecutive rows: their outputs [crp[r0], crp[r1])
+// are contiguous, so its threads stride over them (coalesced writes) and
+// find each output's row by a binary search over the block's row pointers
+// in LDS; the outputs of long rows (written by their own path) are skipped.
 __global__ __launch_bounds__(256) void k_unstage(int m, const int32_t *__restrict__ ub,
-                                                 const int32_t *__restrict__ ubpre, const int32_t *__restrict__ nnzc,
-                                                 const int32_t *__restrict__ crp, const int32_t *__restrict__ scol,
-                                                 const float *__restrict__ sval, int32_t *ccol, float *cval) {
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int nw = gridDim.x * (256 / WAVE);
-    for (int i = blockIdx.x * (256 / WAVE) + threadIdx.x / WAVE; i < m; i += nw) {
-        if (ub[i] > BIN_UB[NBINS - 2]) continue;  // a long row, written by its own path
-        const int src = ubpre[i], dst = crp[i], n = nnzc[i];
-        for (int e = lane; e < n; e += WAVE) {
-            ccol[dst + e] = scol[src + e];
-            cval[dst + e] = sval[src + e];
+                                                 const int32_t *__restrict__ ubpre, const int32_t *__restrict__ crp,
+                                                 const int32_t *__restrict__ scol, const float *__restrict__ sval,
+                                                 int32_t *ccol, float *cval) {
+    __shared__ int32_t s_rp[257], s_src[256];
+    for (int r0 = blockIdx.x * 256; r0 < m; r0 += gridDim.x * 256) {
+        const int nr = min(256, m - r0);
+        __syncthreads();  // the previous block's searches are done
+        for (int x = threadIdx.x; x <= nr; x += 256) s_rp[x] = crp[r0 + x];
+        if (threadIdx.x < nr) {
+            const int i = r0 + threadIdx.x;
+            s_src[threadIdx.x] = ub[i] > BIN_UB[NBINS - 2] ? -1 : ubpre[i];
+        }
+        __syncthreads();
+        const int hi = s_rp[nr];
+        int row = 0;  // the last row x with s_rp[x] <= o (o only grows)
+        for (int o = s_rp[0] + threadIdx.x; o < hi;) {
+            int top = nr - 1;
+            while (row < top) {
+                const int mid = (row + top + 1) >> 1;
+                if (s_rp[mid] <= o) row = mid;
+                else top = mid - 1;
+            }
+            const int src = s_src[row];
+            if (src < 0) {  // a long row: past its outputs, same stride class
+                o += (s_rp[row + 1] - o + 255) / 256 * 256;
+                continue;
+            }
+            const int q = src + (o - s_rp[row]);
+            ccol[o] = scol[q];
+            cval[o] = sval[q];
+            o += 256;
         }
     }
 }
@@ -1148,8 +1172,8 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         }
         // 5. numeric
         if (one_pass)
-            k_unstage<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 256 / WAVE), (unsigned)ctx->cu_count * 32)),
-                        256, 0, ctx->stream>>>(m, ub, ubpre, nnzc, c->row_ptr, scol, sval, c->col_ind, c->val);
+            k_unstage<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 256), (unsigned)ctx->cu_count * 8)), 256, 0,
+                        ctx->stream>>>(m, ub, ubpre, c->row_ptr, scol, sval, c->col_ind, c->val);
         else
             launch_bins<true>(ctx, P03, A, B, nnzc, c->row_ptr, c->col_ind, c->val, nullptr, nullptr, nullptr,
                               nullptr);
