@@ -130,23 +130,32 @@ __global__ __launch_bounds__(T) void k_hist(const int32_t *__restrict__ key, int
 // first item of digit d goes.  Wave w of a workgroup owns the tile's items
 // [w * 1024, (w + 1) * 1024) and keeps them in registers: it counts their
 // digits into its own LDS row, one thread per digit turns the rows into each
-// wave's starting position per digit, and the wave walks its items in order,
+// wave's tile-local starting position per digit (the digits before d, then
+// digit d in the waves before w), and the wave walks its items in order,
 // ranking lanes of equal digit with eight ballots and advancing its row (the
 // LDS operations of one wave execute in order, so no barrier is needed
-// between rounds).  Equal digits keep their input order.
+// between rounds).  The items land in LDS in the tile's digit order; then
+// consecutive threads write consecutive items, so each digit's run of the
+// tile goes out as one contiguous store instead of one line per lane.
+// Equal digits keep their input order.
 __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, const int32_t *__restrict__ idx,
                                                int64_t n, int shift, int ntiles, const int32_t *__restrict__ off,
                                                int32_t *key_out, int32_t *idx_out) {
     constexpr int NW = T / WAVE, PER = TILE_N / NW, R = PER / WAVE;
     static_assert(RADIX == T, "one thread per digit");
     __shared__ int32_t s_cnt[NW][RADIX];
+    __shared__ int32_t s_goff[RADIX];  // global position of the tile's digit-d run, less its tile-local start
+    __shared__ int32_t s_wsum[NW];
+    __shared__ int32_t s_k[TILE_N], s_x[TILE_N];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const uint64_t lt = (1ull << lane) - 1;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 #pragma unroll
         for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
         __syncthreads();
-        const int64_t wb = int64_t(tile) * TILE_N + int64_t(wid) * PER;
+        const int64_t t0 = int64_t(tile) * TILE_N;
+        const int items = int(min<int64_t>(TILE_N, n - t0));
+        const int64_t wb = t0 + int64_t(wid) * PER;
         int k[R], x[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -158,8 +167,17 @@ __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, 
         for (int r = 0; r < R; ++r)
             if (k[r] >= 0) atomicAdd(&s_cnt[wid][(uint32_t(k[r]) >> shift) & (RADIX - 1)], 1);
         __syncthreads();
-        {  // thread d: the waves' starting positions for digit d
-            int run = off[int64_t(tid) * ntiles + tile];
+        {  // thread d: digit d's tile-local start, then each wave's start
+            int tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) tot += s_cnt[w][tid];
+            const int incl = int(wave_incl_scan32(uint32_t(tot), lane));
+            if (lane == WAVE - 1) s_wsum[wid] = incl;
+            __syncthreads();
+            int run = incl - tot;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) run += w < wid ? s_wsum[w] : 0;
+            s_goff[tid] = off[int64_t(tid) * ntiles + tile] - run;
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
                 const int c = s_cnt[w][tid];
@@ -182,11 +200,18 @@ __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, 
             const int pos = valid ? s_cnt[wid][d] + rank : 0;
             if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
             if (valid) {
-                key_out[pos] = k[r];
-                idx_out[pos] = x[r];
+                s_k[pos] = k[r];
+                s_x[pos] = x[r];
             }
         }
-        __syncthreads();  // the next tile clears the rows
+        __syncthreads();
+        for (int q = tid; q < items; q += T) {
+            const int kk = s_k[q];
+            const int g = s_goff[(uint32_t(kk) >> shift) & (RADIX - 1)] + q;
+            key_out[g] = kk;
+            idx_out[g] = s_x[q];
+        }
+        __syncthreads();  // the next tile clears the rows and refills the stage
     }
 }
 
