@@ -6,7 +6,7 @@
 //                edge-parallel over A with the frontier kernels.
 //   2. bins    rows by ub: <= 16 (four rows a wave) and <= 64 (a wave per
 //              row), both in registers; <= 128, <= 256, <= 512, <= 1024,
-//              <= 2048 (LDS tables of 512 .. 8192 slots, 128 .. 512 threads
+//              <= 2048 (LDS tables of 512 .. 8192 slots, 128 .. 1024 threads
 //              per row); > 2048 (the long rows).
 //   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
 //      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
@@ -1017,10 +1017,10 @@ void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, i
         k_rows<256, 2048, NUMERIC, false><<<grid(P.counts[4], 4), 256, 0, ctx->stream>>>(
             P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[5])
-        k_rows<256, 4096, NUMERIC, false><<<grid(P.counts[5], 4), 256, 0, ctx->stream>>>(
+        k_rows<512, 4096, NUMERIC, false><<<grid(P.counts[5], 2), 512, 0, ctx->stream>>>(
             P.lists + 5 * int64_t(m), P.counts[5], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[6])
-        k_rows<512, 8192, NUMERIC, false><<<grid(P.counts[6], 1), 512, 0, ctx->stream>>>(
+        k_rows<1024, 8192, NUMERIC, false><<<grid(P.counts[6], 1), 1024, 0, ctx->stream>>>(
             P.lists + 6 * int64_t(m), P.counts[6], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[LONG])
         k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[LONG], 2), 1024, 0, ctx->stream>>>(
