@@ -51,7 +51,8 @@ __global__ __launch_bounds__(256) void k_compress(const int32_t *__restrict__ ke
 }
 
 // exclusive scan of n int32 into out[0..n] (out[n] = total), one pass; O is
-// int32 (totals below 2^31) or int64
+// int32 (totals below 2^31) or int64 (each thread then sums 16 consecutive
+// items in 64 bits)
 template <class O>
 __global__ __launch_bounds__(256) void k_scan(const int32_t *in, O *out, int64_t n, uint64_t *status,
                                               unsigned int *counter) {
@@ -62,6 +63,44 @@ __global__ __launch_bounds__(256) void k_scan(const int32_t *in, O *out, int64_t
     if (tid == 0) s_bid = (int)atomicAdd(counter, 1u);
     __syncthreads();
     const int bid = s_bid;
+    if constexpr (sizeof(O) == 4) {
+        // 32-bit totals: wave w takes the tile's items [w * 1024, (w + 1) *
+        // 1024) as 16 rows of 64 (coalesced loads and stores), each row
+        // scanned on the VALU (DPP) with the rows before it carried in
+        const int64_t wb = int64_t(bid) * TILE_N + int64_t(wid) * (TILE_N / 4);
+        uint32_t v[ITEMS], ex[ITEMS];
+        uint32_t carry = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t e = wb + k * WAVE + lane;
+            v[k] = e < n ? uint32_t(in[e]) : 0u;
+            const uint32_t incl = wave_incl_sum_dpp(v[k]);
+            ex[k] = carry + incl - v[k];
+            carry += uint32_t(wave_last(int(incl)));
+        }
+        if (lane == 0) s_w[wid] = carry;
+        __syncthreads();
+        uint64_t wpre = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            if (w < wid) wpre += s_w[w];
+            tot += s_w[w];
+        }
+        if (wid == 0) {
+            const uint64_t p = lookback_exclusive(status, bid, tot, lane);
+            if (lane == 0) s_pre = p;
+        }
+        __syncthreads();
+        const uint32_t base = uint32_t(s_pre + wpre);
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int64_t e = wb + k * WAVE + lane;
+            if (e < n) out[e] = O(base + ex[k]);
+            if (e == n - 1) out[n] = O(base + ex[k] + v[k]);
+        }
+        if (n == 0 && bid == 0 && tid == 0) out[0] = 0;
+        return;
+    }
     const int64_t b0 = int64_t(bid) * TILE_N + int64_t(tid) * ITEMS;
     uint32_t v[ITEMS];
     uint64_t sum = 0;
