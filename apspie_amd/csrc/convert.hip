@@ -145,22 +145,42 @@ __global__ __launch_bounds__(256) void k_seed(const int32_t *__restrict__ col, i
     }
 }
 
-// hist[d * ntiles + tile] = items of the tile with digit d
+// hist[d * ntiles + tile] = items of the tile with digit d.  Each wave counts
+// its quarter of the tile into its own LDS row (a quarter of the atomics
+// per counter), full tiles read 16-byte vectors (the key arrays are
+// scratch-pool slots, so tiles of 4096 keys start 16-byte aligned).
 __global__ __launch_bounds__(T) void k_hist(const int32_t *__restrict__ key, int64_t n, int shift, int ntiles,
                                             int32_t *hist) {
-    __shared__ int32_t s_h[RADIX];
-    const int tid = threadIdx.x;
+    constexpr int NW = T / WAVE;
+    __shared__ int32_t s_h[NW][RADIX];
+    const int tid = threadIdx.x, wid = tid / WAVE;
+    auto count = [&](int k) { atomicAdd(&s_h[wid][(uint32_t(k) >> shift) & (RADIX - 1)], 1); };
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        s_h[tid] = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_h[w][tid] = 0;
         __syncthreads();
         const int64_t t0 = int64_t(tile) * TILE_N;
+        if (t0 + TILE_N <= n) {
+            const int4 *kv = reinterpret_cast<const int4 *>(key + t0);
 #pragma unroll
-        for (int j = 0; j < ITEMS; ++j) {
-            const int64_t t = t0 + j * T + tid;
-            if (t < n) atomicAdd(&s_h[(uint32_t(key[t]) >> shift) & (RADIX - 1)], 1);
+            for (int j = 0; j < ITEMS / 4; ++j) {
+                const int4 q = kv[j * T + tid];
+                count(q.x);
+                count(q.y);
+                count(q.z);
+                count(q.w);
+            }
+        } else {
+            for (int j = 0; j < ITEMS; ++j) {
+                const int64_t t = t0 + j * T + tid;
+                if (t < n) count(key[t]);
+            }
         }
         __syncthreads();
-        hist[int64_t(tid) * ntiles + tile] = s_h[tid];
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) c += s_h[w][tid];
+        hist[int64_t(tid) * ntiles + tile] = c;
         __syncthreads();
     }
 }
