@@ -815,10 +815,12 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
 }
 
 // Sort keys of the row pass: the list row of product perm[s] (the last r
-// with poff[r] <= perm[s], searched in LDS when the offsets fit), or nrows
+// with poff[r] <= perm[s]: brow[b] is that row for product b * 2048, the
+// offsets are read from LDS when they fit), or nrows
 // for a product outside B's columns (column key B.n), which sorts after
 // every row and is dropped.
 __global__ __launch_bounds__(256) void k_long_rowkey(int64_t n, int nrows, int ncol, const int32_t *__restrict__ poff,
+                                                     const int32_t *__restrict__ brow,
                                                      const int32_t *__restrict__ ckey,
                                                      const int32_t *__restrict__ perm, int32_t *key, int32_t *idx) {
     constexpr int MAXL = 16384;
@@ -832,12 +834,10 @@ __global__ __launch_bounds__(256) void k_long_rowkey(int64_t n, int nrows, int n
         const int p = perm[s];
         int r = nrows;
         if (ckey[s] != ncol) {
-            int lo = 0, hi = nrows - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= p) lo = mid;
-                else hi = mid - 1;
-            }
+            // the row holding product (p >> 11) << 11, then forward: every
+            // row holds over 2048 products, so at most one step
+            int lo = brow[p >> 11];
+            while (lo + 1 < nrows && off[lo + 1] <= p) ++lo;
             r = lo;
         }
         key[s] = r;
@@ -931,6 +931,7 @@ struct LongRows {
     const int32_t *rows = nullptr;
     int32_t *rstart = nullptr, *flag = nullptr, *runpos = nullptr, *srow = nullptr, *perm = nullptr, *pcol = nullptr;
     double *pval = nullptr;
+    std::vector<int32_t> hpo, hbrow;  // host sources of the offset tables, kept until the stream syncs
 };
 
 // Symbolic phase of the long rows: products, sorts, runs, nnz(C(i,:)).
@@ -939,9 +940,17 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
                        const CsrD &B, int32_t *nnzc) {
     LongRows L;
     const int ng = int(gub.size());
-    std::vector<int32_t> po(size_t(ng) + 1, 0);
+    std::vector<int32_t> &po = L.hpo;
+    po.assign(size_t(ng) + 1, 0);
     for (int r = 0; r < ng; ++r) po[r + 1] = po[r] + gub[r];
     const int64_t n = po[ng];
+    // brow[b] = the row holding product b * 2048
+    std::vector<int32_t> &brow = L.hbrow;
+    brow.resize(size_t(n >> 11) + 1);
+    for (size_t b = 0, r = 0; b < brow.size(); ++b) {
+        while (r + 1 < size_t(ng) && po[r + 1] <= (int64_t(b) << 11)) ++r;
+        brow[b] = int32_t(r);
+    }
     L.nrows = ng;
     L.n = n;
     L.rows = rows;
@@ -952,7 +961,9 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     int32_t *key[2] = {(int32_t *)S(4, (n + 1) * 4), (int32_t *)S(5, (n + 1) * 4)};
     int32_t *idx[2] = {(int32_t *)S(6, (n + 1) * 4), (int32_t *)S(7, (n + 1) * 4)};
     L.rstart = (int32_t *)S(8, (int64_t(ng) + 2) * 4);
+    int32_t *dbrow = (int32_t *)S(9, int64_t(brow.size()) * 4);
     APSPIE_HIP(hipMemcpyAsync(poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    APSPIE_HIP(hipMemcpyAsync(dbrow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
                           ctx->stream>>>(rows, ng, A, B, poff, L.pcol, L.pval, key[0], idx[0]);
     APSPIE_HIP(hipGetLastError());
@@ -961,7 +972,7 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     // the row keys and the permutation go to the other buffer pair, where
     // the row pass starts
     // (each workgroup stages the row offsets in LDS once: a few per CU)
-    k_long_rowkey<<<std::min(g, unsigned(ctx->cu_count) * 4), 256, 0, ctx->stream>>>(n, ng, B.n, poff, key[c1],
+    k_long_rowkey<<<std::min(g, unsigned(ctx->cu_count) * 4), 256, 0, ctx->stream>>>(n, ng, B.n, poff, dbrow, key[c1],
                                                                                      idx[c1], key[c1 ^ 1],
                                                                                      idx[c1 ^ 1]);
     const int c2 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(ng) + 1), c1 ^ 1);
