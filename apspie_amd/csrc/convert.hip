@@ -137,12 +137,9 @@ __global__ __launch_bounds__(256) void k_zero64(uint64_t *p, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) p[i] = 0;
 }
 
-// key[e] = col[e], idx[e] = e
-__global__ __launch_bounds__(256) void k_seed(const int32_t *__restrict__ col, int64_t n, int32_t *key, int32_t *idx) {
-    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += int64_t(gridDim.x) * 256) {
-        key[e] = col[e];
-        idx[e] = int32_t(e);
-    }
+// key[e] = col[e] (the first sort pass takes idx[e] = e itself)
+__global__ __launch_bounds__(256) void k_seed(const int32_t *__restrict__ col, int64_t n, int32_t *key) {
+    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < n; e += int64_t(gridDim.x) * 256) key[e] = col[e];
 }
 
 // hist[d * ntiles + tile] = items of the tile with digit d.  Each wave counts
@@ -198,8 +195,8 @@ __global__ __launch_bounds__(T) void k_hist(const int32_t *__restrict__ key, int
 // tile goes out as one contiguous store instead of one line per lane.
 // Equal digits keep their input order.
 __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, const int32_t *__restrict__ idx,
-                                               int64_t n, int shift, int ntiles, const int32_t *__restrict__ off,
-                                               int32_t *key_out, int32_t *idx_out) {
+                                               bool ident, int64_t n, int shift, int ntiles,
+                                               const int32_t *__restrict__ off, int32_t *key_out, int32_t *idx_out) {
     constexpr int NW = T / WAVE, PER = TILE_N / NW, R = PER / WAVE;
     static_assert(RADIX == T, "one thread per digit");
     __shared__ int32_t s_cnt[NW][RADIX];
@@ -220,7 +217,7 @@ __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, 
         for (int r = 0; r < R; ++r) {
             const int64_t t = wb + r * WAVE + lane;
             k[r] = t < n ? key[t] : -1;
-            x[r] = t < n ? idx[t] : 0;
+            x[r] = t < n ? (ident ? int32_t(t) : idx[t]) : 0;  // ident: the items' own positions
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -320,8 +317,9 @@ void scan_to(apspie_ctx *ctx, const int32_t *in, O *out, int64_t n) {
 void scan32(apspie_ctx *ctx, const int32_t *in, int32_t *out, int64_t n) { scan_to(ctx, in, out, n); }
 void scan_wide(apspie_ctx *ctx, const int32_t *in, int64_t *out, int64_t n) { scan_to(ctx, in, out, n); }
 
-int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start) {
+int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start, bool ident) {
     if (n == 0) return start;
+    if (ident && bits == 0) bits = 1;  // one pass writes the identity the caller reads
     const int ntiles = (int)ceil_div(n, TILE_N);
     const int64_t nh = int64_t(RADIX) * ntiles;
     int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
@@ -331,7 +329,8 @@ int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, in
     for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
         k_hist<<<gt, T, 0, ctx->stream>>>(key[cur], n, shift, ntiles, hist);
         scan32(ctx, hist, offs, nh);
-        k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], n, shift, ntiles, offs, key[cur ^ 1], idx[cur ^ 1]);
+        k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], ident && shift == 0, n, shift, ntiles, offs,
+                                             key[cur ^ 1], idx[cur ^ 1]);
     }
     APSPIE_HIP(hipGetLastError());
     return cur;
@@ -346,10 +345,10 @@ void csr2csc_into(apspie_ctx *ctx, const apspie_csr *a, int32_t *d_col_ptr, int3
     }
     int32_t *key[2] = {(int32_t *)ctx->slot(S_AUX0, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX1, size_t(n) * 4)};
     int32_t *idx[2] = {(int32_t *)ctx->slot(S_AUX2, size_t(n) * 4), (int32_t *)ctx->slot(S_AUX3, size_t(n) * 4)};
-    k_seed<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(a->col_ind, n, key[0], idx[0]);
+    k_seed<<<grid_for(ctx, n, 256), 256, 0, ctx->stream>>>(a->col_ind, n, key[0]);
     int bits = 1;
     while (bits < 31 && (int64_t(1) << bits) < int64_t(a->n)) ++bits;
-    const int cur = radix_pairs(ctx, key, idx, n, bits);
+    const int cur = radix_pairs(ctx, key, idx, n, bits, 0, true);
     compress(ctx, key[cur], n, a->n, d_col_ptr);
     int32_t *rowid = key[cur ^ 1];  // free after the last pass
     k_row_ids<<<grid_for(ctx, int64_t(a->m) * WAVE, 256), 256, 0, ctx->stream>>>(a->m, a->row_ptr, rowid);

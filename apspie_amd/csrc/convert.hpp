@@ -27,9 +27,11 @@ void scan_wide(apspie_ctx *ctx, const int32_t *in, int64_t *out, int64_t n);
 // Stable least-significant-digit radix sort of the pairs (key[start][e],
 // idx[start][e]) by the low `bits` bits of the key (non-negative keys), 8
 // bits a pass, ping-ponging between the two buffers of each array.  Returns
-// which buffer (0 or 1) holds the result.  Uses the S_AUX4, S_AUX5 and
-// S_STATUS slots.
-int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start = 0);
+// which buffer (0 or 1) holds the result.  ident: idx[start] is not read,
+// the first pass takes each item's position (and at least one pass runs).
+// Uses the S_AUX4, S_AUX5 and S_STATUS slots.
+int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, int bits, int start = 0,
+                bool ident = false);
 
 // The CSC of a (rows ascending in each column) into caller memory; cv may be
 // null.  Uses the S_AUX0..S_AUX5 and S_STATUS slots.

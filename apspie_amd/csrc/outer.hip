@@ -95,8 +95,7 @@ __global__ __launch_bounds__(256) void k_temit(int ncol, const int32_t *__restri
                                                const int32_t *__restrict__ csB, const int32_t *__restrict__ tpos,
                                                const int32_t *__restrict__ ssA, const int32_t *__restrict__ riA,
                                                const int32_t *__restrict__ ssB, const int32_t *__restrict__ riB,
-                                               int P, int nsb, int32_t *tA, int32_t *tB, int32_t *tkey,
-                                               int32_t *tidx) {
+                                               int P, int nsb, int32_t *tA, int32_t *tB, int32_t *tkey) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int64_t nw = int64_t(gridDim.x) * (256 / WAVE);
     for (int64_t k = int64_t(blockIdx.x) * (256 / WAVE) + threadIdx.x / WAVE; k < ncol; k += nw) {
@@ -108,7 +107,6 @@ __global__ __launch_bounds__(256) void k_temit(int ncol, const int32_t *__restri
             tA[t] = sa;
             tB[t] = sb;
             tkey[t] = (riA[ssA[sa]] / P) * nsb + riB[ssB[sb]] / P;
-            tidx[t] = t;
         }
     }
 }
@@ -151,7 +149,7 @@ __global__ __launch_bounds__(256) void k_products(int64_t ptot, int64_t ntr, int
                                                   const int32_t *__restrict__ ssB, const int32_t *__restrict__ riA,
                                                   const float *__restrict__ cvA, const int32_t *__restrict__ riB,
                                                   const float *__restrict__ cvB, int32_t *prow, int32_t *pcol,
-                                                  double *pval, int32_t *kcol, int32_t *iota) {
+                                                  double *pval, int32_t *kcol) {
     __shared__ int32_t s_off[TS + 2];
     __shared__ int s_f0, s_n;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -185,7 +183,6 @@ __global__ __launch_bounds__(256) void k_products(int64_t ptot, int64_t ntr, int
             prow[s] = r;
             pcol[s] = c;
             kcol[s] = c;
-            iota[s] = int32_t(s);
             pval[s] = double(cvA ? cvA[ea] : 1.0f) * double(cvB ? cvB[eb] : 1.0f);
         }
         __syncthreads();
@@ -309,9 +306,9 @@ extern "C" int apspie_spgemm_outer(apspie_ctx *ctx, const apspie_csr *a, const a
                 int32_t *tidx[2] = {(int32_t *)S(27, ntr * 4), (int32_t *)S(28, ntr * 4)};
                 k_temit<<<grid(int64_t(ncol) * WAVE, 256), 256, 0, ctx->stream>>>(
                     ncol, D[0].colseg, D[1].colseg, tpos, D[0].seg_start, D[0].ri, D[1].seg_start, D[1].ri, P,
-                    int(nsb), tA, tB, tkey[0], tidx[0]);
+                    int(nsb), tA, tB, tkey[0]);
                 APSPIE_HIP(hipGetLastError());
-                const int tc = cvt::radix_pairs(ctx, tkey, tidx, ntr, bits_for(nsa * nsb));
+                const int tc = cvt::radix_pairs(ctx, tkey, tidx, ntr, bits_for(nsa * nsb), 0, true);
                 // the sorted triples go to the free pair of buffers; the
                 // product counts over the sorted keys (no longer needed)
                 int32_t *oA = tkey[tc ^ 1], *oB = tidx[tc ^ 1], *pcnt = tkey[tc];
@@ -334,10 +331,10 @@ extern "C" int apspie_spgemm_outer(apspie_ctx *ctx, const apspie_csr *a, const a
                 k_ptiles<<<grid(ntr, 256), 256, 0, ctx->stream>>>(ntr, poff, table);
                 k_products<<<std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8)), 256, 0,
                              ctx->stream>>>(ptot, ntr, ntiles, table, poff, oA, oB, D[0].seg_start, D[1].seg_start,
-                                            D[0].ri, D[0].cv, D[1].ri, D[1].cv, prow, pcol, pval, key[0], idx[0]);
+                                            D[0].ri, D[0].cv, D[1].ri, D[1].cv, prow, pcol, pval, key[0]);
                 APSPIE_HIP(hipGetLastError());
                 // 5. sort by (row, column): by column, then stably by row
-                const int c1 = cvt::radix_pairs(ctx, key, idx, ptot, bits_for(p));
+                const int c1 = cvt::radix_pairs(ctx, key, idx, ptot, bits_for(p), 0, true);
                 k_rows_by<<<grid(ptot, 256), 256, 0, ctx->stream>>>(ptot, idx[c1], prow, key[0], idx[0]);
                 const int c2 = cvt::radix_pairs(ctx, key, idx, ptot, bits_for(m));
                 const int32_t *rows = key[c2], *perm = idx[c2];

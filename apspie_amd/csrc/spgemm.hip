@@ -755,14 +755,14 @@ __global__ __launch_bounds__(256) void k_rows_wave(const int32_t *__restrict__ r
 // global hash tables (whose inserts are device-scope atomics, a few G/s):
 // row r of the list writes its products at poff[r] in the order of A's
 // entries, then of B's row: (column, a * b in FP64); key = column (B.n for a
-// column outside B, dropped later as the hash tables skip it), idx =
-// position.  Stable sorts by column and then by row leave each output
-// entry's products in increasing k, so the FP64 run sums are the tables' and
-// Gustavson's.
+// column outside B, dropped later as the hash tables skip it); the first
+// sort pass takes each product's position as its index.  Stable sorts by
+// column and then by row leave each output entry's products in increasing k,
+// so the FP64 run sums are the tables' and Gustavson's.
 template <int T>
 __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
                                                    const int32_t *__restrict__ poff, int32_t *pcol, double *pval,
-                                                   int32_t *key, int32_t *idx) {
+                                                   int32_t *key) {
     __shared__ int32_t s_bs[T];
     __shared__ int32_t s_pre[T];
     __shared__ float s_a[T];
@@ -805,7 +805,6 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
                 const int o = base + t;
                 pcol[o] = ok ? c : 0;
                 key[o] = ok ? c : B.n;
-                idx[o] = o;
                 pval[o] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
             }
             base += tot;
@@ -965,9 +964,9 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     APSPIE_HIP(hipMemcpyAsync(poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
     APSPIE_HIP(hipMemcpyAsync(dbrow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
-                          ctx->stream>>>(rows, ng, A, B, poff, L.pcol, L.pval, key[0], idx[0]);
+                          ctx->stream>>>(rows, ng, A, B, poff, L.pcol, L.pval, key[0]);
     APSPIE_HIP(hipGetLastError());
-    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(B.n) + 1));
+    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(B.n) + 1), 0, true);
     const unsigned g = cvt::grid_for(ctx, n, 256);
     // the row keys and the permutation go to the other buffer pair, where
     // the row pass starts
