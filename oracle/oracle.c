@@ -503,10 +503,12 @@ int orc_spgemm(const orc_csr *a, const orc_csr *b, orc_csr *c, int64_t *products
         int32_t cnt = 0;
         for (int32_t p = a->row_ptr[i]; p < a->row_ptr[i + 1]; ++p) {
             int32_t k = a->col_ind[p];
+            if (k < 0 || k >= b->m) continue;  /* a column of A outside B's rows: no products */
             double av = a->val ? a->val[p] : 1.0;
             prod += b->row_ptr[k + 1] - b->row_ptr[k];
             for (int32_t q = b->row_ptr[k]; q < b->row_ptr[k + 1]; ++q) {
                 int32_t j = b->col_ind[q];
+                if (j < 0 || j >= n) continue; /* a column outside B: dropped, as the product drops it */
                 double bv = b->val ? b->val[q] : 1.0;
                 if (mark[j] != i) {
                     mark[j] = i;

@@ -148,6 +148,54 @@ def _rows_of(rng, lens, draw):
     return rp, col, val
 
 
+def test_spgemm_register_bins_exact(ctx):
+    """Rows of at most 64 products (the register bins: four rows a wave up to
+    16 products, a wave per row up to 64) at every sorting-stage boundary
+    (1, 2, 15, 16, 17, 31, 32, 33, 63, 64 products), with duplicate columns
+    (runs of several terms), columns outside B (dropped), and A rows of over
+    64 entries most of whose B rows are empty (several chunks per row).
+    These rows sum each entry in FP64 in k order, as the oracle does, so C is
+    identical to the oracle's, values included."""
+    rng = np.random.default_rng(23)
+    n, mb = 40, 3000
+    blen = rng.choice([0, 0, 1, 2, 3, 4], mb)
+    brp = np.zeros(mb + 1, np.int32)
+    brp[1:] = np.cumsum(blen)
+    bcol = np.concatenate([np.sort(rng.choice(n + 3, l, replace=False)) for l in blen]).astype(np.int32)
+    bval = rng.uniform(-2, 2, bcol.size).astype(np.float32)
+    hb = A.HostCsr(mb, n, brp, bcol, bval)  # columns n .. n+2 lie outside B
+    full = np.nonzero(blen > 0)[0]
+    empty = np.nonzero(blen == 0)[0]
+    rows = []
+    for target in [1, 2, 15, 16, 17, 31, 32, 33, 63, 64] * 6:
+        ks, tot = [], 0
+        for k in rng.permutation(full):
+            if tot + blen[k] <= target:
+                ks.append(k)
+                tot += blen[k]
+            if tot == target:
+                break
+        if rng.random() < 0.3:  # pad past 64 entries with empty B rows
+            ks += list(rng.choice(empty, 70, replace=False))
+        rows.append(np.sort(np.array(ks, np.int32)))
+    arp = np.zeros(len(rows) + 1, np.int32)
+    arp[1:] = np.cumsum([r.size for r in rows])
+    acol = np.concatenate(rows).astype(np.int32)
+    ha = A.HostCsr(len(rows), mb, arp, acol, rng.uniform(-2, 2, acol.size).astype(np.float32))
+    ub = long_row_products(ha, hb)
+    assert ub.max() <= 64 and {16, 17, 64} <= set(ub.tolist())
+    # the oracle takes columns inside B only: its B drops the others, as
+    # the product does
+    keep = bcol < n
+    krp = np.zeros(mb + 1, np.int32)
+    krp[1:] = np.cumsum(np.bincount(np.repeat(np.arange(mb), blen)[keep], minlength=mb))
+    want, _ = O.spgemm(ocsr(ha), O.Csr(mb, n, krp, bcol[keep], bval[keep]))
+    c, st = ctx.spgemm(ctx.upload(ha), ctx.upload(hb))
+    assert st.edges == ub.sum()  # products counted before the columns outside B drop
+    assert np.array_equal(c.row_ptr, want.row_ptr) and np.array_equal(c.col_ind, want.col_ind)
+    assert np.array_equal(c.val, want.val)
+
+
 def skewed_long_rows():
     """A * B whose rows are all over 2048 products: rows over uniform B rows
     (up to 21000 products), rows whose B rows crowd a band of 300 low columns,
