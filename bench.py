@@ -399,8 +399,9 @@ def main():
                     help="run only the 1 GiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is "
-              "described in DESIGN.md and not run here", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): sharding the traversals and "
+              "products over several GPUs is designed but not built, so no multi-GPU figure is run",
+              file=sys.stderr)
         sys.exit(2)
     if args.copy_only:
         with A.Context(0) as c:
