@@ -108,7 +108,8 @@ struct PickSssp {
             dist[v] = 0u;
         }
         payload = d;
-        const bool isnew = d < x.p && d < threshold();
+        // near/far off: the threshold stays ~0u, so its load is skipped
+        const bool isnew = d < x.p && (!(delta > 0.0f) || d < threshold());
         if (isnew) prev[v] = d;
         if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
         return isnew;
