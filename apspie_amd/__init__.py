@@ -146,6 +146,9 @@ def lib() -> C.CDLL:
         "apspie_mxv": [vp, P(_DCsr), C.c_int, C.c_int, vp, vp, P(Stats)],
         "apspie_mxv_sparse": [vp, P(_DCsr), C.c_int, vp, vp, i32, vp, vp, P(i32), P(Stats)],
         "apspie_csr_load": [C.c_char_p, P(_Csr)],
+        "apspie_host_coo_make_symmetric": [i64, vp, vp, vp, P(i64)],
+        "apspie_host_coo_to_csr": [i32, i64, vp, vp, vp, vp, vp, vp],
+        "apspie_rowptr_from_counts": [vp, vp, i32, vp, P(i64)],
         "apspie_host_csr_free": [P(_Csr)],
         "apspie_rmat": [C.c_int, C.c_int, u32, u32, u32, P(_Csr)],
         "apspie_ctx_set_tuning": [vp, P(Tuning)],
@@ -275,6 +278,39 @@ def rmat(scale: int, edge_factor: int = 16, seed: int = 1, undirected: bool = Fa
     flags = (RMAT_UNDIRECTED if undirected else 0) | (RMAT_WEIGHTED if weight_seed is not None else 0)
     _check(lib().apspie_rmat(scale, edge_factor, seed, flags, weight_seed or 0, C.byref(s)), None, "rmat")
     return _host_from_c(s)
+
+
+def coo_make_symmetric(col: np.ndarray, row: np.ndarray, val: Optional[np.ndarray] = None):
+    """makeSymmetric (src/util.cuh:572-643): returns (col, row, val) with the
+    reversed non-loop entries added, sorted by (row, col), repeats dropped."""
+    e = len(col)
+    c = np.zeros(2 * e, np.int32)
+    r = np.zeros(2 * e, np.int32)
+    c[:e], r[:e] = col, row
+    v = None
+    if val is not None:
+        v = np.zeros(2 * e, np.float32)
+        v[:e] = val
+    out = C.c_int64()
+    _check(lib().apspie_host_coo_make_symmetric(e, c.ctypes.data, r.ctypes.data, v.ctypes.data if v is not None
+                                                else None, C.byref(out)), None, "coo_make_symmetric")
+    k = out.value
+    return c[:k].copy(), r[:k].copy(), (v[:k].copy() if v is not None else None)
+
+
+def coo_to_csr_host(m: int, row: np.ndarray, col: np.ndarray, val: Optional[np.ndarray] = None) -> HostCsr:
+    """buildMatrix (src/matrix.cpp:55-111): stable counting sort by row."""
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    nnz = len(row)
+    rp = np.empty(m + 1, np.int32)
+    ci = np.empty(max(nnz, 1), np.int32)
+    va = np.empty(max(nnz, 1), np.float32) if val is not None else None
+    cv = np.ascontiguousarray(val, dtype=np.float32) if val is not None else None
+    _check(lib().apspie_host_coo_to_csr(m, nnz, row.ctypes.data, col.ctypes.data,
+                                        cv.ctypes.data if cv is not None else None, rp.ctypes.data, ci.ctypes.data,
+                                        va.ctypes.data if va is not None else None), None, "coo_to_csr_host")
+    return HostCsr(m, m, rp, ci[:nnz], va[:nnz] if va is not None else None)
 
 
 def transpose(h: HostCsr) -> HostCsr:
@@ -648,6 +684,22 @@ class Context:
             dr.free()
         return out
 
+    def rowptr_from_counts(self, counts: np.ndarray):
+        """apspie_rowptr_from_counts: (row_ptr int32[n+1], exact total)."""
+        counts = np.ascontiguousarray(counts, dtype=np.int32)
+        n = len(counts)
+        dc = self.alloc(4 * max(n, 1)).upload(counts) if n else None
+        rp = self.alloc(4 * (n + 1))
+        tot = C.c_int64()
+        try:
+            _check(lib().apspie_rowptr_from_counts(self.h, dc.ptr if dc else None, n, rp.ptr, C.byref(tot)),
+                   self.h, "rowptr_from_counts")
+            return rp.download(np.int32, n + 1), tot.value
+        finally:
+            rp.free()
+            if dc:
+                dc.free()
+
     def tc(self, l: DeviceCsr):
         """Triangles from the strict lower triangle L; returns (int count, Stats)."""
         cnt = C.c_int64()
@@ -714,6 +766,7 @@ class Server:
         launches (0: the library default, -1: never)."""
         L = lib()
         self._graphs = list(graphs)  # keep the device CSRs alive
+        self._held = {}  # ticket -> output array the server may still write
         arr = (_DCsr * len(graphs))(*[g.c for g in graphs])
         opts = ServerOpts(lanes, max_bfs_batch, max_sssp_batch, plain_bfs_upto, plain_sssp_upto, budget_bytes)
         h = C.c_void_p()
@@ -722,22 +775,39 @@ class Server:
 
     def submit(self, kind: int, graph: int, src: int, out: Optional[np.ndarray] = None,
                arrival: float = -1.0) -> int:
+        """Queue one request; `out` (or None) receives the row: a C-contiguous,
+        writeable int32 (BFS) or float32 (SSSP) array of at least m entries.
+        The server writes it from its own threads after this call returns, so
+        the array is held here until wait(), drain() or close() releases it."""
+        if out is not None:
+            want = np.int32 if kind == REQ_BFS else np.float32
+            if not isinstance(out, np.ndarray) or out.dtype != want:
+                raise TypeError(f"out must be a numpy {np.dtype(want).name} array")
+            if not (out.flags.c_contiguous and out.flags.writeable):
+                raise ValueError("out must be C-contiguous and writeable")
+            if not 0 <= graph < len(self._graphs) or out.size < self._graphs[graph].m:
+                raise ValueError(f"out holds {out.size} entries, the graph has "
+                                 f"{self._graphs[graph].m if 0 <= graph < len(self._graphs) else '?'} vertices")
         t = C.c_uint64()
         ptr = out.ctypes.data if out is not None else None
         rc = lib().apspie_server_submit(self.h, kind, graph, src, ptr, arrival, C.byref(t))
         if rc != 0:
             raise ApspieError(rc, "server_submit")
+        if out is not None:
+            self._held[t.value] = out
         return t.value
 
     def wait(self, ticket: int) -> float:
         lat = C.c_double()
         rc = lib().apspie_server_wait(self.h, ticket, C.byref(lat))
+        self._held.pop(ticket, None)
         if rc != 0:
             raise ApspieError(rc, lib().apspie_server_last_error(self.h).decode())
         return lat.value
 
     def drain(self) -> None:
         lib().apspie_server_drain(self.h)
+        self._held.clear()
 
     def report(self) -> dict:
         r = ServerReport()
@@ -749,8 +819,9 @@ class Server:
 
     def close(self) -> None:
         if self.h:
-            lib().apspie_server_destroy(self.h)
+            lib().apspie_server_destroy(self.h)  # joins the workers: no write is left in flight
         self.h = None
+        self._held.clear()
 
     def __enter__(self):
         return self

@@ -54,6 +54,63 @@ void host_csr_alloc(apspie_host_csr *h, int32_t m, int32_t n, int64_t nnz, bool 
     }
 }
 
+// makeSymmetric (src/util.cuh:572-643) on COO arrays of capacity 2 * nnz:
+// append the reversed copy of every entry that is not a self-loop (its value
+// negated when skew), sort by (row, col), drop exact repeats (the first
+// kept, values moving with their entries).  Returns the new count.
+int64_t coo_symmetrize(int64_t nnz, int32_t *col, int32_t *row, float *val, bool skew) {
+    int64_t e = nnz;
+    for (int64_t i = 0; i < nnz; ++i)
+        if (col[i] != row[i]) {
+            row[e] = col[i];
+            col[e] = row[i];
+            if (val) val[e] = skew ? -val[i] : val[i];
+            ++e;
+        }
+    std::vector<int64_t> idx(e);
+    for (int64_t i = 0; i < e; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
+        return row[x] != row[y] ? row[x] < row[y] : col[x] < col[y];
+    });
+    std::vector<int32_t> r2, c2;
+    std::vector<float> v2;
+    r2.reserve(e);
+    c2.reserve(e);
+    if (val) v2.reserve(e);
+    for (int64_t i = 0; i < e; ++i) {
+        const int64_t t = idx[i];
+        if (!r2.empty() && r2.back() == row[t] && c2.back() == col[t]) continue;
+        r2.push_back(row[t]);
+        c2.push_back(col[t]);
+        if (val) v2.push_back(val[t]);
+    }
+    e = (int64_t)r2.size();
+    std::copy(r2.begin(), r2.end(), row);
+    std::copy(c2.begin(), c2.end(), col);
+    if (val) std::copy(v2.begin(), v2.end(), val);
+    return e;
+}
+
+// buildMatrix (src/matrix.cpp:55-111): stable counting sort of COO entries
+// by row into row_ptr[m+1] / col_ind[nnz] / val[nnz] (val and coo_val may be
+// null).  False when a row lies outside [0, m).
+bool coo_counting_sort(int32_t m, int64_t nnz, const int32_t *row, const int32_t *col, const float *coo_val,
+                       int32_t *row_ptr, int32_t *col_ind, float *val) {
+    std::vector<int64_t> cur(size_t(m) + 1, 0);
+    for (int64_t i = 0; i < nnz; ++i) {
+        if (row[i] < 0 || row[i] >= m) return false;
+        cur[row[i] + 1]++;
+    }
+    for (int32_t i = 0; i < m; ++i) cur[i + 1] += cur[i];
+    for (int32_t i = 0; i <= m; ++i) row_ptr[i] = int32_t(cur[i]);
+    for (int64_t i = 0; i < nnz; ++i) {
+        const int64_t d = cur[row[i]]++;
+        col_ind[d] = col[i];
+        if (val) val[d] = coo_val ? coo_val[i] : 1.0f;
+    }
+    return true;
+}
+
 // Sort each row's columns (and values) in place, rows split across threads.
 void sort_rows(apspie_host_csr *h) {
     parallel_ranges(h->m, [&](int64_t lo, int64_t hi) {
@@ -234,54 +291,17 @@ int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out) {
             row[j] = b - 1;
             val[j] = w;
         }
-        int64_t e = nnz;
-        if (undirected) {
-            int64_t k = nnz;
-            for (int64_t i = 0; i < nnz; ++i)
-                if (col[i] != row[i]) {
-                    row[k] = col[i];
-                    col[k] = row[i];
-                    val[k] = skew ? -val[i] : val[i];
-                    ++k;
-                }
-            e = k;
-            std::vector<int64_t> idx(e);
-            for (int64_t i = 0; i < e; ++i) idx[i] = i;
-            std::stable_sort(idx.begin(), idx.end(), [&](int64_t x, int64_t y) {
-                return row[x] != row[y] ? row[x] < row[y] : col[x] < col[y];
-            });
-            std::vector<int32_t> r2, c2;
-            std::vector<float> v2;
-            r2.reserve(e);
-            c2.reserve(e);
-            v2.reserve(e);
-            for (int64_t i = 0; i < e; ++i) {
-                const int64_t t = idx[i];
-                if (!r2.empty() && r2.back() == row[t] && c2.back() == col[t]) continue;
-                r2.push_back(row[t]);
-                c2.push_back(col[t]);
-                v2.push_back(val[t]);
-            }
-            e = (int64_t)r2.size();
-            row.swap(r2);
-            col.swap(c2);
-            val.swap(v2);
-        }
+        const int64_t e = undirected ? coo_symmetrize(nnz, col.data(), row.data(), val.data(), skew) : nnz;
         host_csr_alloc(out, m, n, e, weighted);
-        std::vector<int32_t> cur(size_t(m) + 1, 0);
-        for (int64_t i = 0; i < e; ++i) {
-            if (row[i] < 0 || row[i] >= m || col[i] < 0 || col[i] >= n) {
+        for (int64_t i = 0; i < e; ++i)
+            if (col[i] < 0 || col[i] >= n) {
                 apspie_host_csr_free(out);
                 throw Error(APSPIE_EIO, "mtx index out of range");
             }
-            cur[row[i] + 1]++;
-        }
-        for (int i = 0; i < m; ++i) cur[i + 1] += cur[i];
-        memcpy(out->row_ptr, cur.data(), (size_t(m) + 1) * 4);
-        for (int64_t i = 0; i < e; ++i) {
-            const int32_t d = cur[row[i]]++;
-            out->col_ind[d] = col[i];
-            if (weighted) out->val[d] = val[i];
+        if (!coo_counting_sort(m, e, row.data(), col.data(), weighted ? val.data() : nullptr, out->row_ptr,
+                               out->col_ind, out->val)) {
+            apspie_host_csr_free(out);
+            throw Error(APSPIE_EIO, "mtx index out of range");
         }
         if (flags & APSPIE_LOAD_SORT_COLUMNS) sort_rows(out);
     });
@@ -493,6 +513,21 @@ int apspie_csr_load(const char *path, apspie_host_csr *out) {
             apspie_host_csr_free(out);
             throw Error(APSPIE_EIO, std::string("truncated or corrupt CSR cache file: ") + path);
         }
+    });
+}
+
+int apspie_host_coo_make_symmetric(int64_t edge, int32_t *coo_col, int32_t *coo_row, float *coo_val,
+                                   int64_t *out_edge) {
+    if (!out_edge || edge < 0 || (edge > 0 && (!coo_col || !coo_row))) return APSPIE_EINVAL;
+    return guarded(nullptr, [&] { *out_edge = edge ? coo_symmetrize(edge, coo_col, coo_row, coo_val, false) : 0; });
+}
+
+int apspie_host_coo_to_csr(int32_t m, int64_t nnz, const int32_t *coo_row, const int32_t *coo_col,
+                           const float *coo_val, int32_t *row_ptr, int32_t *col_ind, float *val) {
+    if (m < 0 || nnz < 0 || !row_ptr || (nnz > 0 && (!coo_row || !coo_col || !col_ind))) return APSPIE_EINVAL;
+    return guarded(nullptr, [&] {
+        APSPIE_CHECK(coo_counting_sort(m, nnz, coo_row, coo_col, coo_val, row_ptr, col_ind, val), APSPIE_EINVAL,
+                     "COO row index outside [0, m)");
     });
 }
 

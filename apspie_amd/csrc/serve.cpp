@@ -72,8 +72,11 @@ struct apspie_server {
     std::map<uint64_t, std::unique_ptr<Request>> all;
     uint64_t next_id = 1;
     size_t used = 0;
+    uint64_t released = 0;  // bumped whenever a batch returns its memory to the budget
     bool stop = false;
-    int inflight = 0;
+    int inflight = 0;  // worker passes that took requests and have not completed them all
+    std::vector<apspie_ctx *> ctxs;  // one per lane, created with the server
+    std::vector<void *> d_srcs;      // per lane: the batch's source ids (256 int32)
     std::vector<std::thread> workers;
     std::vector<double> lat_ms;
     double first_arrive = 0, last_done = 0;
@@ -122,15 +125,9 @@ void apspie_server::answer_trivial(std::vector<Request *> &rs) {
 }
 
 void apspie_server::worker(int lane) {
-    apspie_ctx *ctx = nullptr;
-    if (apspie_ctx_create(device, &ctx) != APSPIE_OK) {
-        std::lock_guard<std::mutex> lk(mu);
-        err = "lane " + std::to_string(lane) + ": context creation failed";
-        return;
-    }
-    void *d_srcs = nullptr, *d_out = nullptr;
+    apspie_ctx *ctx = ctxs[size_t(lane)];
+    void *d_src = d_srcs[size_t(lane)], *d_out = nullptr;
     size_t out_cap = 0;
-    apspie_malloc(ctx, &d_srcs, 256 * 4);
     std::vector<Request *> batch, easy, cand;
     std::vector<int32_t> srcs;
     for (;;) {
@@ -186,14 +183,23 @@ void apspie_server::worker(int lane) {
                 pending.erase(std::remove_if(pending.begin(), pending.end(), [](const Request *r) { return r->picked; }),
                               pending.end());
             if (batch.empty() && easy.empty()) {
-                // budget full: wait for another lane to release memory
-                cv_work.wait(lk, [&] { return stop || inflight == 0 || used == 0; });
+                // budget full: wait until some lane returns memory (or none
+                // holds any, when the oldest request fits by itself)
+                const uint64_t seen = released;
+                cv_work.wait(lk, [&] { return stop || released != seen || used == 0; });
                 continue;
             }
-            if (!batch.empty()) inflight++;
+            // counted until every request taken here is complete, so drain
+            // cannot return while their rows are still being written
+            inflight++;
         }
         if (!easy.empty()) answer_trivial(easy);
         if (batch.empty()) {
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                inflight--;
+            }
+            cv_done.notify_all();
             cv_work.notify_all();
             continue;
         }
@@ -202,7 +208,7 @@ void apspie_server::worker(int lane) {
         const int k = (int)batch.size();
         srcs.resize(k);
         for (int i = 0; i < k; ++i) srcs[i] = batch[i]->src;
-        int rc = apspie_memcpy_h2d(ctx, d_srcs, srcs.data(), size_t(k) * 4);
+        int rc = apspie_memcpy_h2d(ctx, d_src, srcs.data(), size_t(k) * 4);
         const size_t need = size_t(k) * size_t(g.m) * 4;
         if (rc == APSPIE_OK && need > out_cap) {
             if (d_out) apspie_free(ctx, d_out);
@@ -221,8 +227,8 @@ void apspie_server::worker(int lane) {
             }
         } else if (rc == APSPIE_OK) {
             apspie_stats st;
-            rc = kind == 0 ? apspie_bfs_batch(ctx, &g, (const int32_t *)d_srcs, k, (int32_t *)d_out, &st)
-                           : apspie_sssp_batch(ctx, &g, (const int32_t *)d_srcs, k, (float *)d_out, &st);
+            rc = kind == 0 ? apspie_bfs_batch(ctx, &g, (const int32_t *)d_src, k, (int32_t *)d_out, &st)
+                           : apspie_sssp_batch(ctx, &g, (const int32_t *)d_src, k, (float *)d_out, &st);
         }
         for (int i = 0; i < k && rc == APSPIE_OK; ++i)
             if (batch[i]->out)
@@ -245,14 +251,13 @@ void apspie_server::worker(int lane) {
             if (rc != APSPIE_OK) err = apspie_last_error(ctx);
             batches++;
             batched_reqs += k;
+            released++;
             inflight--;
         }
         cv_done.notify_all();
         cv_work.notify_all();
     }
     if (d_out) apspie_free(ctx, d_out);
-    if (d_srcs) apspie_free(ctx, d_srcs);
-    apspie_ctx_destroy(ctx);
 }
 
 extern "C" {
@@ -307,6 +312,23 @@ int apspie_server_create(int device, const apspie_csr *graphs, int ngraphs, cons
             dc[size_t(v)] = uint8_t(d ? 32 - __builtin_clz(d) : 0);
         }
         s->dclass.push_back(std::move(dc));
+    }
+    // every lane's context and source buffer exist before any request is
+    // taken, so a server that starts can serve
+    for (int l = 0; l < o.lanes; ++l) {
+        apspie_ctx *c = nullptr;
+        void *d = nullptr;
+        int rc = apspie_ctx_create(device, &c);
+        if (rc == APSPIE_OK) {
+            rc = apspie_malloc(c, &d, 256 * 4);
+            if (rc != APSPIE_OK) apspie_ctx_destroy(c);
+        }
+        if (rc != APSPIE_OK) {
+            apspie_server_destroy(s);
+            return rc;
+        }
+        s->ctxs.push_back(c);
+        s->d_srcs.push_back(d);
     }
     for (int l = 0; l < o.lanes; ++l) s->workers.emplace_back([s, l] { s->worker(l); });
     *out = s;
@@ -410,6 +432,10 @@ int apspie_server_destroy(apspie_server *s) {
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
+    for (size_t l = 0; l < s->ctxs.size(); ++l) {
+        apspie_free(s->ctxs[l], s->d_srcs[l]);
+        apspie_ctx_destroy(s->ctxs[l]);
+    }
     delete s;
     return APSPIE_OK;
 }

@@ -157,9 +157,11 @@ __global__ __launch_bounds__(256) void k_gather(const int32_t *__restrict__ idx,
     for (int r = blockIdx.x * 256 + threadIdx.x; r < n; r += gridDim.x * 256) out[r] = x[idx[r]];
 }
 
-// exclusive scan of n int32 into out[0..n], out[n] = total (single pass)
+// exclusive scan of n non-negative int32 into out[0..n], out[n] = total
+// (single pass, 64-bit sums); the exact total also goes to *total64, so the
+// host can refuse a total that out[] cannot hold
 __global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, int n, uint64_t *status,
-                                              unsigned int *counter) {
+                                              unsigned int *counter, unsigned long long *total64) {
     constexpr int ITEMS = 16, TILE_N = 256 * ITEMS;
     __shared__ int s_bid;
     __shared__ uint64_t s_w[4];
@@ -195,9 +197,15 @@ __global__ __launch_bounds__(256) void k_scan(const int32_t *in, int32_t *out, i
     for (int k = 0; k < ITEMS; ++k) {
         if (b0 + k < n) out[b0 + k] = int32_t(run);
         run += uint64_t(uint32_t(v[k]));
-        if (b0 + k == n - 1) out[n] = int32_t(run);
+        if (b0 + k == n - 1) {
+            out[n] = int32_t(run);
+            *total64 = run;
+        }
     }
-    if (n == 0 && bid == 0 && tid == 0) out[0] = 0;
+    if (n == 0 && bid == 0 && tid == 0) {
+        out[0] = 0;
+        *total64 = 0;
+    }
 }
 
 template <int T>
@@ -1038,10 +1046,48 @@ void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, i
     APSPIE_HIP(hipGetLastError());
 }
 
+// row_ptr[0..n] = exclusive scan of n non-negative counts; returns the total.
+// A total at or past the CSR limit (2^31 - 4096 entries, what every entry
+// point accepts) runs on_fail and throws APSPIE_ELIMIT: row_ptr then holds
+// wrapped offsets, and nothing may be allocated or written by them.
+// Synchronises the stream.
+template <class F>
+int64_t rowptr_scan(apspie_ctx *ctx, const int32_t *counts, int n, int32_t *row_ptr, F &&on_fail) {
+    const int nb = std::max(1, (int)ceil_div(n, 256 * 16));
+    uint64_t *st = (uint64_t *)ctx->slot(S_STATUS, size_t(nb) * 8 + 64);
+    unsigned *ctr = (unsigned *)(st + nb);
+    unsigned long long *tot = (unsigned long long *)(st + nb + 1);
+    k_zero_i32<<<std::max(1u, std::min(ceil_div(nb * 2 + 4, 256), 1024u)), 256, 0, ctx->stream>>>(
+        (int32_t *)st, int64_t(nb) * 2 + 4);
+    k_scan<<<nb, 256, 0, ctx->stream>>>(counts, row_ptr, n, st, ctr, tot);
+    APSPIE_HIP(hipGetLastError());
+    unsigned long long total = 0;
+    APSPIE_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, ctx->stream));
+    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    if (total >= (1ull << 31) - 4096) {
+        on_fail();
+        throw Error(APSPIE_ELIMIT, "the result would hold " + std::to_string(total) +
+                                       " entries; a CSR holds fewer than 2^31 - 4096");
+    }
+    return int64_t(total);
+}
+
 }  // namespace sg
 }  // namespace apspie
 
 using namespace apspie;
+
+extern "C" int apspie_rowptr_from_counts(apspie_ctx *ctx, const int32_t *d_counts, int32_t n, int32_t *d_row_ptr,
+                                         int64_t *total) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        APSPIE_CHECK(n >= 0 && d_row_ptr != nullptr && (n == 0 || d_counts != nullptr), APSPIE_EINVAL,
+                     "bad arguments");
+        const int64_t t = sg::rowptr_scan(ctx, d_counts, n, d_row_ptr, [] {});
+        if (total) *total = t;
+    });
+}
 
 extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
                              apspie_stats *stats) {
@@ -1162,17 +1208,7 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
         c->m = m;
         c->n = b->n;
-        {
-            const int nb = std::max(1, (int)ceil_div(m, 256 * 16));
-            uint64_t *st = (uint64_t *)ctx->slot(S_STATUS, size_t(nb) * 8 + 64);
-            unsigned *ctr = (unsigned *)(st + nb);
-            k_zero_i32<<<std::max(1u, std::min(ceil_div(nb * 2 + 2, 256), 1024u)), 256, 0, ctx->stream>>>(
-                (int32_t *)st, int64_t(nb) * 2 + 2);
-            k_scan<<<nb, 256, 0, ctx->stream>>>(nnzc, c->row_ptr, m, st, ctr);
-        }
-        int32_t total = 0;
-        APSPIE_HIP(hipMemcpyAsync(&total, c->row_ptr + m, 4, hipMemcpyDeviceToHost, ctx->stream));
-        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        const int32_t total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
         c->nnz = total;
         if (hipMalloc(&c->col_ind, size_t(std::max(total, 1)) * 4) != hipSuccess ||
             hipMalloc(&c->val, size_t(std::max(total, 1)) * 4) != hipSuccess) {

@@ -119,7 +119,12 @@ struct PickSssp {
         const uint32_t d = v == seed ? 0u : x.d;
         return d < x.p && d >= threshold() ? d : ~0u;
     }
-    __device__ uint32_t raised(uint32_t dmin) const { return __float_as_uint(__uint_as_float(dmin) + delta); }
+    // dmin + delta, but at least the next float above dmin: where delta is at
+    // most half an ulp of dmin the sum rounds back to dmin, and a threshold
+    // that does not rise would admit nothing
+    __device__ uint32_t raised(uint32_t dmin) const {
+        return max(__float_as_uint(__uint_as_float(dmin) + delta), dmin + 1u);
+    }
     __device__ void on_new(int) const {}
     __device__ int direction(const Ctl &c, unsigned long long, int) const {
         if (bin_ratio > 0 && int64_t(c.E) * bin_ratio > nnz) return DIR_BINNED;

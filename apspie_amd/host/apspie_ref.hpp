@@ -15,8 +15,10 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -339,8 +341,155 @@ inline int mXm(const int edge, const int m, const T *d_cscValA, const int *d_csc
 }
 
 // ---------------------------------------------------------------------------
-// loader: readMtx / makeSymmetric (src/util.cuh:370-643) as one call returning
-// host CSR arrays with the reference's conventions.
+// loader: the reference's host calls (src/util.cuh:355-511, src/matrix.cpp:55-111)
+// with their signatures, so a driver written like test/bfs/main.cu:40-120 or
+// test/sssp/main.cu:25-110 compiles against this header.  Errors throw
+// apspie_ref::Error where the reference prints and carries on.
+
+// readEdge, src/util.cuh:355: skips the lines that start with '%' and reads
+// "m n edge".  The reference reads stdin whatever file it is given (its
+// drivers freopen the .mtx onto stdin); this reads inputFile.
+inline void readEdge(int &m, int &n, int &edge, FILE *inputFile) {
+    if (!inputFile) throw apspie_ref::Error(APSPIE_EINVAL, "readEdge: null file");
+    int c = fgetc(inputFile), old = 0;
+    while (c != EOF) {
+        if ((old == '\n' || old == 0) && c != '%') {
+            ungetc(c, inputFile);
+            break;
+        }
+        old = c;
+        c = fgetc(inputFile);
+    }
+    if (fscanf(inputFile, "%d %d %d", &m, &n, &edge) != 3)
+        throw apspie_ref::Error(APSPIE_EIO, "readEdge: no \"m n edge\" line");
+}
+
+// readMtx, src/util.cuh:370: `edge` lines "a b [w]" after readEdge; a - 1
+// goes to h_cooColInd, b - 1 to h_cooRowInd.  The file is weighted iff the
+// character right after the first line's second integer is a space; else
+// every value is 1.  Returns whether it is weighted.  Reads stdin, as the
+// reference does, unless another stream is given.
+template <typename typeVal>
+inline bool readMtx(int edge, int *h_cooColInd, int *h_cooRowInd, typeVal *h_cooVal, FILE *in = stdin) {
+    bool weighted = true;
+    int c = 0;
+    for (int j = 0; j < edge; ++j) {
+        if (fscanf(in, "%d", &h_cooColInd[j]) != 1 || fscanf(in, "%d", &h_cooRowInd[j]) != 1)
+            throw apspie_ref::Error(APSPIE_EIO, "readMtx: not enough rows in mtx file");
+        if (j == 0) c = fgetc(in);
+        if (c != ' ') {
+            h_cooVal[j] = typeVal(1);
+            if (j == 0) weighted = false;
+        } else {
+            float w = 0.f;
+            if (fscanf(in, "%f", &w) != 1) throw apspie_ref::Error(APSPIE_EIO, "readMtx: bad value");
+            h_cooVal[j] = typeVal(w);
+        }
+        h_cooRowInd[j]--;
+        h_cooColInd[j]--;
+    }
+    return weighted;
+}
+
+// makeSymmetric, src/util.cuh:572: the arrays hold room for 2 * edge
+// entries; appends the reversed non-loop entries, sorts by (row, col), drops
+// exact repeats; returns the new entry count.  Values move with their entries.
+template <typename typeVal>
+inline int makeSymmetric(int edge, int *h_cooColInd, int *h_cooRowInd, typeVal *h_cooVal) {
+    int64_t out = 0;
+    if (std::is_same<typeVal, float>::value) {
+        apspie_ref::check(apspie_host_coo_make_symmetric(edge, h_cooColInd, h_cooRowInd, (float *)h_cooVal, &out),
+                          nullptr, "makeSymmetric");
+    } else {
+        std::vector<float> v(size_t(2) * size_t(edge > 0 ? edge : 0));
+        for (int i = 0; i < edge; ++i) v[i] = float(h_cooVal[i]);
+        apspie_ref::check(apspie_host_coo_make_symmetric(edge, h_cooColInd, h_cooRowInd, v.data(), &out), nullptr,
+                          "makeSymmetric");
+        for (int64_t i = 0; i < out; ++i) h_cooVal[i] = typeVal(v[i]);
+    }
+    return int(out);
+}
+
+// The host-array buildMatrix test/bfs/main.cu:83 calls: stable counting sort
+// of the COO entries by h_cooRowInd into caller-allocated CSR arrays.
+template <typename typeVal>
+inline void buildMatrix(int *h_csrRowPtr, int *h_csrColInd, typeVal *h_csrVal, const int m, const int edge,
+                        const int *h_cooRowInd, const int *h_cooColInd, const typeVal *h_cooVal) {
+    std::vector<float> cv(size_t(edge > 0 ? edge : 0)), csr(cv.size());
+    for (int i = 0; i < edge; ++i) cv[i] = float(h_cooVal[i]);
+    apspie_ref::check(apspie_host_coo_to_csr(m, edge, h_cooRowInd, h_cooColInd, cv.data(), h_csrRowPtr, h_csrColInd,
+                                             csr.data()),
+                      nullptr, "buildMatrix");
+    for (int i = 0; i < edge; ++i) h_csrVal[i] = typeVal(csr[i]);
+}
+
+// coo2csr, src/util.cuh:456 (cusparseXcoo2csr): row offsets d_csrRowPtrA[m+1]
+// of the row-sorted COO row indices, on the most recently created context.
+inline void coo2csr(const int *d_cooRowIndA, const int edge, const int m, int *d_csrRowPtrA,
+                    mgpu::CudaContext &context = *mgpu::default_context()) {
+    apspie_ref::check(apspie_coo_to_csr(context.get(), d_cooRowIndA, edge, m, d_csrRowPtrA), context.get(), "coo2csr");
+}
+
+// csr2csc, src/util.cuh:473 (cusparseScsr2csc) for an m x m matrix: the CSC
+// arrays d_cscColPtrA[m+1], d_cscRowIndA[edge] and d_cscValA[edge] (rows
+// ascending in each column).  The reference asks cuSPARSE for the structure
+// only (CUSPARSE_ACTION_SYMBOLIC) and leaves d_cscValA unwritten; the values
+// are moved here too.
+template <typename typeVal>
+inline void csr2csc(const int m, const int edge, const typeVal *d_csrValA, const int *d_csrRowPtrA,
+                    const int *d_csrColIndA, typeVal *d_cscValA, int *d_cscRowIndA, int *d_cscColPtrA,
+                    mgpu::CudaContext &context = *mgpu::default_context()) {
+    static_assert(sizeof(typeVal) == 4, "FP32 values");
+    const apspie_csr g = apspie_ref::csr_of(m, edge, d_csrRowPtrA, d_csrColIndA, (const float *)d_csrValA);
+    apspie_ref::check(apspie_csr2csc(context.get(), &g, d_cscColPtrA, d_cscRowIndA,
+                                     d_csrValA ? (float *)d_cscValA : nullptr),
+                      context.get(), "csr2csc");
+}
+
+// d_scratch / allocScratch (src/scratch.hpp:5-31, src/util.cuh:646): the
+// reference's per-call scratch arrays.  The context owns a pooled scratch
+// here, so the struct is empty and only keeps callers' signatures.
+struct d_scratch {};
+inline void allocScratch(d_scratch **d, const int /*edge*/, const int /*m*/) { *d = new d_scratch(); }
+
+// mXv, src/mXv.cuh:250 (sparse form): the nnz listed entries
+// (d_randVecInd, d_randVecVal) times M, M given by the "csc" arrays read as
+// rows: the reached indices come back ascending in d_resultInd with their
+// plus-times sums in d_resultVal (room for m), and nnz becomes their count.
+// As in the reference, nnz == 0 returns at once.
+template <typename T>
+inline void mXv(const int *d_randVecInd, const T *d_randVecVal, const int edge, const int m, int &nnz,
+                const T *d_cscVal, const int *d_cscColPtr, const int *d_cscRowInd, int *d_resultInd, T *d_resultVal,
+                d_scratch * /*d*/, mgpu::CudaContext &context) {
+    static_assert(sizeof(T) == sizeof(float), "FP32 vectors");
+    if (nnz == 0) return;
+    const apspie_csr g = apspie_ref::csr_of(m, edge, d_cscColPtr, d_cscRowInd, (const float *)d_cscVal);
+    int32_t ny = 0;
+    apspie_ref::check(apspie_mxv_sparse(context.get(), &g, APSPIE_SR_PLUS_TIMES, d_randVecInd,
+                                        (const float *)d_randVecVal, nnz, d_resultInd, (float *)d_resultVal, &ny,
+                                        nullptr),
+                      context.get(), "mXv");
+    nnz = ny;
+}
+
+// verify, src/util.cuh:261 (with CompareResults :186-258): prints CORRECT or
+// the first mismatches and returns their count.  Exact comparison (the
+// reference allows FP32 results 10 % off and equates -1 with > 1e38; the
+// results here are bit-exact, with 1.70141e38 for unreachable vertices).
+template <typename value>
+inline int verify(const int m, const value *h_result, const value *h_resultCPU) {
+    int errors = 0;
+    for (int i = 0; i < m; ++i)
+        if (!(h_result[i] == h_resultCPU[i])) {
+            if (errors < 10) printf("mismatch at %d: %g vs %g\n", i, double(h_result[i]), double(h_resultCPU[i]));
+            errors++;
+        }
+    printf(errors ? "INCORRECT: %d errors\n" : "CORRECT\n", errors);
+    return errors;
+}
+
+// The whole loader as one call: readMtx + [makeSymmetric] + buildMatrix
+// returning host CSR arrays (apspie_load_mtx).
 namespace apspie_ref {
 
 struct HostGraph {

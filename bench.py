@@ -34,6 +34,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), else the architecture."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def reached_edges(h, reached_mask):
     """Graph500-style traversed edges: out-edges of reached vertices."""
     return int(h.degrees()[reached_mask].sum())
@@ -557,7 +569,7 @@ def main():
             "kind": "port",
             "sample": f"one full heap-Dijkstra (restating src/testSssp.cpp:33-115) over the same s{args.scale} "
                       f"graph from the same source: {cpu_s:.2f} s",
-            "cpu_model": platform.processor() or platform.machine(),
+            "cpu_model": cpu_model(),
             "nproc": os.cpu_count(),
             "matches_gpu": ok,
         }

@@ -199,6 +199,22 @@ int apspie_hbm_copy_bandwidth(apspie_ctx *ctx, size_t bytes, int iters, double *
 int apspie_load_mtx(const char *path, uint32_t flags, apspie_host_csr *out);
 void apspie_host_csr_free(apspie_host_csr *h);
 
+/* makeSymmetric (src/util.cuh:572-643) on host COO arrays that hold room
+ * for 2 * edge entries: the reversed copy of every entry that is not a
+ * self-loop is appended, the entries are sorted by (row, col) and exact
+ * repeats dropped (self-loops stay); values move with their entries (the
+ * reference's compaction moves the indices only).  coo_val may be NULL.
+ * *out_edge = the new entry count. */
+int apspie_host_coo_make_symmetric(int64_t edge, int32_t *coo_col, int32_t *coo_row, float *coo_val,
+                                   int64_t *out_edge);
+/* buildMatrix's CSR construction (src/matrix.cpp:55-111; also the host-array
+ * form test/bfs/main.cu:83 calls): stable counting sort of host COO entries
+ * by row into caller-allocated row_ptr[m+1], col_ind[nnz], val[nnz] (val
+ * NULL: no values; coo_val NULL: ones).  APSPIE_EINVAL for a row outside
+ * [0, m). */
+int apspie_host_coo_to_csr(int32_t m, int64_t nnz, const int32_t *coo_row, const int32_t *coo_col,
+                           const float *coo_val, int32_t *row_ptr, int32_t *col_ind, float *val);
+
 /* Binary CSR cache (no counterpart in the reference, which re-parses its .mtx
  * text on every run): header ("APSPCSR1", version, flags, m, n, nnz), the
  * arrays, a 64-bit check word.  Loading verifies the check word and the
@@ -315,6 +331,15 @@ int apspie_spmspv_or(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_in, 
  * spgemm<T>(C, A, B) (src/spgemm.cuh:345). */
 int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *b, apspie_csr_owned *c,
                   apspie_stats *stats);
+
+/* Row offsets from per-row counts: d_row_ptr[0..n] = exclusive scan of the n
+ * non-negative d_counts, *total (may be NULL) = their sum, exact in 64 bits.
+ * The Scan with a host-returned total that turns row lengths into C's row
+ * pointers (src/spgemmKernel.cuh:667).  APSPIE_ELIMIT when the total reaches
+ * 2^31 - 4096, the most a CSR here holds (d_row_ptr then holds wrapped
+ * offsets); apspie_spgemm refuses such a C the same way before allocating it. */
+int apspie_rowptr_from_counts(apspie_ctx *ctx, const int32_t *d_counts, int32_t n, int32_t *d_row_ptr,
+                              int64_t *total);
 
 /* C = A * B^T, C(r, c) = sum_k A(r, k) * B(c, k), by the reference's own
  * scheme: row strips of part_size rows of A and of B in DCSC, the column

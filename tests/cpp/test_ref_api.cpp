@@ -4,8 +4,11 @@
 // "REF_API OK" and exits 0 when all checks pass.
 #include <hip/hip_runtime.h>
 
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../apspie_amd/host/apspie_ref.hpp"
@@ -46,10 +49,166 @@ static orc_csr as_orc(const apspie_host_csr &h) {
     return o;
 }
 
+
+// test/bfs/main.cu:40-150 step by step: readEdge, readMtx, makeSymmetric,
+// the host-array buildMatrix, device copies, bfs, verify against the CPU BFS.
+// Two slips of that driver are not repeated: it passes makeSymmetric the
+// already doubled edge count (writing past its 2 * edge arrays) and reads the
+// values into h_csrValA while symmetrizing h_cooValA.
+static void bfs_driver(const char *path, bool undirected, mgpu::CudaContext &context) {
+    int m = 0, n = 0, edge = 0;
+    FILE *f = fopen(path, "r");
+    EXPECT(f != nullptr, "open mtx");
+    if (!f) return;
+    readEdge(m, n, edge, f);
+    if (undirected) edge = 2 * edge;
+    typedef float typeVal;
+    std::vector<typeVal> h_csrValA(edge), h_cooValA(edge);
+    std::vector<int> h_csrColIndA(edge), h_csrRowPtrA(m + 1), h_cooColIndA(edge), h_cooRowIndA(edge);
+    std::vector<int> h_bfsResult(m), h_bfsResultCPU(m);
+    if (undirected) {
+        readMtx<typeVal>(edge / 2, h_cooColIndA.data(), h_cooRowIndA.data(), h_cooValA.data(), f);
+        edge = makeSymmetric(edge / 2, h_cooColIndA.data(), h_cooRowIndA.data(), h_cooValA.data());
+    } else {
+        readMtx<typeVal>(edge, h_cooColIndA.data(), h_cooRowIndA.data(), h_cooValA.data(), f);
+    }
+    fclose(f);
+    buildMatrix<typeVal>(h_csrRowPtrA.data(), h_csrColIndA.data(), h_csrValA.data(), m, edge, h_cooRowIndA.data(),
+                         h_cooColIndA.data(), h_cooValA.data());
+    // the loader in one call gives the same CSR
+    apspie_ref::HostGraph hg;
+    apspie_ref::readMtxGraph(path, undirected, &hg);
+    bool same = hg.h.nnz == edge && hg.h.m == m;
+    for (int i = 0; same && i <= m; ++i) same &= hg.h.row_ptr[i] == h_csrRowPtrA[i];
+    for (int p = 0; same && p < edge; ++p) same &= hg.h.col_ind[p] == h_csrColIndA[p];
+    EXPECT(same, "readEdge + readMtx + makeSymmetric + buildMatrix == apspie_load_mtx");
+    int *d_bfsResult = nullptr, *d_csrRowPtrA = dev_copy(h_csrRowPtrA.data(), size_t(m) + 1),
+        *d_csrColIndA = dev_copy(h_csrColIndA.data(), size_t(edge));
+    (void)hipMalloc(&d_bfsResult, size_t(m) * sizeof(int));
+    orc_csr o;
+    o.m = m;
+    o.n = m;
+    o.nnz = edge;
+    o.row_ptr = h_csrRowPtrA.data();
+    o.col_ind = h_csrColIndA.data();
+    o.val = nullptr;
+    for (int source = 0; source < m; source += (m > 64 ? 7 : 1)) {
+        int depth = 1000;
+        depth = orc_bfs(&o, source, depth, h_bfsResultCPU.data(), nullptr, nullptr);  // bfsCPU
+        bfs(source, edge, m, h_csrRowPtrA.data(), d_csrRowPtrA, d_csrColIndA, d_bfsResult, depth, context);
+        (void)hipMemcpy(h_bfsResult.data(), d_bfsResult, size_t(m) * sizeof(int), hipMemcpyDeviceToHost);
+        EXPECT(verify(m, h_bfsResult.data(), h_bfsResultCPU.data()) == 0, "bfs driver");
+    }
+    for (void *p : {(void *)d_bfsResult, (void *)d_csrRowPtrA, (void *)d_csrColIndA}) (void)hipFree(p);
+}
+
+// test/sssp/main.cu:25-130: readMtx straight into the CSR column and value
+// arrays (the file lists its rows in order), coo2csr on the device, csr2csc,
+// sssp, verify against the CPU SSSP.  The file is written here from a loaded
+// graph with integer weights, rows in order.
+static void sssp_driver(const char *src_path, mgpu::CudaContext &context) {
+    apspie_ref::HostGraph hg;
+    apspie_ref::readMtxGraph(src_path, true, &hg);
+    char path[] = "/tmp/apspie_sssp_driver_XXXXXX";
+    const int fd = mkstemp(path);
+    EXPECT(fd >= 0, "temp file");
+    if (fd < 0) return;
+    FILE *w = fdopen(fd, "w");
+    fprintf(w, "%%%%MatrixMarket matrix coordinate real general\n%d %d %lld\n", hg.h.m, hg.h.m, (long long)hg.h.nnz);
+    for (int r = 0; r < hg.h.m; ++r)
+        for (int p = hg.h.row_ptr[r]; p < hg.h.row_ptr[r + 1]; ++p)
+            fprintf(w, "%d %d %d\n", hg.h.col_ind[p] + 1, r + 1, 1 + (p * 37) % 64);
+    fclose(w);
+    int m = 0, n = 0, edge = 0;
+    FILE *f = fopen(path, "r");
+    readEdge(m, n, edge, f);
+    typedef float Value;
+    std::vector<Value> h_csrValA(edge), h_ssspResult(m), h_ssspResultCPU(m);
+    std::vector<int> h_csrRowPtrA(m + 1), h_csrColIndA(edge), h_cooRowIndA(edge);
+    readMtx<Value>(edge, h_csrColIndA.data(), h_cooRowIndA.data(), h_csrValA.data(), f);
+    fclose(f);
+    unlink(path);
+    Value *d_csrValA = dev_copy(h_csrValA.data(), size_t(edge)), *d_cscValA = nullptr, *d_ssspResult = nullptr;
+    int *d_csrColIndA = dev_copy(h_csrColIndA.data(), size_t(edge)),
+        *d_cooRowIndA = dev_copy(h_cooRowIndA.data(), size_t(edge)), *d_csrRowPtrA = nullptr, *d_cscRowIndA = nullptr,
+        *d_cscColPtrA = nullptr;
+    (void)hipMalloc(&d_ssspResult, size_t(m) * sizeof(float));
+    (void)hipMalloc(&d_csrRowPtrA, (size_t(m) + 1) * sizeof(int));
+    (void)hipMalloc(&d_cscValA, size_t(edge) * sizeof(Value));
+    (void)hipMalloc(&d_cscRowIndA, size_t(edge) * sizeof(int));
+    (void)hipMalloc(&d_cscColPtrA, (size_t(m) + 1) * sizeof(int));
+    coo2csr(d_cooRowIndA, edge, m, d_csrRowPtrA);
+    (void)hipMemcpy(h_csrRowPtrA.data(), d_csrRowPtrA, (size_t(m) + 1) * sizeof(int), hipMemcpyDeviceToHost);
+    bool rp = true;
+    for (int i = 0; i <= m; ++i) rp &= h_csrRowPtrA[i] == hg.h.row_ptr[i];
+    EXPECT(rp, "coo2csr");
+    csr2csc<Value>(m, edge, d_csrValA, d_csrRowPtrA, d_csrColIndA, d_cscValA, d_cscRowIndA, d_cscColPtrA);
+    apspie_host_csr hc{m, m, edge, h_csrRowPtrA.data(), h_csrColIndA.data(), h_csrValA.data()}, ht{};
+    apspie_ref::check(apspie_host_csr_transpose(&hc, &ht), nullptr, "transpose");
+    EXPECT(host_copy(d_cscColPtrA, size_t(m) + 1) == std::vector<int>(ht.row_ptr, ht.row_ptr + m + 1) &&
+               host_copy(d_cscRowIndA, size_t(edge)) == std::vector<int>(ht.col_ind, ht.col_ind + edge) &&
+               host_copy(d_cscValA, size_t(edge)) == std::vector<float>(ht.val, ht.val + edge),
+           "csr2csc");
+    apspie_host_csr_free(&ht);
+    orc_csr o;
+    o.m = m;
+    o.n = m;
+    o.nnz = edge;
+    o.row_ptr = h_csrRowPtrA.data();
+    o.col_ind = h_csrColIndA.data();
+    o.val = h_csrValA.data();
+    for (int source = 0; source < m; source += (m > 64 ? 11 : 1)) {
+        orc_sssp_dijkstra(&o, source, h_ssspResultCPU.data());  // ssspCPU
+        for (float &x : h_ssspResultCPU)
+            if (x == -1.0f) x = APSPIE_SSSP_INF;
+        sssp<Value>(source, edge, m, d_csrValA, d_csrRowPtrA, d_csrColIndA, d_ssspResult, 1000, context);
+        (void)hipMemcpy(h_ssspResult.data(), d_ssspResult, size_t(m) * sizeof(float), hipMemcpyDeviceToHost);
+        EXPECT(verify<float>(m, h_ssspResult.data(), h_ssspResultCPU.data()) == 0, "sssp driver");
+    }
+    // mXv, sparse form (src/mXv.cuh:250): listed entries every third vertex
+    std::vector<int> xi;
+    std::vector<float> xv, xd(m, 0.0f);
+    std::vector<uint8_t> listed(m, 0);
+    for (int v = 0; v < m; v += 3) {
+        xi.push_back(v);
+        xv.push_back(0.5f + float(v % 7));
+        xd[v] = xv.back();
+        listed[v] = 1;
+    }
+    int nnz = int(xi.size());
+    int *d_xi = dev_copy(xi.data(), xi.size()), *d_yi = nullptr;
+    float *d_xv = dev_copy(xv.data(), xv.size()), *d_yv = nullptr;
+    (void)hipMalloc(&d_yi, size_t(m) * sizeof(int));
+    (void)hipMalloc(&d_yv, size_t(m) * sizeof(float));
+    d_scratch *d = nullptr;
+    allocScratch(&d, edge, m);
+    mXv<float>(d_xi, d_xv, edge, m, nnz, d_csrValA, d_csrRowPtrA, d_csrColIndA, d_yi, d_yv, d, context);
+    delete d;
+    std::vector<float> yo(m);
+    std::vector<uint8_t> reached(m);
+    orc_mxv(&o, 0, xd.data(), listed.data(), yo.data(), reached.data());
+    std::vector<int> wi;
+    std::vector<float> wv;
+    for (int v = 0; v < m; ++v)
+        if (reached[v]) {
+            wi.push_back(v);
+            wv.push_back(yo[v]);
+        }
+    EXPECT(nnz == int(wi.size()) && host_copy(d_yi, size_t(nnz)) == wi && host_copy(d_yv, size_t(nnz)) == wv,
+           "mXv sparse");
+    for (void *p : {(void *)d_csrValA, (void *)d_cscValA, (void *)d_ssspResult, (void *)d_csrColIndA,
+                    (void *)d_cooRowIndA, (void *)d_csrRowPtrA, (void *)d_cscRowIndA, (void *)d_cscColPtrA,
+                    (void *)d_xi, (void *)d_yi, (void *)d_xv, (void *)d_yv})
+        (void)hipFree(p);
+}
+
 int main(int argc, char **argv) {
     const char *mtx = argc > 1 ? argv[1] : "tests/golden/mtx/chesapeake.mtx";
     mgpu::ContextPtr context = mgpu::CreateCudaDevice(0);
     try {
+        bfs_driver(mtx, false, *context);
+        bfs_driver(mtx, true, *context);
+        sssp_driver(mtx, *context);
         // BFS (test/bfs/main.cu): undirected chesapeake, every source
         apspie_ref::HostGraph hg;
         apspie_ref::readMtxGraph(mtx, true, &hg);

@@ -86,3 +86,24 @@ def test_delta_unreachable_and_isolated_sources(tuned):
     for src in (int(np.nonzero(deg == 0)[0][0]), int(np.nonzero(deg == 1)[0][0])):
         dist, _ = tuned.sssp(g, src)
         assert np.array_equal(dist, oracle_dist(h, src)), src
+
+
+def test_delta_below_half_an_ulp_of_the_distances(tuned):
+    """Distances past 2^28 with delta 16: there dmin + delta rounds back to
+    dmin, so the threshold must still rise to the next float (else no round
+    admits anything and the traversal stalls)."""
+    tuned.set_tuning(sssp_delta=16)
+    n = 40  # a path 0 -> 1 -> ... with weights 2^27: dist[k] = k * 2^27
+    path = A.HostCsr(n, n, np.concatenate([np.arange(n, dtype=np.int32), [n - 1]]).astype(np.int32),
+                     np.arange(1, n, dtype=np.int32), np.full(n - 1, 2.0 ** 27, np.float32))
+    dist, _ = tuned.sssp(tuned.upload(path), 0)
+    assert np.array_equal(dist, (np.arange(n) * 2.0 ** 27).astype(np.float32))
+    # RMAT with weights (1..64) * 2^22: rounded sums, the same least fixed point
+    h = A.rmat(10, 8, 4410, undirected=False, weight_seed=5410)
+    h.val = (h.val * np.float32(2.0 ** 22)).astype(np.float32)
+    g = tuned.upload(h)
+    for src in (h.argmax_degree(), 3):
+        d, _ = tuned.sssp(g, src)
+        want = oracle_dist(h, src)
+        assert want[want < INF].max() > 2.0 ** 28
+        assert np.array_equal(d, want), src

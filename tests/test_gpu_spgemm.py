@@ -267,3 +267,22 @@ def test_spgemm_long_rows_sorted_in_lds_exact(ctx):
         ctx.set_tuning(**saved)
     assert np.array_equal(c.row_ptr, want.row_ptr) and np.array_equal(c.col_ind, want.col_ind)
     assert np.array_equal(c.val, want.val)
+
+def test_rowptr_scan_refuses_a_total_a_csr_cannot_hold(ctx):
+    """C's row pointers come from one 64-bit-exact scan of the row counts; a
+    total at or past 2^31 - 4096 is refused (apspie_spgemm checks it before
+    allocating C) instead of wrapping."""
+    rng = np.random.RandomState(11)
+    counts = rng.randint(0, 1000, 100_000).astype(np.int32)
+    rp, tot = ctx.rowptr_from_counts(counts)
+    assert tot == int(counts.sum())
+    assert np.array_equal(rp, np.concatenate([[0], np.cumsum(counts)]).astype(np.int32))
+    rp0, tot0 = ctx.rowptr_from_counts(np.zeros(0, np.int32))
+    assert tot0 == 0 and list(rp0) == [0]
+    big = np.array([2 ** 30, 2 ** 30 - 4096, 0, 5], np.int32)  # sums to 2^31 - 4091
+    with pytest.raises(A.ApspieError) as e:
+        ctx.rowptr_from_counts(big)
+    assert e.value.code == -5
+    edge = np.array([2 ** 30, 2 ** 30 - 4097], np.int32)  # one below the limit: accepted
+    rp2, tot2 = ctx.rowptr_from_counts(edge)
+    assert tot2 == 2 ** 31 - 4097 and rp2[-1] == 2 ** 31 - 4097

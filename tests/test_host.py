@@ -198,3 +198,69 @@ def test_csr_cache_rejects_damage(tmp_path):
         A.load_csr(f)
     with pytest.raises(A.ApspieError):
         A.load_csr(str(tmp_path / "missing.csr"))
+
+
+def _read_coo(path):
+    """The .mtx entries as (col, row, val) the way readMtx stores them (a - 1
+    into the column array, b - 1 into the row array), parsed independently."""
+    lines = [ln for ln in open(path).read().splitlines() if ln.strip() and not ln.startswith("%")]
+    nnz = int(lines[0].split()[2])
+    ent = [ln.split() for ln in lines[1:1 + nnz]]
+    col = np.array([int(e[0]) - 1 for e in ent], np.int32)
+    row = np.array([int(e[1]) - 1 for e in ent], np.int32)
+    val = np.array([float(e[2]) if len(e) > 2 else 1.0 for e in ent], np.float32)
+    return col, row, val
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_make_symmetric_and_build_match_the_loader(name):
+    """makeSymmetric + buildMatrix (the reference's host calls, apspie_host_coo_*)
+    give the CSR the one-call loader and the oracle give."""
+    p = os.path.join(MTX, name + ".mtx")
+    col, row, val = _read_coo(p)
+    m = int([ln for ln in open(p) if not ln.startswith("%")][0].split()[0])
+    c2, r2, v2 = A.coo_make_symmetric(col, row, val)
+    # symmetric, no repeats, self-loops kept, sorted by (row, col)
+    pairs = list(zip(r2.tolist(), c2.tolist()))
+    assert pairs == sorted(set(pairs))
+    assert set(pairs) == {(int(r), int(c)) for r, c in zip(row, col)} | {(int(c), int(r)) for r, c in zip(row, col)}
+    h = A.coo_to_csr_host(m, r2, c2, v2)
+    want = O.read_mtx(p, True)
+    assert np.array_equal(h.row_ptr, want.row_ptr) and np.array_equal(h.col_ind, want.col_ind)
+    # directed: a stable counting sort of the file order by row
+    hd = A.coo_to_csr_host(m, row, col, val)
+    wd = O.read_mtx(p, False)
+    assert np.array_equal(hd.row_ptr, wd.row_ptr) and np.array_equal(hd.col_ind, wd.col_ind)
+    order = np.argsort(row, kind="stable")
+    assert np.array_equal(hd.col_ind, col[order]) and np.array_equal(hd.val, val[order])
+
+
+def test_make_symmetric_keeps_values_with_entries():
+    col = np.array([1, 0, 2, 2], np.int32)
+    row = np.array([0, 1, 2, 0], np.int32)
+    val = np.array([5.0, 5.0, 7.0, 3.0], np.float32)
+    c, r, v = A.coo_make_symmetric(col, row, val)
+    assert list(zip(r.tolist(), c.tolist(), v.tolist())) == [(0, 1, 5.0), (0, 2, 3.0), (1, 0, 5.0), (2, 0, 3.0),
+                                                            (2, 2, 7.0)]
+    with pytest.raises(A.ApspieError):  # a row outside [0, m)
+        A.coo_to_csr_host(2, np.array([0, 2], np.int32), np.array([1, 1], np.int32))
+
+
+def test_server_submit_checks_the_output_array():
+    """The row is written from the server's threads after submit returns, so
+    a short, mistyped or non-contiguous array is refused before it is queued."""
+    srv = A.Server.__new__(A.Server)
+    srv._graphs = [type("G", (), {"m": 10})()]
+    srv._held = {}
+    srv.h = None
+    with pytest.raises(TypeError):
+        srv.submit(A.REQ_BFS, 0, 0, np.empty(10, np.float32))
+    with pytest.raises(TypeError):
+        srv.submit(A.REQ_SSSP, 0, 0, np.empty(10, np.int32))
+    with pytest.raises(ValueError):
+        srv.submit(A.REQ_BFS, 0, 0, np.empty(9, np.int32))
+    with pytest.raises(ValueError):
+        srv.submit(A.REQ_BFS, 0, 0, np.empty(20, np.int32)[::2])
+    with pytest.raises(ValueError):
+        srv.submit(A.REQ_BFS, 1, 0, np.empty(10, np.int32))
+    assert srv._held == {}

@@ -295,6 +295,39 @@ def test_sssp_invariants_rmat():
     assert sf == ri + 1 and er >= a.degrees()[d >= 0].sum()
 
 
+@pytest.mark.parametrize("seed,scale,ef", [(21, 6, 4), (22, 7, 8), (23, 7, 3)])
+def test_sssp_bf_counters_by_hop_limited_paths(seed, scale, ef):
+    """The schedule counters of orc_sssp_bf, from the characterisation of its
+    rounds rather than its loop: after round t every distance is the lightest
+    walk of at most t edges (D_t, by dense min-plus products), the frontier
+    F_t is {v : D_t[v] < D_t-1[v]} (F_0 = {source}), e_relax sums the
+    out-degrees of every F_t, sum_f their sizes, r_imp the improvements."""
+    a = O.rmat(scale, ef, seed, False, weight_seed=seed + 7)
+    src = O.argmax_degree(a)
+    m = a.m
+    W = np.full((m, m), np.inf)
+    for u in range(m):
+        for p in range(a.row_ptr[u], a.row_ptr[u + 1]):
+            W[u, a.col_ind[p]] = min(W[u, a.col_ind[p]], a.val[p])
+    deg = a.degrees()
+    D = np.full(m, np.inf)
+    D[src] = 0.0
+    F = np.zeros(m, bool)
+    F[src] = True
+    iters = er = sf = ri = 0
+    while F.any():
+        iters += 1
+        sf += int(F.sum())
+        er += int(deg[F].sum())
+        nxt = np.minimum(D, (D[:, None] + W).min(axis=0))  # walks of one more edge
+        F = nxt < D
+        ri += int(F.sum())
+        D = nxt
+    d, it, e_relax, sum_f, r_imp = O.sssp_bf(a, src)
+    assert (it, e_relax, sum_f, r_imp) == (iters, er, sf, ri)
+    assert np.array_equal(d, np.where(np.isinf(D), -1.0, D).astype(np.float32))
+
+
 def test_sssp_step_brute_force():
     a = O.rmat(8, 8, 9, False, weight_seed=10)
     rng = np.random.RandomState(0)
