@@ -162,6 +162,7 @@ def lib() -> C.CDLL:
         "apspie_bfs": [vp, P(_DCsr), i32, vp, P(Stats)],
         "apspie_sssp": [vp, P(_DCsr), i32, vp, P(Stats)],
         "apspie_sssp_step": [vp, P(_DCsr), vp, vp, P(Stats)],
+        "apspie_sssp_noalloc": [vp, P(_DCsr), i32, vp, P(Stats)],
         "apspie_mis": [vp, P(_DCsr), vp, vp, P(Stats)],
         "apspie_spmspv_or": [vp, P(_DCsr), vp, vp],
         "apspie_powerlaw_csr": [i32, i32, C.c_double, i32, u32, P(_Csr)],
@@ -523,6 +524,14 @@ class Context:
     def sssp_into(self, g: DeviceCsr, src: int, d_dist: DeviceBuffer) -> Stats:
         st = Stats()
         _check(lib().apspie_sssp(self.h, C.byref(g.c), src, d_dist.ptr, C.byref(st)), self.h, "sssp")
+        return st
+
+    def sssp_noalloc_into(self, g: DeviceCsr, src: int, d_dist: DeviceBuffer) -> Stats:
+        """apspie_sssp_noalloc: the same traversal with its scratch in the
+        library's fixed device array (no allocation in the call)."""
+        st = Stats()
+        _check(lib().apspie_sssp_noalloc(self.h, C.byref(g.c), src, d_dist.ptr, C.byref(st)), self.h,
+               "sssp_noalloc")
         return st
 
     def sssp(self, g: DeviceCsr, src: int):

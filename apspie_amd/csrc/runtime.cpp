@@ -3,6 +3,7 @@
 #include <new>
 
 #include "internal.hpp"
+#include "traverse.hpp"
 
 using namespace apspie;
 
@@ -102,6 +103,7 @@ int apspie_ctx_create(int device, apspie_ctx **out) {
         APSPIE_HIP(hipEventCreate(&ctx->ev1));
         APSPIE_HIP(hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device));
         APSPIE_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        trav::noalloc_setup(device);
     });
     if (rc != APSPIE_OK) {
         fprintf(stderr, "apspie_ctx_create: %s\n", ctx->err.c_str());

@@ -372,6 +372,67 @@ Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz) {
 
 Queue queue_of(const Bufs &b) { return Queue{b.queue, b.qstart, b.qoff, (uint32_t *)b.qval, b.tiles}; }
 
+// ----------------------------------------------------------------------------
+// fixed scratch of the allocation-free SSSP (apspie_sssp_noalloc): one array
+// per device, loaded with the library's code object, never allocated or freed
+// by a call.  Host code reaches it only through the address
+// hipGetSymbolAddress gives for each device (the array's name is a host-side
+// stand-in, not a device address).
+
+constexpr size_t NOALLOC_BYTES = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_noalloc_mu[MAX_DEVICES];     // serialises the calls that share a device's array
+void *g_noalloc_ptr[MAX_DEVICES] = {};    // the array's address on each device
+std::mutex g_noalloc_setup_mu;
+
+// Looks the array up for the current device (once per device).  Called with
+// `device` current, by apspie_ctx_create.
+void noalloc_setup(int device) {
+    APSPIE_CHECK(device >= 0 && device < MAX_DEVICES, APSPIE_ELIMIT, "device id beyond the fixed scratch table");
+    std::lock_guard<std::mutex> lk(g_noalloc_setup_mu);
+    if (g_noalloc_ptr[device]) return;
+    void *ptr = nullptr;
+    APSPIE_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    APSPIE_CHECK(ptr != nullptr, APSPIE_EHIP, "no device address for the fixed scratch");
+    g_noalloc_ptr[device] = ptr;
+}
+
+// Bytes a single-source SSSP over (m, nnz) carves from one block: distance
+// snapshots, the four queue arrays, the tile table, the look-back status and
+// the control block, each 256-byte aligned.
+size_t sssp_carve_bytes(int m, int64_t nnz) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t nblk = size_t(std::max(1, (int)ceil_div(m, CHUNK)));
+    return 5 * al((size_t(m) + 1) * 4) + al((size_t(nnz) / TILE + 4) * 4) + al(nblk * 2 * 8) + al(sizeof(Control));
+}
+
+// The SSSP buffers carved from `base` (sssp_carve_bytes(m, nnz) bytes); the
+// BFS-only flags and bitmaps stay null.
+Bufs sssp_carve(apspie_ctx *ctx, unsigned char *base, int m, int64_t nnz) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        void *p = base + at;
+        at += al(bytes);
+        return p;
+    };
+    Bufs b;
+    b.nblk = std::max(1, (int)ceil_div(m, CHUNK));
+    b.prev = (uint32_t *)take((size_t(m) + 1) * 4);
+    b.queue = (int32_t *)take((size_t(m) + 1) * 4);
+    b.qstart = (int32_t *)take((size_t(m) + 1) * 4);
+    b.qoff = (int32_t *)take((size_t(m) + 1) * 4);
+    b.qval = (float *)take((size_t(m) + 1) * 4);
+    b.tiles = (int32_t *)take((size_t(nnz) / TILE + 4) * 4);
+    b.status = (uint64_t *)take(size_t(b.nblk) * 2 * 8);
+    b.C = (Control *)take(sizeof(Control));
+    b.expand_grid =
+        std::max(1u, std::min(ceil_div(nnz, TILE) + 1, unsigned(ctx->cu_count * ctx->tune.expand_wg_per_cu)));
+    return b;
+}
+
 unsigned init_grid(const Bufs &b, int m) {
     return std::max(1u, std::min(ceil_div(std::max<int64_t>(m, int64_t(b.nblk) * CHUNK / 16), WG), 4096u));
 }
@@ -418,11 +479,14 @@ void launch_expand(apspie_ctx *ctx, const Bufs &b, const apspie_csr *g, const Op
     ctx->prof_end();
 }
 
+// fixed: buffers carved from the fixed scratch (apspie_sssp_noalloc); the
+// traversal then takes no slot of the pool, so it runs push rounds only
+// (pull and binned rounds need buffers of their own; the results are the same)
 template <int MODE>
 void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, float *dist_f,
-         apspie_stats *stats) {
-    Bufs b = get_bufs(ctx, g->m, g->nnz);
-    const bool pull = g->csc_col_ptr != nullptr && g->nnz > 0 &&
+         apspie_stats *stats, const Bufs *fixed = nullptr) {
+    Bufs b = fixed ? *fixed : get_bufs(ctx, g->m, g->nnz);
+    const bool pull = !fixed && g->csc_col_ptr != nullptr && g->nnz > 0 &&
                       (MODE == MODE_BFS ? ctx->tune.bfs_alpha > 0 : ctx->tune.sssp_pull_ratio > 0);
     uint32_t *fval = nullptr;
     if (pull) {  // growing a slot leaves the others in place
@@ -436,7 +500,8 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     // binned relaxation of dense SSSP rounds: entry buffer, per-unit offset
     // table and per-bucket totals
     const int nb = (int)ceil_div(g->m, BUCKET);
-    const bool binned = MODE == MODE_SSSP && ctx->tune.sssp_bin_ratio > 0 && g->nnz > 0 && nb <= MAX_BUCKETS;
+    const bool binned =
+        !fixed && MODE == MODE_SSSP && ctx->tune.sssp_bin_ratio > 0 && g->nnz > 0 && nb <= MAX_BUCKETS;
     uint2 *bin_out = nullptr;
     int32_t *bin_tbl = nullptr;
     uint32_t *bin_tot = nullptr;
@@ -568,6 +633,35 @@ int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist
         APSPIE_CHECK(!g->csc_col_ptr || (g->csc_val != nullptr) == (g->val != nullptr), APSPIE_EINVAL,
                      "the CSC must carry weights exactly when the CSR does");
         trav::run<trav::MODE_SSSP>(ctx, g, src, nullptr, d_dist, stats);
+    });
+}
+
+int apspie_sssp_noalloc(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist, apspie_stats *stats) {
+    if (!ctx) return APSPIE_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    return guarded(ctx, [&] {
+        using namespace trav;
+        validate_traversal(g, src);
+        APSPIE_CHECK(d_dist != nullptr, APSPIE_EINVAL, "null dist");
+        const size_t need = sssp_carve_bytes(g->m, g->nnz);
+        APSPIE_CHECK(need <= NOALLOC_BYTES, APSPIE_ELIMIT,
+                     "apspie_sssp_noalloc: the graph needs " + std::to_string(need) + " bytes of scratch, the fixed "
+                     "scratch holds " + std::to_string(NOALLOC_BYTES) + " (about 20 B per vertex + 2 B per 1024 "
+                     "edges); use apspie_sssp");
+        APSPIE_CHECK(ctx->device < MAX_DEVICES && g_noalloc_ptr[ctx->device] != nullptr, APSPIE_EHIP,
+                     "fixed scratch not set up on this device");
+        std::lock_guard<std::mutex> dev(g_noalloc_mu[ctx->device]);
+        const Bufs b = sssp_carve(ctx, (unsigned char *)g_noalloc_ptr[ctx->device], g->m, g->nnz);
+        // per-kernel timing creates events on demand: off for this call
+        const bool prof = ctx->profiling;
+        ctx->profiling = false;
+        try {
+            run<MODE_SSSP>(ctx, g, src, nullptr, d_dist, stats, &b);
+        } catch (...) {
+            ctx->profiling = prof;
+            throw;
+        }
+        ctx->profiling = prof;
     });
 }
 

@@ -81,5 +81,9 @@ struct Bufs {
 
 Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz);
 
+// Looks up the fixed scratch array of apspie_sssp_noalloc on `device` (the
+// current device), once per device; apspie_ctx_create calls it.
+void noalloc_setup(int device);
+
 }  // namespace trav
 }  // namespace apspie

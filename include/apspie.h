@@ -289,6 +289,20 @@ int apspie_bfs(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *d_dep
 int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist,
                 apspie_stats *stats);
 
+/* apspie_sssp without any allocation in the call.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The traversal's scratch
+ * is a fixed 48 MiB device array of the library (one per GPU, looked up when
+ * a context is created and held until the process ends) instead of the
+ * context's pool, so the graph may need at most 48 MiB of it: about 20 B per
+ * vertex plus 2 B per 1024 edges (m up to about 2.4 million); larger graphs
+ * get APSPIE_ELIMIT.  Push rounds only (the pull and binned knobs are
+ * ignored; results and counters are apspie_sssp's).  Per-kernel timing is
+ * off for the call.  Calls on one GPU share its array: the library
+ * serialises them, and any other use of that memory while a call runs would
+ * race. */
+int apspie_sssp_noalloc(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist, apspie_stats *stats);
+
 /* Round log of the last frontier call (apspie_bfs, apspie_sssp, ...) on this context: for
  * round i (i < 64), the frontier it produced (vertices with out-degree > 0
  * and their out-edges) and whether the next round expanded it by push (0)
