@@ -87,10 +87,15 @@ def run_served(server: Server, reqs: List[Req], open_loop: bool, outs: Optional[
     return rep
 
 
-def run_plain(ctx: Context, graphs: List[DeviceCsr], reqs: List[Req], open_loop: bool) -> dict:
-    """The same requests one at a time through apspie_bfs / apspie_sssp."""
+def run_plain(ctx: Context, graphs: List[DeviceCsr], reqs: List[Req], open_loop: bool,
+              degrees: Optional[List[np.ndarray]] = None) -> dict:
+    """The same requests one at a time through apspie_bfs / apspie_sssp.
+    With `degrees` (each graph's out-degrees), a request whose source has no
+    out-edges is answered without a launch, as the server answers it on the
+    host, so the two paths do the same work per request."""
     bufs = [ctx.alloc(4 * g.m) for g in graphs]
     lat = []
+    skipped = 0
     t0 = now() + 0.05
     for r in reqs:
         arr = t0 + r.arrival if open_loop else t0
@@ -99,7 +104,9 @@ def run_plain(ctx: Context, graphs: List[DeviceCsr], reqs: List[Req], open_loop:
             if delay > 0:
                 time.sleep(delay)
         g = graphs[r.graph]
-        if r.kind == REQ_BFS:
+        if degrees is not None and degrees[r.graph][r.src] == 0:
+            skipped += 1
+        elif r.kind == REQ_BFS:
             ctx.bfs_into(g, r.src, bufs[r.graph])
         else:
             ctx.sssp_into(g, r.src, bufs[r.graph])
@@ -112,6 +119,7 @@ def run_plain(ctx: Context, graphs: List[DeviceCsr], reqs: List[Req], open_loop:
     return {
         "mode": "open-loop" if open_loop else "saturation",
         "completed": len(reqs),
+        "answered_without_launch": skipped,
         "p50_ms": float(lat[int(0.50 * (len(lat) - 1) + 0.5)]),
         "p99_ms": float(lat[int(0.99 * (len(lat) - 1) + 0.5)]),
         "qps": len(reqs) / (t_end - first),

@@ -65,7 +65,9 @@ def time_steps(ctx, fn, steps):
 
 def sssp_class_bytes(rounds, m, nnz):
     """Algorithmic bytes per traversal of each SSSP kernel class, from the
-    round log (frontier i = (vertices, edges, pull) is expanded by round i+1).
+    round log (frontier i = (vertices, edges, pull) is expanded by round i+1;
+    round 0 is the seed kernel's, every later one a compaction's, and one more
+    compaction finds the frontier empty).
     push expansion: 12 B per relaxed edge (column, weight, distance probe),
       12 B per frontier vertex (row start, edge offset, distance snapshot),
       4 B per improved vertex (distance write);
@@ -75,8 +77,10 @@ def sssp_class_bytes(rounds, m, nnz):
     binned round, bin kernel: 8 B per relaxed edge (column, weight) + 12 B
       per frontier vertex + 8 B per candidate written;
     binned round, apply kernel: 8 B per candidate read + 4 B per vertex
-      (distance slices) + 4 B per improved vertex."""
-    out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0}
+      (distance slices) + 4 B per improved vertex;
+    compaction: 8 B per vertex (distance and round-start copy) + 28 B per
+      queued vertex (copy update, row-pointer pair, 16-byte queue entry)."""
+    out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0, "compact": 0.0}
     for i, (nf, e, mode) in enumerate(rounds):
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
         if mode == 1:
@@ -86,16 +90,22 @@ def sssp_class_bytes(rounds, m, nnz):
             out["apply"] += 8.0 * e + 4.0 * m + 4.0 * nxt
         else:
             out["expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
+        out["compact"] += 8.0 * m + 28.0 * nxt
     return out
 
 
 def bfs_class_bytes(rounds, m):
-    """Algorithmic bytes per traversal of each BFS kernel class.
+    """Algorithmic bytes per traversal of each BFS kernel class (round 0 is
+    the seed kernel's, every later one a compaction's, and one more compaction
+    finds the frontier empty).
     push expansion: 8 B per expanded edge (column, visited-bitmap probe),
       8 B per frontier vertex (row start, edge offset), 1 B per new vertex;
     bottom-up step: the m/8-byte visited bitmap, 12 B per unvisited vertex
-      (column-pointer pair, first in-edge), 1 B per new vertex."""
-    out = {"expand": 0.0, "pull": 0.0}
+      (column-pointer pair, first in-edge), 1 B per new vertex;
+    compaction: 1 B per vertex (touched flag), the visited bitmap read and
+      written and the frontier bitmap written (3m/8 B), 29 B per new vertex
+      (flag clear, depth, row-pointer pair, 16-byte queue entry)."""
+    out = {"expand": 0.0, "pull": 0.0, "compact": 0.0}
     seen = 0
     for i, (nf, e, pull) in enumerate(rounds):
         seen += nf
@@ -104,12 +114,15 @@ def bfs_class_bytes(rounds, m):
             out["pull"] += m / 8.0 + 12.0 * (m - seen) + nxt
         else:
             out["expand"] += 8.0 * e + 8.0 * nf + nxt
+        out["compact"] += m + 3.0 * m / 8.0 + 29.0 * nxt
     # with a CSC, one launch per round runs either form (k_round_bfs)
     out["round"] = out["expand"] + out["pull"]
     return out
 
 
 KERNEL_NAMES = {
+    ("sssp", "compact"): "k_compact<PickSssp>",
+    ("bfs", "compact"): "k_compact<PickBfs>",
     ("sssp", "expand"): "k_expand<OpRelax>",
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",
@@ -135,6 +148,14 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
     avg_ms = ms / max(launches, 1)
     bytes_per_launch = bytes_by_class[cls] * steps / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    classes = {}
+    for c, b in bytes_by_class.items():  # every class that ran, the same way
+        if prof[c][1] and prof[c][0] > 0 and (algo, c) in KERNEL_NAMES:
+            a = b * steps / (prof[c][0] * 1e-3) / 1e9
+            classes[KERNEL_NAMES[(algo, c)]] = {"achieved": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4),
+                                                "avg_launch_us": round(prof[c][0] / prof[c][1] * 1e3, 3),
+                                                "launches_per_step": prof[c][1] / steps,
+                                                "bytes_per_launch": int(b * steps / prof[c][1])}
     return {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -148,7 +169,24 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
         "bytes_per_launch": int(bytes_per_launch),
         "ms_per_step": {c: round(prof[c][0] / steps, 4) for c in prof if prof[c][1]},
         "bytes_per_step": {c: int(v) for c, v in bytes_by_class.items()},
+        "classes": classes,
     }
+
+
+def attach_traffic(roof, workload):
+    """HBM bytes per launch of the roofline's kernel on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same dispatches
+    (profiles/<kernel>_<workload>_traffic.json, written from `bench.py
+    --profile-workload` runs), scaled by a copy kernel's counters."""
+    kname = roof["kernel"].split("<")[0] + ("_" + roof["kernel"].split("<")[1].rstrip(">") if "<" in roof["kernel"]
+                                            else "")
+    path = os.path.join(ROOT, "profiles", f"{kname}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return
+    tr = json.load(open(path))
+    roof["traffic"] = int(tr["traffic_bytes_per_launch"])
+    roof["traffic_source"] = os.path.relpath(path, ROOT)
+    roof["traffic_over_algorithmic"] = round(roof["traffic"] / max(roof["bytes_per_launch"], 1), 3)
 
 
 def batch_sources(h, k, seed):
@@ -375,6 +413,7 @@ def bench_served(ctx, n):
     hosts = serving.build_pool()
     graphs = [ctx.upload(h) for h in hosts]
     ms = [h.m for h in hosts]
+    degrees = [h.degrees() for h in hosts]
     del hosts
     log(f"serving pool of {len(graphs)} graphs ready ({time.time() - t:.1f}s)")
     reqs = serving.make_stream(ms, n=n, rate=2000.0)
@@ -385,13 +424,77 @@ def bench_served(ctx, n):
         log(f"served open-loop: {res['served_open_loop']}")
         res["served_saturation"] = serving.run_served(srv, reqs, open_loop=False)
         log(f"served saturation: {res['served_saturation']}")
-    npl = min(n, 2000)
-    res["plain_open_loop"] = serving.run_plain(ctx, graphs, reqs[:npl], open_loop=True)
-    res["plain_saturation"] = serving.run_plain(ctx, graphs, reqs[:npl], open_loop=False)
-    res["plain_requests"] = npl
-    log(f"plain: {res['plain_open_loop']} {res['plain_saturation']}")
-    res["saturation_qps_ratio"] = round(res["served_saturation"]["qps"] / res["plain_saturation"]["qps"], 2)
+    # plain over the whole stream: every request launched, then with the
+    # server's own shortcut (sources without out-edges answered without a
+    # launch), the like-for-like comparison
+    res["plain_open_loop"] = serving.run_plain(ctx, graphs, reqs, open_loop=True, degrees=degrees)
+    res["plain_saturation"] = serving.run_plain(ctx, graphs, reqs, open_loop=False)
+    res["plain_saturation_same_shortcut"] = serving.run_plain(ctx, graphs, reqs, open_loop=False,
+                                                              degrees=degrees)
+    res["plain_requests"] = n
+    log(f"plain: {res['plain_open_loop']} {res['plain_saturation']} {res['plain_saturation_same_shortcut']}")
+    sq = res["served_saturation"]["qps"]
+    res["saturation_qps_ratio"] = round(sq / res["plain_saturation"]["qps"], 2)
+    res["saturation_qps_ratio_like_for_like"] = round(sq / res["plain_saturation_same_shortcut"]["qps"], 2)
     return res
+
+
+def profile_workload(args):
+    """One workload alone, for rocprofv3 runs whose dispatches are only that
+    workload's: a cold call, `warmup` more and `steps` more, with the knobs of
+    --tuning (else the library defaults).  Prints the workload's round log and
+    per-class algorithmic bytes so counter averages per dispatch can be set
+    against them."""
+    w = args.profile_workload
+    ctx = A.Context(0)
+    if args.tuning:
+        ctx.set_tuning(**json.loads(args.tuning))
+    calls = 1 + args.warmup + args.steps
+    info = {"workload": w, "calls": calls, "tuning": ctx.tuning()}
+    if w in ("sssp_rmat_s22", "bfs_rmat_s22"):
+        if w == "sssp_rmat_s22":
+            h = A.rmat(22, 16, SSSP_SEED, undirected=False, weight_seed=SSSP_WSEED)
+            src = h.argmax_degree()
+            g = ctx.upload(h, csc=A.transpose(h))
+            d = ctx.alloc(4 * h.m)
+            fn = lambda: ctx.sssp_into(g, src, d)  # noqa: E731
+        else:
+            h = A.rmat(22, 16, BFS_SEED, undirected=True)
+            g = ctx.upload(h, csc="symmetric")
+            d = ctx.alloc(4 * h.m)
+            fn = lambda: ctx.bfs_into(g, 0, d)  # noqa: E731
+        for _ in range(calls):
+            st = fn()
+        rounds = ctx.rounds()
+        info.update(m=h.m, nnz=h.nnz, rounds=rounds, device_ms=st.kernel_ms, stats=st.as_dict(),
+                    class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz) if w.startswith("sssp")
+                                 else bfs_class_bytes(rounds, h.m)))
+    elif w == "spgemm_powerlaw_1M":
+        h = A.powerlaw(1 << 20, 1 << 20, 8.0, 4096, 3003)
+        a = ctx.upload(h)
+        for _ in range(calls):
+            c, st = ctx.spgemm_device(a, a)
+            ctx.free_owned(c)
+        info.update(stats=st.as_dict())
+    elif w == "tc_rmat_s20":
+        lo = A.strict_lower(A.rmat(20, 16, 3020, undirected=True))
+        gl = ctx.upload(lo)
+        for _ in range(calls):
+            _, st = ctx.tc(gl)
+        info.update(stats=st.as_dict())
+    elif w == "mis_rmat_s21":
+        from apspie_amd.serving import mt19937_stream
+
+        hm = A.rmat(21, 16, 4021, undirected=True)
+        gm = ctx.upload(hm)
+        prio = mt19937_stream(4021, hm.m).astype(np.uint32)
+        for _ in range(calls):
+            _, st = ctx.mis(gm, prio)
+        info.update(stats=st.as_dict())
+    else:
+        raise SystemExit(f"unknown workload {w}")
+    ctx.close()
+    print(json.dumps(info), flush=True)
 
 
 def main():
@@ -409,6 +512,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB device copy (counter calibration) and exit")
+    ap.add_argument("--profile-workload", choices=["sssp_rmat_s22", "bfs_rmat_s22", "spgemm_powerlaw_1M",
+                                                   "tc_rmat_s20", "mis_rmat_s21"],
+                    help="run one workload alone (for rocprofv3) and exit")
+    ap.add_argument("--tuning", help="JSON object of tuning knobs for --profile-workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): sharding the traversals and "
@@ -418,6 +525,9 @@ def main():
     if args.copy_only:
         with A.Context(0) as c:
             print(json.dumps({"copy_gbps": round(c.hbm_copy_bandwidth(1 << 30, 6), 1)}))
+        return
+    if args.profile_workload:
+        profile_workload(args)
         return
 
     t = time.time()
@@ -455,14 +565,8 @@ def main():
 
     rounds = ctx.rounds()
     roof = kernel_roofline(ctx, run, max(3, args.steps // 2), sssp_class_bytes(rounds, h.m, h.nnz), "sssp")
-    kfile = roof["kernel"].split("<")[0]
-    traffic_file = os.path.join(ROOT, "profiles", f"{kfile}_sssp_rmat_s22_traffic.json")
-    if os.path.exists(traffic_file) and args.scale == SSSP_SCALE:
-        # HBM bytes per launch of the same kernel on the same workload, from
-        # rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy kernel
-        tr = json.load(open(traffic_file))
-        roof["traffic"] = int(tr["traffic_bytes_per_launch"])
-        roof["traffic_source"] = os.path.relpath(traffic_file, ROOT)
+    if args.scale == SSSP_SCALE:
+        attach_traffic(roof, "sssp_rmat_s22")
     log(f"roofline({roof['kernel']}): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak); "
         f"rounds {rounds}")
 
@@ -523,6 +627,7 @@ def main():
         te_b = reached_edges(hb, depth >= 0)
         rounds_b = ctx.rounds()
         roof_b = kernel_roofline(ctx, runb, max(3, args.steps // 2), bfs_class_bytes(rounds_b, hb.m), "bfs")
+        attach_traffic(roof_b, "bfs_rmat_s22")
         out["bfs"] = {
             "workload": f"bfs_rmat_s{BFS_SCALE}_undirected_ef16_src0",
             "nnz": hb.nnz,
