@@ -181,7 +181,7 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
     __shared__ int32_t s_vert[TILE + 1];
     __shared__ int32_t s_wmax[NW];
     const int level = C->cur_level;
-    const Ctl cur = C->ctl[(level - 1) & 1];
+    const Round cur = round_of(C, level);
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int sub = lane / K, src = lane % K;
     for (int t = blockIdx.x; t < cur.nTiles; t += gridDim.x) {
@@ -368,11 +368,11 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
     const OpMs<W> op{fr, st};
     auto compact = [&] {
         k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
-                                                               b.status, b.C);
+                                                               b.C);
     };
     // seeding round (round 0) runs eagerly, the rest from the captured graph
     k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
-                                                           b.status, b.C);
+                                                           b.C);
     APSPIE_HIP(hipGetLastError());
     const std::string key = graph_key("msbfs", g, b, {st, fr, depth, b.status, b.tiles}, W * 1000 + k);
     run_rounds(ctx, b, key, g->m + 2, [&] {
@@ -397,14 +397,14 @@ void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k
     k_mss_seed<K><<<1, 64, 0, ctx->stream>>>(k, srcs, dist, chg);
     const PickChanged pk{chg, fr, g->m};
     k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
-                                                             b.status, b.C);
+                                                             b.C);
     APSPIE_HIP(hipGetLastError());
     const std::string key = graph_key("mssssp", g, b, {dist, chg, fr, b.status, b.tiles}, K);
     run_rounds(ctx, b, key, g->m * 2 + 4, [&] {
         k_expand_ms_sssp<K><<<b.expand_grid, WG, 0, ctx->stream>>>(g->m, b.C, queue_of(b), g->col_ind, g->val,
                                                                     dist, fr, chg);
         k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
-                                                                 b.status, b.C);
+                                                                 b.C);
     });
     k_mss_transpose<K><<<ceil_div(g->m, 64), 256, 0, ctx->stream>>>(g->m, k, dist, (uint32_t *)out);
     APSPIE_HIP(hipGetLastError());

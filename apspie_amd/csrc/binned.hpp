@@ -64,7 +64,7 @@ __global__ __launch_bounds__(WG) void k_bin_relax(int level_arg, int m, int nb, 
     // totals start cleared by the traversal's initialisation
     if (blockIdx.x == 0)
         for (int b = tid; b < MAX_BUCKETS; b += WG) totals[((level & 1) ^ 1) * MAX_BUCKETS + b] = 0;
-    const Ctl cur = C->ctl[(level - 1) & 1];
+    const Round cur = round_of(C, level);
     if (cur.pull != DIR_BINNED) return;
     const int G = gridDim.x, w = blockIdx.x;
     const int t0 = bin_tile_lo(w, G, cur.nTiles), t1 = bin_tile_lo(w + 1, G, cur.nTiles);

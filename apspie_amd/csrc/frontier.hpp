@@ -1,13 +1,16 @@
 // frontier.hpp -- the two generic gfx950 kernels every frontier algorithm is
 // built from (BFS, SSSP, MIS, Boolean SpMSpV):
 //
-//   k_compact<Pick>  one coalesced pass over the vertex range in id order.
-//                    Pick decides which vertices enter the new frontier (and
-//                    may update per-vertex state on the way).  Entries with
-//                    out-degree > 0 are queued with (vertex, row start, edge
-//                    offset, 32-bit payload) through a single-pass decoupled
-//                    look-back scan, and the tile -> first-entry table for the
-//                    expansion is written.
+//   k_compact<Pick>  one coalesced pass over the vertex range, a chunk of
+//                    CHUNK vertices per workgroup.  Pick decides which
+//                    vertices enter the new frontier (and may update
+//                    per-vertex state on the way).  Entries with out-degree
+//                    > 0 are queued with (vertex, row start, edge offset,
+//                    32-bit payload) in their chunk group's region (one
+//                    returning atomic add per chunk on the group's counter
+//                    gives its place; nothing waits on another workgroup),
+//                    and the tile -> first-entry table for the expansion is
+//                    written.
 //   k_expand<Op>     the queued entries' edges, cut into equal tiles of TILE
 //                    edges; each tile finds its entries from the table, an
 //                    inclusive max-scan over entry start marks gives every edge
@@ -31,13 +34,32 @@ namespace trav {
 
 using namespace dev;
 
+// Queue regions: group g's entries sit at [g * qcap, g * qcap + entries of
+// g), its tile table at [g * tcap, ...); offsets and tile numbers inside a
+// group are the group's own.
 struct Queue {
     int32_t *vert;    // vertex id
     int32_t *start;   // row_ptr[v]
-    int32_t *off;     // exclusive offset of the entry's first edge
+    int32_t *off;     // exclusive offset of the entry's first edge within its group
     uint32_t *val;    // payload (distance bits, priority, ...)
-    int32_t *tiles;   // first entry of each edge tile
+    int32_t *tiles;   // first entry of each edge tile of each group
+    int32_t qcap, tcap;
 };
+
+// The frontier a round expands: the totals and, per group, its first tile in
+// the round's tile numbering and its (entries << 32) | edges.
+struct Round : Ctl {
+    const int32_t *gtile;
+    const unsigned long long *gcnt;
+};
+__device__ __forceinline__ Round round_of(const Control *C, int level) {
+    const int pp = (level - 1) & 1;
+    Round r;
+    static_cast<Ctl &>(r) = C->ctl[pp];
+    r.gtile = C->gtile[pp];
+    r.gcnt = C->gcnt[pp];
+    return r;
+}
 
 // host helpers shared by the frontier algorithms (traverse.hip)
 Queue queue_of(const Bufs &b);
@@ -46,8 +68,8 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
 
 // Compaction.  Wave w of workgroup b owns vertices [b*CHUNK + w*SEGV, +SEGV)
 // and walks them in ROUNDS rounds of 64 consecutive vertices; in-wave ranks
-// come from ballots (mbcnt) and wave scans, the workgroup offset from the
-// look-back.
+// come from ballots (mbcnt) and wave scans, the workgroup's place in its
+// chunk group from one returning atomic add of its (entries, edges).
 //
 // Pick interface, either one phase
 //   bool pick(int v, uint32_t &payload)       v may be >= m (padding lanes)
@@ -70,6 +92,8 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
 // A negative level_arg means "device-driven": the round number is read from
 // C->cur_level and the compaction advances it when it completes, so a
 // captured sequence of launches can be replayed unchanged (hipGraph).
+// The workgroup that finishes last (counted per group, then over the groups)
+// writes the round's totals, tile numbering, direction and log entry.
 template <class P, class = void>
 struct pick_phases {
     struct Pre {};
@@ -100,19 +124,17 @@ template <class P>
 struct has_defer<P, std::void_t<decltype(&P::defer_key)>> : std::true_type {};
 
 template <class Pick>
-__global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, int nblk, int always,
-                                                 const int32_t *__restrict__ row_ptr, Queue q,
-                                                 uint64_t *status, Control *C) {
+__global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_compact(Pick pk, int level_arg, int m, int nblk, int always,
+                                                 const int32_t *__restrict__ row_ptr, Queue q, Control *C) {
     constexpr int NW = WGC / WAVE, SEGV = CHUNK / NW, ROUNDS = SEGV / WAVE;
     static_assert(ROUNDS <= 32, "round bits must fit a 32-bit mask");
-    __shared__ int s_bid;
     __shared__ uint64_t s_wsum[NW];
     __shared__ uint32_t s_new[NW];
     __shared__ uint32_t s_def[NW], s_dmin[NW];
-    __shared__ uint64_t s_prefix;
+    __shared__ uint64_t s_base;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-    // every block reads the round before it takes a dynamic id, so the last
-    // id's block may advance cur_level once it has finished
+    // every workgroup reads the round before its completion is counted, so
+    // the one that completes last may advance cur_level
     // (an atomic load keeps level_arg in a register: a select between the
     // two addresses of a volatile load would put it in scratch memory)
     const int level = level_arg >= 0 ? level_arg
@@ -123,18 +145,18 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         if (blockIdx.x == 0 && tid == 0) *next = Ctl{0, 0, 0, 0};
         return;
     }
-    if (tid == 0) s_bid = (int)atomicAdd(&C->blk_counter[par], 1u);
-    __syncthreads();
-    const int bid = s_bid;
-    // state for the launch after this one (its previous user has finished)
-    if (tid == 0) status[(par ^ 1) * nblk + bid] = 0;
-    if (bid == 0 && tid < SPREAD) {
+    const int bid = blockIdx.x, grp = bid % NGROUP;
+    // counters of the round after this one (their last readers, the
+    // expansion of this round among them, have finished)
+    if (bid == 0 && tid < NGROUP) {
+        C->gcnt[par ^ 1][tid] = 0;
+        C->gdone[par ^ 1][tid] = 0;
+        C->nnew[par ^ 1][tid] = 0;
         if (tid == 0) {
-            C->blk_counter[par ^ 1] = 0;
+            C->alldone[par ^ 1] = 0;
             C->ndefer[par ^ 1] = 0;
             C->dmin[par ^ 1] = ~0u;
         }
-        C->nnew[par ^ 1][tid] = 0;
     }
 
     const int vbase = bid * CHUNK + wid * SEGV;
@@ -165,29 +187,33 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         }
     }
     if constexpr (has_on_balls<Pick>::value) pk.on_balls(myball, vbase, lane, ROUNDS);
-    // row extents of the picked vertices (picked vertices are < m, so
-    // row_ptr[v + 1] exists); every load is issued before any is used
-    int32_t rs[ROUNDS], dg[ROUNDS];  // dg holds the row end until it becomes the degree
-#pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-        rs[r] = 0;
-        dg[r] = 0;
-        if ((newbits >> r) & 1u) {
-            const int v = vbase + r * WAVE + lane;
-            rs[r] = row_ptr[v];
-            dg[r] = row_ptr[v + 1];
-        }
-    }
+    // degrees of the picked vertices (picked vertices are < m, so
+    // row_ptr[v + 1] exists); every load is issued before any is used.  The
+    // row extents are not kept past the workgroup's reduction (they are read
+    // again, from cache, when the entries are written): fewer live registers
+    // let two 1024-thread workgroups share a compute unit.
     uint32_t qb = 0, lane_cnt = 0, lane_es = 0;
     const uint32_t lane_new = __popc(newbits);
+    {
+        int32_t rs[ROUNDS], re[ROUNDS];
 #pragma unroll
-    for (int r = 0; r < ROUNDS; ++r) {
-        const int d = dg[r] - rs[r];
-        dg[r] = d;
-        if (d > 0) {
-            qb |= 1u << r;
-            lane_cnt++;
-            lane_es += d;
+        for (int r = 0; r < ROUNDS; ++r) {
+            rs[r] = 0;
+            re[r] = 0;
+            if ((newbits >> r) & 1u) {
+                const int v = vbase + r * WAVE + lane;
+                rs[r] = row_ptr[v];
+                re[r] = row_ptr[v + 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ROUNDS; ++r) {
+            const int d = re[r] - rs[r];
+            if (d > 0) {
+                qb |= 1u << r;
+                lane_cnt++;
+                lane_es += d;
+            }
         }
     }
     const uint32_t wcnt = wave_total32(lane_cnt), wes = wave_total32(lane_es), wnew = wave_total32(lane_new);
@@ -196,7 +222,7 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         dmin = wave_min_u32(dmin);
     }
     if (lane == 0) {
-        s_wsum[wid] = (uint64_t(wcnt) << 31) | wes;
+        s_wsum[wid] = (uint64_t(wcnt) << 32) | wes;
         s_new[wid] = wnew;
         s_def[wid] = ndef;
         s_dmin[wid] = dmin;
@@ -210,83 +236,115 @@ __global__ __launch_bounds__(WGC) void k_compact(Pick pk, int level_arg, int m, 
         total += s_wsum[w];
         bnew += s_new[w];
     }
-    if (wid == 0) {
+    if (tid == 0) {
         if constexpr (has_defer<Pick>::value) {
-            // waiting vertices are counted before this workgroup publishes its
-            // prefix, so the last workgroup sees every count
-            if (lane == 0) {
-                uint32_t bdef = 0, bmin = ~0u;
+            // waiting vertices are counted before this workgroup's completion,
+            // so the last workgroup sees every count
+            uint32_t bdef = 0, bmin = ~0u;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) {
-                    bdef += s_def[w];
-                    bmin = min(bmin, s_dmin[w]);
-                }
-                if (bdef) {
-                    atomicAdd(&C->ndefer[par], bdef);
-                    atomicMin(&C->dmin[par], bmin);
-                }
+            for (int w = 0; w < NW; ++w) {
+                bdef += s_def[w];
+                bmin = min(bmin, s_dmin[w]);
             }
-            __threadfence();
+            if (bdef) {
+                atomicAdd(&C->ndefer[par], bdef);
+                atomicMin(&C->dmin[par], bmin);
+            }
         }
-        const uint64_t p = lookback_exclusive(status + par * nblk, bid, total, lane);
-        if (lane == 0) s_prefix = p;
+        if (bnew) {
+            atomicAdd(&C->nnew[par][bid % SPREAD], bnew);
+            atomicAdd(&C->acc.vertices[bid % SPREAD], (unsigned long long)bnew);
+        }
+        // this chunk's place in its group: the entries and edges before it
+        // (a chunk that queues nothing needs none)
+        s_base = total ? atomicAdd(&C->gcnt[par][grp], (unsigned long long)total) : 0ull;
     }
     __syncthreads();
-    const uint64_t base = s_prefix + wpre;
-    int pos = int(base >> 31);
-    int off = int(base & 0x7fffffffu);
+    // this chunk's counts are all in: count it done (the return is awaited
+    // only after this wave's writes, which hide the round trip)
+    unsigned done = 0;
+    if (tid == 0) {
+        __threadfence();
+        done = atomicAdd(&C->gdone[par][grp], 1u) + 1u;
+    }
+    const uint64_t base = s_base + wpre;
+    int pos = int(base >> 32);
+    int off = int(base & 0xffffffffu);
+    const size_t qo = size_t(grp) * q.qcap;
+    int32_t *const tl = q.tiles + size_t(grp) * q.tcap;
 #pragma unroll
     for (int r = 0; r < ROUNDS; ++r) {
         const bool qd = (qb >> r) & 1u;
         const uint64_t qball = __ballot(qd);
         if (!qball) continue;
+        const int v = vbase + r * WAVE + lane;
+        const int r0 = qd ? row_ptr[v] : 0;
+        const int d = qd ? row_ptr[v + 1] - r0 : 0;
         const int rank =
             __builtin_amdgcn_mbcnt_hi(uint32_t(qball >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(qball), 0));
-        const uint32_t d = qd ? uint32_t(dg[r]) : 0u;
-        const uint32_t dincl = wave_incl_sum_dpp(d);
+        const uint32_t dincl = wave_incl_sum_dpp(uint32_t(d));
         if (qd) {
             const int p = pos + rank;
-            const int o = off + int(dincl - d);
-            q.vert[p] = vbase + r * WAVE + lane;
-            q.start[p] = rs[r];
-            q.off[p] = o;
-            q.val[p] = vv[r];
+            const int o = off + int(dincl) - d;
+            q.vert[qo + p] = v;
+            q.start[qo + p] = r0;
+            q.off[qo + p] = o;
+            q.val[qo + p] = vv[r];
             // tiles whose first edge falls inside this entry's edge range
-            for (int t = (o + TILE - 1) / TILE; t * TILE < o + int(d); ++t) q.tiles[t] = p;
+            for (int t = (o + TILE - 1) / TILE; t * TILE < o + d; ++t) tl[t] = p;
         }
         pos += __popcll(qball);
         off += wave_last(int(dincl));
     }
-    if (tid == 0) {
-        if (bnew) {
-            atomicAdd(&C->nnew[par][bid % SPREAD], bnew);
-            atomicAdd(&C->acc.vertices[bid % SPREAD], (unsigned long long)bnew);
-            // levels only grows: most workgroups see it already set
-            if (*(volatile int *)&C->acc.levels < level + 1) atomicMax(&C->acc.levels, level + 1);
-        }
-        if (bid == nblk - 1) {
-            const uint64_t g = s_prefix + total;
-            Ctl c;
-            c.nF = int(g >> 31);
-            c.E = int(g & 0x7fffffffu);
-            c.nTiles = (c.E + TILE - 1) / TILE;
-            const unsigned long long before = atomicAdd(&C->acc.edges, (unsigned long long)c.E);
-            c.pull = 0;
-            if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, before + c.E, C->ctl[par ^ 1].pull);
-            if constexpr (has_defer<Pick>::value) {
-                __threadfence();
-                // nothing left below the threshold: raise it past the smallest
-                // waiting distance and pick again next round
-                if (c.nF == 0 && __hip_atomic_load(&C->ndefer[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    C->thr = pk.raised(__hip_atomic_load(&C->dmin[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-                    c.pull = DIR_REPICK;
-                }
-            }
-            *next = c;
-            if (level < LOG_ROUNDS) C->log[level] = c;
-            if (level_arg < 0) C->cur_level = level + 1;
+    if (tid != 0 || done != unsigned((nblk - 1 - grp) / NGROUP + 1)) return;
+    // the last chunk of its group; the last such group finishes the round
+    if (atomicAdd(&C->alldone[par], 1u) + 1u != unsigned(min(nblk, NGROUP))) return;
+    __threadfence();
+    // every count this round made, loaded together
+    unsigned long long gc[NGROUP];
+    unsigned nn[SPREAD];
+#pragma unroll
+    for (int g = 0; g < NGROUP; ++g) gc[g] = __hip_atomic_load(&C->gcnt[par][g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int g = 0; g < SPREAD; ++g) nn[g] = __hip_atomic_load(&C->nnew[par][g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the running edge total has one writer per round: the round's last chunk
+    const unsigned long long before = __hip_atomic_load(&C->acc.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int prev_pull = C->ctl[par ^ 1].pull;
+    unsigned nd = 0;
+    uint32_t dm = ~0u;
+    if constexpr (has_defer<Pick>::value) {
+        nd = __hip_atomic_load(&C->ndefer[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        dm = __hip_atomic_load(&C->dmin[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    Ctl c{0, 0, 0, 0};
+    int tile0 = 0;
+    unsigned picked = 0;  // a round that picked anything counts as a level
+#pragma unroll
+    for (int g = 0; g < SPREAD; ++g) picked += nn[g];
+#pragma unroll
+    for (int g = 0; g < NGROUP; ++g) {
+        const int eg = int(gc[g] & 0xffffffffu);
+        C->gtile[par][g] = tile0;
+        tile0 += (eg + TILE - 1) / TILE;
+        c.nF += int(gc[g] >> 32);
+        c.E += eg;
+    }
+    C->gtile[par][NGROUP] = tile0;
+    c.nTiles = tile0;
+    __hip_atomic_store(&C->acc.edges, before + (unsigned long long)c.E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (picked && C->acc.levels < level + 1) C->acc.levels = level + 1;
+    if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, before + c.E, prev_pull);
+    if constexpr (has_defer<Pick>::value) {
+        // nothing left below the threshold: raise it past the smallest
+        // waiting distance and pick again next round
+        if (c.nF == 0 && nd) {
+            C->thr = pk.raised(dm);
+            c.pull = DIR_REPICK;
         }
     }
+    *next = c;
+    if (level < LOG_ROUNDS) C->log[level] = c;
+    if (level_arg < 0) C->cur_level = level + 1;
 }
 
 // Round 0 of a single-source traversal: what k_compact<Pick> computes when
@@ -302,7 +360,10 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
     const int start = row_ptr[seed], deg = row_ptr[seed + 1] - start;
     const int ntiles = (deg + TILE - 1) / TILE;
     for (int t = threadIdx.x; t < ntiles; t += WG) q.tiles[t] = 0;
+    // the entry sits in group 0; every later group starts past its tiles
+    if (threadIdx.x <= NGROUP) C->gtile[0][threadIdx.x] = threadIdx.x == 0 ? 0 : ntiles;
     if (threadIdx.x != 0) return;
+    C->gcnt[0][0] = deg > 0 ? (1ull << 32) | (unsigned long long)deg : 0ull;
     using PP = pick_phases<Pick>;
     uint32_t payload = 0;
     const typename PP::Pre pre = PP::load(pk, seed);
@@ -363,33 +424,45 @@ __device__ __forceinline__ void slot_max_scan(int (&x)[IPT], int32_t *s_wmax) {
 //   static constexpr bool kVert                    needs the entry's vertex id
 //   static constexpr bool kVal                     needs the entry's payload
 // Columns outside [0, m) are skipped.
-// Stages tile t of the queued entries' edges into LDS (entry start marks,
+// Stages tile t (in the round's numbering over the groups) of the queued
+// entries' edges into LDS (entry start marks,
 // col_ind base, optional payload and vertex id) and gives this thread the
 // owner entry of its IPT edge slots (slot = wid*SEG + r*64 + lane) through an
 // inclusive max-scan.  Returns the number of valid slots in the tile.
 template <bool KVAL, bool KVERT>
-__device__ __forceinline__ int tile_owners(int t, const Ctl &cur, const Queue &q, int32_t *s_own, int32_t *s_base,
+__device__ __forceinline__ int tile_owners(int t, const Round &cur, const Queue &q, int32_t *s_own, int32_t *s_base,
                                            uint32_t *s_val, int32_t *s_vert, int32_t *s_wmax, int (&own)[IPT]) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-    const int e0 = t * TILE;
-    const int f0 = q.tiles[t];
-    const int f1 = (t + 1 < cur.nTiles) ? q.tiles[t + 1] : cur.nF - 1;
+    // the group holding tile t: the last whose first tile is <= t (empty
+    // groups share their first tile with the next one)
+    int g = 0;
+#pragma unroll
+    for (int k = 1; k < NGROUP; ++k) g += cur.gtile[k] <= t ? 1 : 0;
+    const int lt = t - cur.gtile[g], ntg = cur.gtile[g + 1] - cur.gtile[g];
+    const unsigned long long gc = cur.gcnt[g];
+    const int nfg = int(gc >> 32), eg = int(gc & 0xffffffffu);
+    const size_t qo = size_t(g) * q.qcap;
+    const int32_t *tl = q.tiles + size_t(g) * q.tcap;
+    const int e0 = lt * TILE;
+    const int f0 = tl[lt];
+    const int f1 = (lt + 1 < ntg) ? tl[lt + 1] : nfg - 1;
     const int cnt = min(f1 - f0 + 1, TILE + 1);  // every queued entry has >= 1 edge
     for (int i = tid; i < TILE; i += WG) s_own[i] = 0;
     __syncthreads();
     for (int k = tid; k < cnt; k += WG) {
-        const int o = q.off[f0 + k] - e0;
-        s_base[k] = q.start[f0 + k] - o;
-        if (KVAL) s_val[k] = q.val[f0 + k];
-        if (KVERT) s_vert[k] = q.vert[f0 + k];
+        const size_t e = qo + f0 + k;
+        const int o = q.off[e] - e0;
+        s_base[k] = q.start[e] - o;
+        if (KVAL) s_val[k] = q.val[e];
+        if (KVERT) s_vert[k] = q.vert[e];
         if (o > 0 && o < TILE) s_own[o] = k;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < IPT; ++r) own[r] = s_own[wid * SEG + r * WAVE + lane];
     slot_max_scan(own, s_wmax);
-    return min(TILE, cur.E - e0);
+    return min(TILE, eg - e0);
 }
 
 // Slots whose loads an op issues before applying any (Op::kBatch when the
@@ -407,7 +480,7 @@ struct op_batch<T, std::void_t<decltype(T::kBatch)>> {
 // The tiles t = bid, bid + nblk, ... of the frontier `cur` (block-uniform
 // arguments; every thread of the workgroup calls it).
 template <class Op>
-__device__ __forceinline__ void expand_tiles(const Op &op, int m, const Ctl &cur, const Queue &q,
+__device__ __forceinline__ void expand_tiles(const Op &op, int m, const Round &cur, const Queue &q,
                                              const int32_t *__restrict__ col_ind, int bid, int nblk) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     __shared__ int32_t s_own[TILE];
@@ -451,7 +524,7 @@ template <class Op>
 __global__ __launch_bounds__(WG) void k_expand(Op op, int level_arg, int m, const Control *C, Queue q,
                                                const int32_t *__restrict__ col_ind) {
     const int level = level_arg >= 0 ? level_arg : C->cur_level;  // round of the next compaction
-    const Ctl cur = C->ctl[(level - 1) & 1];
+    const Round cur = round_of(C, level);
     if (cur.pull) return;  // this round runs as a pull step
     expand_tiles(op, m, cur, q, col_ind, blockIdx.x, gridDim.x);
 }

@@ -122,7 +122,7 @@ template <class Pick>
 void compact(apspie_ctx *ctx, const Bufs &b, const apspie_csr *g, const Pick &pk, int level) {
     ctx->prof_begin(APSPIE_KCLASS_COMPACT);
     k_compact<Pick><<<b.nblk, WGC, 0, ctx->stream>>>(pk, level, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
-                                                      b.status, b.C);
+                                                      b.C);
     ctx->prof_end();
 }
 

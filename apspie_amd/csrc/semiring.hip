@@ -215,7 +215,7 @@ void mxv(apspie_ctx *ctx, const apspie_csr *a, int sr, int presence, const float
     k_init<<<gi, 256, 0, ctx->stream>>>(sr, a->n, ysum, yord, reached, b.nblk, b.status, b.C);
     trav::k_compact<PickPresent><<<b.nblk, WGC, 0, ctx->stream>>>(PickPresent{x, listed, a->m, sr, presence}, 0, a->m,
                                                                     b.nblk, 1, a->row_ptr, trav::queue_of(b),
-                                                                    b.status, b.C);
+                                                                    b.C);
     trav::k_expand<OpSemiring><<<b.expand_grid, WG, 0, ctx->stream>>>(OpSemiring{a->val, sr, ysum, yord, reached},
                                                                       1, a->n, b.C, trav::queue_of(b), a->col_ind);
     k_finish<<<gi, 256, 0, ctx->stream>>>(sr, a->n, ysum, yord, y);

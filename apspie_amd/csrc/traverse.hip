@@ -90,15 +90,38 @@ struct PickSssp {
     int64_t nnz;
     int ratio;      // pull when the frontier's edges * ratio exceed nnz
     int bin_ratio;  // binned relaxation when the frontier's edges * bin_ratio exceed nnz (0: off)
-    // near/far: a changed vertex whose distance is at or past *thr waits
-    // (keeps its old snapshot, so it stays changed) until the frontier below
-    // the threshold runs dry and the threshold rises by delta (0: off)
-    const uint32_t *thr;
-    float delta;
     struct Pre {
         uint32_t d, p;
     };
     __device__ Pre load(int v) const { return v < m ? Pre{dist[v], prev[v]} : Pre{0u, 0u}; }
+    __device__ bool pick(int v, const Pre &x, uint32_t &payload) const {
+        if (v >= m) return false;
+        uint32_t d = x.d;
+        if (v == seed) {
+            d = 0u;
+            dist[v] = 0u;
+        }
+        payload = d;
+        const bool isnew = d < x.p;
+        if (isnew) prev[v] = d;
+        if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
+        return isnew;
+    }
+    __device__ void on_new(int) const {}
+    __device__ int direction(const Ctl &c, unsigned long long, int) const {
+        if (bin_ratio > 0 && int64_t(c.E) * bin_ratio > nnz) return DIR_BINNED;
+        if (fval && ratio > 0 && int64_t(c.E) * ratio > nnz) return DIR_PULL;
+        return DIR_PUSH;
+    }
+};
+
+// Near/far SSSP: a changed vertex whose distance is at or past *thr waits
+// (keeps its old snapshot, so it stays changed) until the frontier below the
+// threshold runs dry and the threshold rises by delta.  (A type of its own:
+// the plain compaction keeps fewer registers without the waiting counts.)
+struct PickSsspNF : PickSssp {
+    const uint32_t *thr;
+    float delta;
     __device__ uint32_t threshold() const { return __hip_atomic_load(thr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
     __device__ bool pick(int v, const Pre &x, uint32_t &payload) const {
         if (v >= m) return false;
@@ -108,14 +131,13 @@ struct PickSssp {
             dist[v] = 0u;
         }
         payload = d;
-        // near/far off: the threshold stays ~0u, so its load is skipped
-        const bool isnew = d < x.p && (!(delta > 0.0f) || d < threshold());
+        const bool isnew = d < x.p && d < threshold();
         if (isnew) prev[v] = d;
         if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
         return isnew;
     }
     __device__ uint32_t defer_key(int v, const Pre &x) const {
-        if (!(delta > 0.0f) || v >= m) return ~0u;
+        if (v >= m) return ~0u;
         const uint32_t d = v == seed ? 0u : x.d;
         return d < x.p && d >= threshold() ? d : ~0u;
     }
@@ -124,12 +146,6 @@ struct PickSssp {
     // that does not rise would admit nothing
     __device__ uint32_t raised(uint32_t dmin) const {
         return max(__float_as_uint(__uint_as_float(dmin) + delta), dmin + 1u);
-    }
-    __device__ void on_new(int) const {}
-    __device__ int direction(const Ctl &c, unsigned long long, int) const {
-        if (bin_ratio > 0 && int64_t(c.E) * bin_ratio > nnz) return DIR_BINNED;
-        if (fval && ratio > 0 && int64_t(c.E) * ratio > nnz) return DIR_PULL;
-        return DIR_PUSH;
     }
 };
 
@@ -207,7 +223,7 @@ __global__ __launch_bounds__(WG) void k_round_bfs(OpBfs op, int level_arg, int m
                                                   const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
                                                   const uint32_t *__restrict__ fbits) {
     const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    const Ctl cur = C->ctl[(level - 1) & 1];
+    const Round cur = round_of(C, level);
     if (cur.pull == DIR_PULL) {
         pull_bfs_rows(m, cp, ri, op.visited, fbits, op.flags, blockIdx.x, gridDim.x);
         return;
@@ -351,18 +367,27 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
 // ----------------------------------------------------------------------------
 // host side
 
+// Entries per group region of the queue: a group's chunks hold at most
+// CHUNK picked vertices each.  Tiles per group region: a group holds at most
+// every edge.
+int32_t group_entries(int nblk) { return int32_t(ceil_div(nblk, NGROUP)) * CHUNK; }
+int32_t group_tiles(int64_t nnz) { return int32_t(nnz / TILE + 2); }
+
 Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz) {
     Bufs b;
     b.nblk = std::max(1, (int)ceil_div(m, CHUNK));
     const size_t mpad = size_t(b.nblk) * CHUNK;
+    b.qcap = group_entries(b.nblk);
+    b.tcap = group_tiles(nnz);
+    const size_t qn = size_t(NGROUP) * size_t(b.qcap);
     b.flags = (uint8_t *)ctx->slot(S_FLAGS, mpad);
     b.visited = (uint32_t *)ctx->slot(S_VISITED, mpad / 8);
     b.prev = (uint32_t *)ctx->slot(S_PREV, (size_t(m) + 1) * 4);
-    b.queue = (int32_t *)ctx->slot(S_QUEUE, (size_t(m) + 1) * 4);
-    b.qstart = (int32_t *)ctx->slot(S_QSTART, (size_t(m) + 1) * 4);
-    b.qoff = (int32_t *)ctx->slot(S_QOFF, (size_t(m) + 1) * 4);
-    b.qval = (float *)ctx->slot(S_QVAL, (size_t(m) + 1) * 4);
-    b.tiles = (int32_t *)ctx->slot(S_TILES, (size_t(nnz) / TILE + 4) * 4);
+    b.queue = (int32_t *)ctx->slot(S_QUEUE, qn * 4);
+    b.qstart = (int32_t *)ctx->slot(S_QSTART, qn * 4);
+    b.qoff = (int32_t *)ctx->slot(S_QOFF, qn * 4);
+    b.qval = (float *)ctx->slot(S_QVAL, qn * 4);
+    b.tiles = (int32_t *)ctx->slot(S_TILES, size_t(NGROUP) * size_t(b.tcap) * 4);
     b.status = (uint64_t *)ctx->slot(S_STATUS, size_t(b.nblk) * 2 * 8);
     b.C = (Control *)ctx->slot(S_CTL, sizeof(Control));
     b.expand_grid =
@@ -370,7 +395,7 @@ Bufs get_bufs(apspie_ctx *ctx, int m, int64_t nnz) {
     return b;
 }
 
-Queue queue_of(const Bufs &b) { return Queue{b.queue, b.qstart, b.qoff, (uint32_t *)b.qval, b.tiles}; }
+Queue queue_of(const Bufs &b) { return Queue{b.queue, b.qstart, b.qoff, (uint32_t *)b.qval, b.tiles, b.qcap, b.tcap}; }
 
 // ----------------------------------------------------------------------------
 // fixed scratch of the allocation-free SSSP (apspie_sssp_noalloc): one array
@@ -400,12 +425,14 @@ void noalloc_setup(int device) {
 }
 
 // Bytes a single-source SSSP over (m, nnz) carves from one block: distance
-// snapshots, the four queue arrays, the tile table, the look-back status and
-// the control block, each 256-byte aligned.
+// snapshots, the four queue arrays (a region per chunk group), the tile
+// tables, the look-back status and the control block, each 256-byte aligned.
 size_t sssp_carve_bytes(int m, int64_t nnz) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
-    const size_t nblk = size_t(std::max(1, (int)ceil_div(m, CHUNK)));
-    return 5 * al((size_t(m) + 1) * 4) + al((size_t(nnz) / TILE + 4) * 4) + al(nblk * 2 * 8) + al(sizeof(Control));
+    const int nblk = std::max(1, (int)ceil_div(m, CHUNK));
+    const size_t qn = size_t(NGROUP) * size_t(group_entries(nblk));
+    return al((size_t(m) + 1) * 4) + 4 * al(qn * 4) + al(size_t(NGROUP) * size_t(group_tiles(nnz)) * 4) +
+           al(size_t(nblk) * 2 * 8) + al(sizeof(Control));
 }
 
 // The SSSP buffers carved from `base` (sssp_carve_bytes(m, nnz) bytes); the
@@ -420,12 +447,15 @@ Bufs sssp_carve(apspie_ctx *ctx, unsigned char *base, int m, int64_t nnz) {
     };
     Bufs b;
     b.nblk = std::max(1, (int)ceil_div(m, CHUNK));
+    b.qcap = group_entries(b.nblk);
+    b.tcap = group_tiles(nnz);
+    const size_t qn = size_t(NGROUP) * size_t(b.qcap);
     b.prev = (uint32_t *)take((size_t(m) + 1) * 4);
-    b.queue = (int32_t *)take((size_t(m) + 1) * 4);
-    b.qstart = (int32_t *)take((size_t(m) + 1) * 4);
-    b.qoff = (int32_t *)take((size_t(m) + 1) * 4);
-    b.qval = (float *)take((size_t(m) + 1) * 4);
-    b.tiles = (int32_t *)take((size_t(nnz) / TILE + 4) * 4);
+    b.queue = (int32_t *)take(qn * 4);
+    b.qstart = (int32_t *)take(qn * 4);
+    b.qoff = (int32_t *)take(qn * 4);
+    b.qval = (float *)take(qn * 4);
+    b.tiles = (int32_t *)take(size_t(NGROUP) * size_t(b.tcap) * 4);
     b.status = (uint64_t *)take(size_t(b.nblk) * 2 * 8);
     b.C = (Control *)take(sizeof(Control));
     b.expand_grid =
@@ -468,7 +498,7 @@ template <class Pick>
 void launch_compact(apspie_ctx *ctx, const Bufs &b, const apspie_csr *g, const Pick &pk, int level, int always) {
     ctx->prof_begin(APSPIE_KCLASS_COMPACT);
     k_compact<Pick><<<b.nblk, WGC, 0, ctx->stream>>>(pk, level, g->m, b.nblk, always, g->row_ptr, queue_of(b),
-                                                      b.status, b.C);
+                                                      b.C);
     ctx->prof_end();
 }
 
@@ -536,9 +566,20 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                            PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}};
         else
             return PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
-                            binned ? ctx->tune.sssp_bin_ratio : 0, &b.C->thr, delta};
+                            binned ? ctx->tune.sssp_bin_ratio : 0};
     };
-    auto compact = [&](int level) { launch_compact(ctx, b, g, pick_for(level, -1), level, 0); };
+    auto near_far = [&](const PickSssp &pk) {
+        PickSsspNF nf;
+        static_cast<PickSssp &>(nf) = pk;
+        nf.thr = &b.C->thr;
+        nf.delta = delta;
+        return nf;
+    };
+    auto compact = [&](int level) {
+        if constexpr (MODE == MODE_SSSP)
+            if (delta > 0.0f) return launch_compact(ctx, b, g, near_far(pick_for(level, -1)), level, 0);
+        launch_compact(ctx, b, g, pick_for(level, -1), level, 0);
+    };
     auto expand = [&](int level) {
         if (MODE == MODE_BFS && pull) {
             ctx->prof_begin(APSPIE_KCLASS_ROUND);
@@ -572,7 +613,14 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     {
         const auto pk0 = pick_for(0, src);
         ctx->prof_begin(APSPIE_KCLASS_COMPACT);
-        k_seed<std::decay_t<decltype(pk0)>><<<1, WG, 0, ctx->stream>>>(pk0, src, g->row_ptr, queue_of(b), b.C);
+        bool seeded = false;
+        if constexpr (MODE == MODE_SSSP)
+            if (delta > 0.0f) {
+                k_seed<PickSsspNF><<<1, WG, 0, ctx->stream>>>(near_far(pk0), src, g->row_ptr, queue_of(b), b.C);
+                seeded = true;
+            }
+        if (!seeded)
+            k_seed<std::decay_t<decltype(pk0)>><<<1, WG, 0, ctx->stream>>>(pk0, src, g->row_ptr, queue_of(b), b.C);
         ctx->prof_end();
     }
     // chunks of 4, 2, 2, 4, 8, ... rounds between frontier checks: RMAT
@@ -646,7 +694,7 @@ int apspie_sssp_noalloc(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float
         const size_t need = sssp_carve_bytes(g->m, g->nnz);
         APSPIE_CHECK(need <= NOALLOC_BYTES, APSPIE_ELIMIT,
                      "apspie_sssp_noalloc: the graph needs " + std::to_string(need) + " bytes of scratch, the fixed "
-                     "scratch holds " + std::to_string(NOALLOC_BYTES) + " (about 20 B per vertex + 2 B per 1024 "
+                     "scratch holds " + std::to_string(NOALLOC_BYTES) + " (about 20 B per vertex + 32 B per 1024 "
                      "edges); use apspie_sssp");
         APSPIE_CHECK(ctx->device < MAX_DEVICES && g_noalloc_ptr[ctx->device] != nullptr, APSPIE_EHIP,
                      "fixed scratch not set up on this device");

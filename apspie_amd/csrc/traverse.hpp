@@ -50,9 +50,23 @@ struct Acc {
 
 constexpr int LOG_ROUNDS = 64;  // rounds whose frontier descriptor is kept for apspie_ctx_rounds
 
+// The frontier a compaction builds is split by chunk group: vertex chunk b
+// (CHUNK vertices, one compaction workgroup) belongs to group b % NGROUP, and
+// each group fills its own region of the queue and of the tile table.  A
+// chunk takes its place in its group with one returning atomic add on the
+// group's counter, so no workgroup waits for another and the counters are
+// spread over NGROUP addresses (one address would serialise every
+// workgroup's atomic).  Entry order inside a group is the order the chunks
+// arrived in; every frontier algorithm here is order-independent.
+constexpr int NGROUP = SPREAD;
+
 struct Control {
-    Ctl ctl[2];                 // by round parity
-    unsigned int blk_counter[2];  // dynamic workgroup ids, by round parity
+    Ctl ctl[2];                 // by round parity: totals over the groups
+    // per round parity and group: (entries << 32) | edges, and chunks done
+    unsigned long long gcnt[2][NGROUP];
+    unsigned int gdone[2][NGROUP];
+    unsigned int alldone[2];    // groups whose chunks are all done
+    int32_t gtile[2][NGROUP + 1];  // first tile of each group in the round's tile numbering
     unsigned int nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
     Acc acc;
     int cur_level;  // device-driven rounds: the round the next compaction runs
@@ -74,6 +88,7 @@ struct Bufs {
     uint32_t *fbits = nullptr;  // frontier bitmap written by each compaction (pull steps only)
     int32_t *trows = nullptr;   // SSSP pull: CSC row holding the first edge of each tile
     int32_t *queue = nullptr, *qstart = nullptr, *qoff = nullptr, *tiles = nullptr;
+    int32_t qcap = 0, tcap = 0;  // entries and tiles per group region
     float *qval = nullptr;
     uint64_t *status = nullptr;
     Control *C = nullptr;

@@ -295,7 +295,7 @@ int apspie_sssp(apspie_ctx *ctx, const apspie_csr *g, int32_t src, float *d_dist
  * is a fixed 48 MiB device array of the library (one per GPU, looked up when
  * a context is created and held until the process ends) instead of the
  * context's pool, so the graph may need at most 48 MiB of it: about 20 B per
- * vertex plus 2 B per 1024 edges (m up to about 2.4 million); larger graphs
+ * vertex plus 32 B per 1024 edges (m up to about 2.3 million); larger graphs
  * get APSPIE_ELIMIT.  Push rounds only (the pull and binned knobs are
  * ignored; results and counters are apspie_sssp's).  Per-kernel timing is
  * off for the call.  Calls on one GPU share its array: the library
