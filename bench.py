@@ -469,6 +469,25 @@ def profile_workload(args):
         info.update(m=h.m, nnz=h.nnz, rounds=rounds, device_ms=st.kernel_ms, stats=st.as_dict(),
                     class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz) if w.startswith("sssp")
                                  else bfs_class_bytes(rounds, h.m)))
+    elif w in ("bfs_batch_s20_k256", "sssp_batch_s20_k64"):
+        import ctypes as Ct
+
+        bfs_b = w.startswith("bfs")
+        seed = 4020 if bfs_b else 4120
+        h = A.rmat(20, 16, seed, undirected=bfs_b, weight_seed=None if bfs_b else seed + 1_000_000)
+        g = ctx.upload(h)
+        k = 256 if bfs_b else 64
+        srcs = batch_sources(h, k, seed)
+        dsrc = ctx.alloc(4 * k).upload(srcs)
+        dout = ctx.alloc(4 * k * h.m)
+        fn = A.lib().apspie_bfs_batch if bfs_b else A.lib().apspie_sssp_batch
+        walls = []
+        for _ in range(calls):
+            st = A.Stats()
+            t0 = time.perf_counter()
+            assert fn(ctx.h, Ct.byref(g.c), dsrc.ptr, k, dout.ptr, Ct.byref(st)) == 0
+            walls.append((time.perf_counter() - t0) * 1e3)
+        info.update(stats=st.as_dict(), wall_ms=walls)
     elif w == "spgemm_powerlaw_1M":
         h = A.powerlaw(1 << 20, 1 << 20, 8.0, 4096, 3003)
         a = ctx.upload(h)
@@ -513,7 +532,8 @@ def main():
     ap.add_argument("--copy-only", action="store_true",
                     help="run only the 1 GiB device copy (counter calibration) and exit")
     ap.add_argument("--profile-workload", choices=["sssp_rmat_s22", "bfs_rmat_s22", "spgemm_powerlaw_1M",
-                                                   "tc_rmat_s20", "mis_rmat_s21"],
+                                                   "tc_rmat_s20", "mis_rmat_s21", "bfs_batch_s20_k256",
+                                                   "sssp_batch_s20_k64"],
                     help="run one workload alone (for rocprofv3) and exit")
     ap.add_argument("--tuning", help="JSON object of tuning knobs for --profile-workload")
     args = ap.parse_args()
