@@ -134,10 +134,7 @@ __global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *sr
     for (int64_t i = gtid; i < int64_t(m) * 2 * W; i += stride) st[i] = 0;
     for (int64_t i = gtid; i < int64_t(m) * k; i += stride) depth[i] = -1;
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
-    if (gtid == 0) {
-        Control z{};
-        *C = z;
-    }
+    control_clear(C, gtid, stride, ~0u);
 }
 
 template <int W>
@@ -266,10 +263,7 @@ __global__ __launch_bounds__(256) void k_mss_init(int m, int k, const int32_t *s
     for (int64_t i = gtid; i < int64_t(m) * K; i += stride) dist[i] = INF_BITS;
     for (int64_t i = gtid; i < m; i += stride) chg[i] = 0;
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
-    if (gtid == 0) {
-        Control z{};
-        *C = z;
-    }
+    control_clear(C, gtid, stride, ~0u);
 }
 
 template <int K>
@@ -329,7 +323,10 @@ int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &base_key, int 
     for (;;) {
         APSPIE_HIP(hipGraphLaunch(exec, ctx->stream));
         rounds += CH;
-        APSPIE_HIP(hipMemcpyAsync(hc, b.C, sizeof(Control), hipMemcpyDeviceToHost, ctx->stream));
+        // the descriptors and the round counter lead the control block
+        static_assert(offsetof(Control, cur_level) + sizeof(int) <= 4096, "fits the pinned staging");
+        APSPIE_HIP(hipMemcpyAsync(hc, b.C, offsetof(Control, cur_level) + sizeof(int), hipMemcpyDeviceToHost,
+                                  ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         // frontier written by the last compaction that ran (an exhausted
         // traversal stops advancing cur_level, see k_compact)

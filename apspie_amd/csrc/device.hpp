@@ -83,6 +83,15 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
     return uint32_t(wave_last(int(x)));
 }
 
+// Orders this thread's returning atomics, whose results were folded into
+// dep, before every later memory operation of the thread: the value of a
+// returning atomic is back only once memory has performed it, and the empty
+// asm both needs dep (so the wave waits for those values) and is a compiler
+// barrier.  The "last workgroup finishes" pattern needs no more when the last
+// workgroup reads only such counters; a device-scope fence would also do, but
+// on gfx950 it writes back and invalidates the whole L2.
+__device__ __forceinline__ void after_values(uint32_t dep) { asm volatile("" ::"v"(dep) : "memory"); }
+
 // Sum across the wave; every lane receives the total.
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
 #pragma unroll

@@ -24,6 +24,7 @@
 // (unless told to always run), so the host can launch rounds speculatively.
 #pragma once
 
+#include <cstddef>
 #include <type_traits>
 
 #include "device.hpp"
@@ -50,7 +51,7 @@ struct Queue {
 // the round's tile numbering and its (entries << 32) | edges.
 struct Round : Ctl {
     const int32_t *gtile;
-    const unsigned long long *gcnt;
+    const Padded<unsigned long long> *gcnt;
 };
 __device__ __forceinline__ Round round_of(const Control *C, int level) {
     const int pp = (level - 1) & 1;
@@ -59,6 +60,25 @@ __device__ __forceinline__ Round round_of(const Control *C, int level) {
     r.gtile = C->gtile[pp];
     r.gcnt = C->gcnt[pp];
     return r;
+}
+
+// Clears the control block with every thread of an initialisation launch
+// (thread gtid of stride, over 16-byte words), then the near/far threshold is
+// thr0 (all ones: none) and the smallest waiting distances all ones.
+__device__ __forceinline__ void control_clear(Control *C, int64_t gtid, int64_t stride, uint32_t thr0) {
+    static_assert(sizeof(Control) % 16 == 0, "the control block is a whole number of 16-byte words");
+    constexpr int64_t n = int64_t(sizeof(Control) / 16);
+    constexpr int64_t ithr = int64_t(offsetof(Control, thr) / 4), idmin = int64_t(offsetof(Control, dmin) / 4);
+    uint4 *w = (uint4 *)C;
+    for (int64_t i = gtid; i < n; i += stride) {
+        uint32_t x[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t k = 4 * i + j;
+            x[j] = k == ithr ? thr0 : (k == idmin || k == idmin + 1) ? ~0u : 0u;
+        }
+        w[i] = make_uint4(x[0], x[1], x[2], x[3]);
+    }
 }
 
 // host helpers shared by the frontier algorithms (traverse.hip)
@@ -149,10 +169,12 @@ __global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     // counters of the round after this one (their last readers, the
     // expansion of this round among them, have finished)
     if (bid == 0 && tid < NGROUP) {
-        C->gcnt[par ^ 1][tid] = 0;
-        C->gdone[par ^ 1][tid] = 0;
-        C->nnew[par ^ 1][tid] = 0;
+        C->gcnt[par ^ 1][tid].v = 0;
+        C->gdone[par ^ 1][tid].v = 0;
+        C->nnew[par ^ 1][tid].v = 0;
+        C->rfound[par ^ 1][tid].v = 0;
         if (tid == 0) {
+            C->qoverflow[par ^ 1] = 0;
             C->alldone[par ^ 1] = 0;
             C->ndefer[par ^ 1] = 0;
             C->dmin[par ^ 1] = ~0u;
@@ -236,10 +258,11 @@ __global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
         total += s_wsum[w];
         bnew += s_new[w];
     }
+    // every count of this chunk is a returning atomic whose value is awaited
+    // before the chunk counts itself done, so the last workgroup sees them all
+    uint32_t dep = 0;
     if (tid == 0) {
         if constexpr (has_defer<Pick>::value) {
-            // waiting vertices are counted before this workgroup's completion,
-            // so the last workgroup sees every count
             uint32_t bdef = 0, bmin = ~0u;
 #pragma unroll
             for (int w = 0; w < NW; ++w) {
@@ -247,25 +270,25 @@ __global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
                 bmin = min(bmin, s_dmin[w]);
             }
             if (bdef) {
-                atomicAdd(&C->ndefer[par], bdef);
-                atomicMin(&C->dmin[par], bmin);
+                dep ^= atomicAdd(&C->ndefer[par], bdef);
+                dep ^= atomicMin(&C->dmin[par], bmin);
             }
         }
         if (bnew) {
-            atomicAdd(&C->nnew[par][bid % SPREAD], bnew);
-            atomicAdd(&C->acc.vertices[bid % SPREAD], (unsigned long long)bnew);
+            dep ^= atomicAdd(&C->nnew[par][bid % SPREAD].v, bnew);
+            dep ^= uint32_t(atomicAdd(&C->acc.vertices[bid % SPREAD].v, (unsigned long long)bnew));
         }
         // this chunk's place in its group: the entries and edges before it
         // (a chunk that queues nothing needs none)
-        s_base = total ? atomicAdd(&C->gcnt[par][grp], (unsigned long long)total) : 0ull;
+        s_base = total ? atomicAdd(&C->gcnt[par][grp].v, (unsigned long long)total) : 0ull;
     }
     __syncthreads();
-    // this chunk's counts are all in: count it done (the return is awaited
-    // only after this wave's writes, which hide the round trip)
+    // count the chunk done (the return is awaited only after this wave's
+    // writes, which hide the round trip)
     unsigned done = 0;
     if (tid == 0) {
-        __threadfence();
-        done = atomicAdd(&C->gdone[par][grp], 1u) + 1u;
+        after_values(dep);
+        done = atomicAdd(&C->gdone[par][grp].v, 1u) + 1u;
     }
     const uint64_t base = s_base + wpre;
     int pos = int(base >> 32);
@@ -299,14 +322,14 @@ __global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     if (tid != 0 || done != unsigned((nblk - 1 - grp) / NGROUP + 1)) return;
     // the last chunk of its group; the last such group finishes the round
     if (atomicAdd(&C->alldone[par], 1u) + 1u != unsigned(min(nblk, NGROUP))) return;
-    __threadfence();
-    // every count this round made, loaded together
+    // every count this round made, loaded together (atomic loads: from where
+    // the atomics were performed)
     unsigned long long gc[NGROUP];
     unsigned nn[SPREAD];
 #pragma unroll
-    for (int g = 0; g < NGROUP; ++g) gc[g] = __hip_atomic_load(&C->gcnt[par][g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int g = 0; g < NGROUP; ++g) gc[g] = __hip_atomic_load(&C->gcnt[par][g].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
-    for (int g = 0; g < SPREAD; ++g) nn[g] = __hip_atomic_load(&C->nnew[par][g], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int g = 0; g < SPREAD; ++g) nn[g] = __hip_atomic_load(&C->nnew[par][g].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the running edge total has one writer per round: the round's last chunk
     const unsigned long long before = __hip_atomic_load(&C->acc.edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int prev_pull = C->ctl[par ^ 1].pull;
@@ -363,7 +386,7 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
     // the entry sits in group 0; every later group starts past its tiles
     if (threadIdx.x <= NGROUP) C->gtile[0][threadIdx.x] = threadIdx.x == 0 ? 0 : ntiles;
     if (threadIdx.x != 0) return;
-    C->gcnt[0][0] = deg > 0 ? (1ull << 32) | (unsigned long long)deg : 0ull;
+    C->gcnt[0][0].v = deg > 0 ? (1ull << 32) | (unsigned long long)deg : 0ull;
     using PP = pick_phases<Pick>;
     uint32_t payload = 0;
     const typename PP::Pre pre = PP::load(pk, seed);
@@ -376,8 +399,8 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
         q.off[0] = 0;
         q.val[0] = payload;
     }
-    C->nnew[0][0] = 1;
-    C->acc.vertices[0] = 1;
+    C->nnew[0][0].v = 1;
+    C->acc.vertices[0].v = 1;
     C->acc.levels = 1;
     C->acc.edges = (unsigned long long)deg;
     Ctl c;
@@ -440,7 +463,7 @@ __device__ __forceinline__ int tile_owners(int t, const Round &cur, const Queue 
 #pragma unroll
     for (int k = 1; k < NGROUP; ++k) g += cur.gtile[k] <= t ? 1 : 0;
     const int lt = t - cur.gtile[g], ntg = cur.gtile[g + 1] - cur.gtile[g];
-    const unsigned long long gc = cur.gcnt[g];
+    const unsigned long long gc = cur.gcnt[g].v;
     const int nfg = int(gc >> 32), eg = int(gc & 0xffffffffu);
     const size_t qo = size_t(g) * q.qcap;
     const int32_t *tl = q.tiles + size_t(g) * q.tcap;

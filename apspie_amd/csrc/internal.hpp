@@ -56,6 +56,10 @@ enum Slot : int {
     S_BINOUT,      // uint2 per frontier edge: binned SSSP candidates
     S_BINTBL,      // int32 per (bin workgroup, bucket): offsets of the binned runs
     S_BINTOT,      // uint32 per (parity, bucket): binned entries per bucket
+    S_QUEUE1,      // BFS with a CSC: the queue arrays of odd rounds
+    S_QSTART1,
+    S_QOFF1,
+    S_TILES1,
     S_AUX0,
     S_AUX1,
     S_AUX2,
@@ -98,6 +102,7 @@ struct apspie_ctx {
         int sssp_delta = 0;
         int spgemm_long_rows = 1;
         int spgemm_passes = 1;
+        int bfs_group_cap = 0;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

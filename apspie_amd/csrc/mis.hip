@@ -107,10 +107,7 @@ __global__ __launch_bounds__(WG) void k_mis_init(int m, int nblk, uint8_t *state
         if (out && i < m) out[i] = 0;
     }
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
-    if (gtid == 0) {
-        Control z{};
-        *C = z;
-    }
+    control_clear(C, gtid, stride, ~0u);
 }
 
 __global__ __launch_bounds__(WG) void k_mis_finish(int m, const uint8_t *state, int32_t *out) {
@@ -158,7 +155,8 @@ int apspie_mis(apspie_ctx *ctx, const apspie_csr *g, const uint32_t *d_prio, int
         k_mis_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(g->m, b.nblk, state, blocked, nullptr, b.status,
                                                                b.C);
         ctx->prof_end();
-        volatile uint32_t *host = (volatile uint32_t *)ctx->pinned;
+        static_assert(sizeof(Control::nnew[0]) <= 4096, "the joined counts fit the pinned staging");
+        volatile Padded<uint32_t> *host = (volatile Padded<uint32_t> *)ctx->pinned;
         int level = 0, rounds = 0, chunk = 4;
         for (;;) {
             for (int i = 0; i < chunk; ++i, ++rounds) {
@@ -169,11 +167,11 @@ int apspie_mis(apspie_ctx *ctx, const apspie_csr *g, const uint32_t *d_prio, int
             }
             APSPIE_HIP(hipGetLastError());
             // vertices that joined in the last round; none => nothing undecided
-            APSPIE_HIP(hipMemcpyAsync((void *)host, &b.C->nnew[(level - 1) & 1][0], 4 * SPREAD,
+            APSPIE_HIP(hipMemcpyAsync((void *)host, &b.C->nnew[(level - 1) & 1][0], sizeof(b.C->nnew[0]),
                                       hipMemcpyDeviceToHost, ctx->stream));
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
             uint32_t joined = 0;
-            for (int i = 0; i < SPREAD; ++i) joined += host[i];
+            for (int i = 0; i < SPREAD; ++i) joined += host[i].v;
             if (joined == 0) break;
             APSPIE_CHECK(rounds <= g->m + 1, APSPIE_ENOCONV, "MIS did not converge");
             chunk = std::min(chunk * 2, 16);

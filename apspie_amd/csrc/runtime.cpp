@@ -150,7 +150,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
         t->batch_chunk < 1 || t->batch_chunk > 16 ||
         (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64) ||
         t->sssp_delta < 0 || t->sssp_delta > (1 << 24) || (t->spgemm_long_rows != 0 && t->spgemm_long_rows != 1) ||
-        (t->spgemm_passes != 1 && t->spgemm_passes != 2)) {
+        (t->spgemm_passes != 1 && t->spgemm_passes != 2) || t->bfs_group_cap < 0 || t->bfs_group_cap > (1 << 30)) {
         ctx->err = "tuning knob out of range";
         return APSPIE_EINVAL;
     }
@@ -166,6 +166,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.sssp_delta = t->sssp_delta;
     ctx->tune.spgemm_long_rows = t->spgemm_long_rows;
     ctx->tune.spgemm_passes = t->spgemm_passes;
+    ctx->tune.bfs_group_cap = t->bfs_group_cap;
     return APSPIE_OK;
 }
 
@@ -183,6 +184,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->sssp_delta = ctx->tune.sssp_delta;
     t->spgemm_long_rows = ctx->tune.spgemm_long_rows;
     t->spgemm_passes = ctx->tune.spgemm_passes;
+    t->bfs_group_cap = ctx->tune.bfs_group_cap;
     return APSPIE_OK;
 }
 

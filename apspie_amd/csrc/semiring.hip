@@ -116,10 +116,7 @@ __global__ __launch_bounds__(256) void k_init(int sr, int n, double *ysum, uint3
         if (reached) reached[i] = 0;
     }
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
-    if (gtid == 0) {
-        Control z{};
-        *C = z;
-    }
+    control_clear(C, gtid, stride, ~0u);
 }
 
 __global__ __launch_bounds__(256) void k_finish(int sr, int n, const double *ysum, const uint32_t *yord, float *y) {

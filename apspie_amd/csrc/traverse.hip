@@ -54,7 +54,9 @@ struct PickBfs {
     int level, seed;
     PullRuleBfs rule;
     using Pre = uint8_t;
-    __device__ Pre load(int v) const { return flags[v]; }  // flags are padded to nblk * CHUNK
+    // flags are padded to nblk * CHUNK; null when the rounds are k_round_bfs
+    // ones (only the seed kernel runs this pick then)
+    __device__ Pre load(int v) const { return flags ? flags[v] : Pre(0); }
     __device__ bool pick(int v, Pre f, uint32_t &) const {
         if (f) flags[v] = 0;
         return f != 0 || v == seed;
@@ -193,44 +195,336 @@ struct OpRelax {
 // pull steps (rounds whose compaction set Ctl::pull), over the CSC: cp[m+1]
 // column pointers, ri[nnz] source rows, cv[nnz] values
 
-// BFS bottom-up: every unvisited vertex scans its in-edges for a parent in
-// the frontier bitmap and stops at the first one.  On RMAT graphs nearly all
-// vertices find one at their first in-edge (the hubs, lowest ids, come
-// first), so a thread per vertex keeps the wave converged.
-__device__ __forceinline__ void pull_bfs_rows(int m, const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
-                                              const uint32_t *__restrict__ visited,
-                                              const uint32_t *__restrict__ fbits, uint8_t *flags, int bid, int nblk) {
-    for (int v = bid * WG + threadIdx.x; v < m; v += nblk * WG) {
-        if ((visited[v >> 5] >> (v & 31)) & 1u) continue;
-        const int a = cp[v], b = cp[v + 1];
-        for (int e = a; e < b; ++e) {
-            const int u = ri[e];
-            if ((unsigned)u < (unsigned)m && ((fbits[u >> 5] >> (u & 31)) & 1u)) {
-                flags[v] = 1;
-                break;
-            }
+// BFS with a CSC (rounds may run bottom-up).  One launch per round
+// (k_round_bfs) and no pass over the vertex range: a vertex the round reaches
+// gets its depth and visited bit there, enters the round's frontier bitmap
+// (the input of a bottom-up successor) and is appended to the round's queue
+// (the input of a top-down successor); the workgroup that finishes last
+// writes the round's descriptor.  Queues alternate between two sets of
+// arrays by round parity, since a round reads its predecessor's queue while
+// it writes its own.
+struct BfsRound {
+    uint32_t *visited;
+    int32_t *depth;
+    uint32_t *fb;  // three frontier bitmaps of fbw words, round L's in bitmap L mod 3
+    int fbw;
+    const int32_t *row_ptr;
+    PullRuleBfs rule;
+    int cap;  // entries a round may append to one group region (<= the region's size)
+};
+
+// Lanes of a wave with equal adjacent keys form runs.  Returns the OR of x
+// over this lane's run up to this lane; excl gets the OR over the run's lanes
+// before this one, tail the run's last lane.  Every lane of the wave calls it.
+__device__ __forceinline__ uint32_t run_or(int key, uint32_t x, int lane, uint32_t &excl, int &tail) {
+    const int kprev = __shfl_up(key, 1);
+    const uint64_t heads = __ballot(lane == 0 || kprev != key);
+    const int h = 63 - __clzll((long long)(heads & (~0ull >> (63 - lane))));  // this run's first lane
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d);
+        if (lane - d >= h) x |= y;
+    }
+    const uint32_t p = __shfl_up(x, 1);
+    excl = lane - 1 >= h ? p : 0u;
+    const uint64_t tails = (heads >> 1) | (1ull << 63);
+    tail = __ffsll((long long)(tails & (~0ull << lane))) - 1;
+    return x;
+}
+
+// Appends the vertices this thread holds (get(j), j < K, -1: none) with
+// out-degree > 0 to group grp's region of queue qn: one returning atomic add
+// of the workgroup's (entries, edges) on the group's counter, entries in
+// thread order.  A workgroup whose entries do not fit the region's first cap
+// entries writes none and raises the overflow flag (its counts still go to
+// the counter: the totals stay exact; the flag's atomic is folded into dep).
+// s_scan holds NW + 1 words; contains barriers.
+template <int K, class Get>
+__device__ __forceinline__ void append_entries(const Get &get, const int32_t *__restrict__ row_ptr, const Queue &qn,
+                                               int cap, Padded<unsigned long long> *gcnt, int grp,
+                                               unsigned *overflow, uint64_t *s_scan, uint32_t &dep) {
+    constexpr int NW = WG / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    uint32_t cnt = 0, es = 0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int v = get(j);
+        if (v >= 0) {
+            const int d = row_ptr[v + 1] - row_ptr[v];
+            cnt += d > 0;
+            es += d;
         }
+    }
+    const uint64_t x = (uint64_t(cnt) << 32) | es;
+    const uint64_t incl = wave_incl_scan(x, lane);
+    if (lane == WAVE - 1) s_scan[wid] = incl;
+    __syncthreads();
+    uint64_t wpre = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w < wid) wpre += s_scan[w];
+        tot += s_scan[w];
+    }
+    __syncthreads();
+    if (tid == 0) s_scan[NW] = tot ? atomicAdd(&gcnt[grp].v, tot) : 0ull;
+    __syncthreads();
+    const uint64_t gb = s_scan[NW];
+    __syncthreads();  // s_scan is free again for the caller
+    if (!tot) return;
+    if ((gb >> 32) + (tot >> 32) > uint64_t(cap)) {
+        if (tid == 0) dep ^= atomicOr(overflow, 1u);
+        return;
+    }
+    const uint64_t base = gb + wpre + incl - x;
+    int pos = int(base >> 32), off = int(base & 0xffffffffu);
+    const size_t qo = size_t(grp) * qn.qcap;
+    int32_t *const tl = qn.tiles + size_t(grp) * qn.tcap;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const int v = get(j);
+        if (v < 0) continue;
+        const int r0 = row_ptr[v], d = row_ptr[v + 1] - r0;
+        if (d <= 0) continue;
+        qn.vert[qo + pos] = v;
+        qn.start[qo + pos] = r0;
+        qn.off[qo + pos] = off;
+        for (int t = (off + TILE - 1) / TILE; t * TILE < off + d; ++t) tl[t] = pos;
+        ++pos;
+        off += d;
     }
 }
 
-// One BFS round in one launch: the bottom-up step over every block when the
-// compaction chose it, else the push expansion over the first expand_blocks
-// blocks (the rest exit).  Separate expansion and bottom-up launches would
-// leave one of the two with nothing to do every round, and a launch of a few
-// thousand workgroups that exit at once still costs several microseconds.
-__global__ __launch_bounds__(WG) void k_round_bfs(OpBfs op, int level_arg, int m, int expand_blocks,
-                                                  const Control *C, Queue q, const int32_t *__restrict__ col_ind,
-                                                  const int32_t *__restrict__ cp, const int32_t *__restrict__ ri,
-                                                  const uint32_t *__restrict__ fbits) {
-    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+constexpr int PULL_CHUNK = 1024;  // vertices per bottom-up chunk (32 bitmap words, 4 per thread)
+
+// Bottom-up: chunks of PULL_CHUNK consecutive vertices, one workgroup each at
+// a time.  Every unvisited vertex scans its in-edges for a parent in the
+// previous frontier's bitmap and stops at the first one (on RMAT graphs
+// nearly all find one at their first in-edge: the hubs, lowest ids, come
+// first); each thread walks its vertices on its own and marks the ones it
+// reaches in an LDS bitmap of the chunk, which the workgroup then writes
+// whole into the visited and frontier words of the chunk and appends.
+//
+// Top-down: the queued frontier's edges in TILE-edge tiles (tile_owners).
+// An unvisited target is claimed with an atomic OR on its visited word; the
+// lanes of a wave whose targets share a word (a source's sorted columns put
+// them side by side) OR their bits together first and the run's last lane
+// issues one atomic for all, since atomics on one line are serialised.  A
+// lane owns its vertex when the bit was clear before the run's atomic and no
+// earlier lane of the run holds the same vertex; owned vertices are listed in
+// LDS and appended per tile.
+//
+// (eight waves per SIMD: the bottom-up scan is bound by the latency of its
+// dependent loads)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_round_bfs(
+    BfsRound rb, int level, int m, int expand_blocks, Control *C, Queue qc, Queue qn,
+    const int32_t *__restrict__ col_ind, const int32_t *__restrict__ cp, const int32_t *__restrict__ ri) {
+    constexpr int NW = WG / WAVE, SEG = TILE / NW;
+    static_assert(PULL_CHUNK == 4 * WG && PULL_CHUNK / 32 <= TILE, "chunk shape");
+    __shared__ int32_t s_own[TILE];  // tile owners; then this tile's claims, or the chunk bitmap
+    __shared__ int32_t s_base[TILE + 1];
+    __shared__ int32_t s_wmax[NW];
+    __shared__ uint64_t s_scan[NW + 1];
+    __shared__ uint32_t s_found[NW];
+    __shared__ int s_ncl;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    if (C->rebuild) return;  // an earlier round's queue is being rebuilt: this launch is void
+    const int par = level & 1;
     const Round cur = round_of(C, level);
-    if (cur.pull == DIR_PULL) {
-        pull_bfs_rows(m, cp, ri, op.visited, fbits, op.flags, blockIdx.x, gridDim.x);
+    if (cur.nF == 0) {  // nothing left: the descriptor of this round is empty too
+        if (blockIdx.x == 0 && tid == 0) C->ctl[par] = Ctl{0, 0, 0, 0};
         return;
     }
-    if (cur.pull || int(blockIdx.x) >= expand_blocks) return;
-    expand_tiles(op, m, cur, q, col_ind, blockIdx.x, expand_blocks);
+    const uint32_t *fcur = rb.fb + size_t((level + 2) % 3) * rb.fbw;
+    uint32_t *fnext = rb.fb + size_t(level % 3) * rb.fbw;
+    // the bitmap of the round after this one (last read by the previous round)
+    uint32_t *fclear = rb.fb + size_t((level + 1) % 3) * rb.fbw;
+    for (int i = blockIdx.x * WG + tid; i < rb.fbw; i += gridDim.x * WG) fclear[i] = 0u;
+    const bool pull = cur.pull == DIR_PULL;
+    const int nchunks = (m + PULL_CHUNK - 1) / PULL_CHUNK;
+    const int parts = pull ? min(int(gridDim.x), nchunks) : min(expand_blocks, cur.nTiles);
+    const int bid = blockIdx.x, grp = bid % NGROUP;
+    if (bid >= parts) return;
+    Padded<unsigned long long> *gcnt = C->gcnt[par];
+    uint32_t found = 0;
+    uint32_t dep = 0;  // returning atomics of thread 0 awaited before the completion count
+    if (pull) {
+        uint32_t *s_bits = (uint32_t *)s_own;
+        for (int ch = bid; ch < nchunks; ch += gridDim.x) {
+            const int base = ch * PULL_CHUNK, nv = min(PULL_CHUNK, m - base), nw = (nv + 31) / 32;
+            for (int i = tid; i < PULL_CHUNK / 32; i += WG) s_bits[i] = 0u;  // every word append_entries reads
+            __syncthreads();
+            for (int i = tid; i < nv; i += WG) {
+                const int v = base + i;
+                if ((rb.visited[v >> 5] >> (v & 31)) & 1u) continue;
+                const int a = cp[v], b = cp[v + 1];
+                for (int e = a; e < b; ++e) {
+                    const int u = ri[e];
+                    if ((unsigned)u < (unsigned)m && ((fcur[u >> 5] >> (u & 31)) & 1u)) {
+                        rb.depth[v] = level;
+                        found++;
+                        atomicOr(&s_bits[i >> 5], 1u << (i & 31));
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            for (int i = tid; i < nw; i += WG) {  // base is a multiple of 32
+                const uint32_t bits = s_bits[i];
+                const int wi = (base >> 5) + i;
+                if (bits) rb.visited[wi] |= bits;
+                fnext[wi] = bits;
+            }
+            // this thread's four vertices of the chunk, 4 * tid .. 4 * tid + 3
+            const uint32_t nib = (uint32_t(s_bits[tid >> 3]) >> (4 * (tid & 7))) & 0xfu;
+            append_entries<4>([&](int j) { return (nib >> j) & 1u ? base + 4 * tid + j : -1; }, rb.row_ptr, qn,
+                              rb.cap, gcnt, grp, &C->qoverflow[par], s_scan, dep);
+            __syncthreads();  // the next chunk clears s_bits
+        }
+    } else {
+        int32_t *s_list = s_own;
+        if (tid == 0) s_ncl = 0;
+        for (int t = bid; t < cur.nTiles; t += expand_blocks) {
+            int own[IPT];
+            const int elim = tile_owners<false, false>(t, cur, qc, s_own, s_base, nullptr, nullptr, s_wmax, own);
+            int col[IPT];
+#pragma unroll
+            for (int r = 0; r < IPT; ++r) {
+                const int s = wid * SEG + r * WAVE + lane;
+                col[r] = s < elim ? col_ind[s_base[own[r]] + s] : -1;
+                if ((unsigned)col[r] >= (unsigned)m) col[r] = -1;
+            }
+            __syncthreads();  // s_own becomes the claim list
+            // claims, in batches of CB slots: every run's atomic of a batch is
+            // issued before any result is used
+            constexpr int CB = 2;
+#pragma unroll
+            for (int r0 = 0; r0 < IPT; r0 += CB) {
+                uint32_t w[CB], old[CB], mine[CB];
+                int tail[CB];
+#pragma unroll
+                for (int j = 0; j < CB; ++j) w[j] = col[r0 + j] >= 0 ? rb.visited[col[r0 + j] >> 5] : ~0u;
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    const int c = col[r0 + j];
+                    const uint32_t bit = c >= 0 && !((w[j] >> (c & 31)) & 1u) ? 1u << (c & 31) : 0u;
+                    uint32_t excl;
+                    const uint32_t incl = run_or(c >= 0 ? c >> 5 : -1, bit, lane, excl, tail[j]);
+                    mine[j] = bit & ~excl;  // no earlier lane of the run holds this vertex
+                    old[j] = ~0u;
+                    if (tail[j] == lane && incl) old[j] = atomicOr(&rb.visited[c >> 5], incl);
+                    if (tail[j] == lane) w[j] = incl;  // the run's bits, kept by its last lane
+                }
+#pragma unroll
+                for (int j = 0; j < CB; ++j) {
+                    const uint32_t before = __shfl(old[j], tail[j]);
+                    const int c = col[r0 + j];
+                    // the run's new bits enter the frontier bitmap in one atomic
+                    if (tail[j] == lane && (w[j] & ~before)) atomicOr(&fnext[c >> 5], w[j] & ~before);
+                    if (mine[j] & ~before) {
+                        rb.depth[c] = level;
+                        found++;
+                        s_list[atomicAdd(&s_ncl, 1)] = c;  // at most TILE claims per tile
+                    }
+                }
+            }
+            __syncthreads();
+            const int ncl = s_ncl;
+            append_entries<IPT>(
+                [&](int j) {
+                    const int i = tid * IPT + j;
+                    return i < ncl ? s_list[i] : -1;
+                },
+                rb.row_ptr, qn, rb.cap, gcnt, grp, &C->qoverflow[par], s_scan, dep);
+            __syncthreads();  // every thread is done with s_list and s_ncl
+            if (tid == 0) s_ncl = 0;  // (the next tile's first barrier orders this)
+        }
+    }
+    // reached vertices of this workgroup, then its completion
+    found = wave_total32(found);
+    if (lane == 0) s_found[wid] = found;
+    __syncthreads();
+    if (tid != 0) return;
+    uint32_t f = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) f += s_found[w];
+    if (f) dep ^= atomicAdd(&C->rfound[par][bid % SPREAD].v, f);
+    // (the group counters' returns were awaited inside append_entries)
+    after_values(dep);
+    const unsigned done = atomicAdd(&C->gdone[par][grp].v, 1u) + 1u;
+    if (done != unsigned((parts - 1 - grp) / NGROUP + 1)) return;
+    if (atomicAdd(&C->alldone[par], 1u) + 1u != unsigned(min(parts, NGROUP))) return;
+    // the last workgroup: the round's descriptor, from atomic loads
+    unsigned long long gc[NGROUP];
+#pragma unroll
+    for (int g = 0; g < NGROUP; ++g) gc[g] = __hip_atomic_load(&gcnt[g].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned nfound = 0;
+#pragma unroll
+    for (int g = 0; g < SPREAD; ++g)
+        nfound += __hip_atomic_load(&C->rfound[par][g].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned over = __hip_atomic_load(&C->qoverflow[par], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long before = C->acc.edges;
+    Ctl c{0, 0, 0, 0};
+    int gt[NGROUP + 1];
+    int tile0 = 0;
+#pragma unroll
+    for (int g = 0; g < NGROUP; ++g) {
+        const int eg = int(gc[g] & 0xffffffffu);
+        gt[g] = tile0;
+        tile0 += (eg + TILE - 1) / TILE;
+        c.nF += int(gc[g] >> 32);
+        c.E += eg;
+    }
+    gt[NGROUP] = tile0;
+    c.nTiles = tile0;
+    c.pull = rb.rule.decide(c, before + (unsigned long long)c.E, cur.pull);
+    if (over && c.nF > 0 && c.pull != DIR_PULL) {
+        // the next round expands a queue this round could not hold: a pass
+        // over this round's bitmap (k_compact<PickBfsBits>) queues it and
+        // writes the descriptor; its counters start from zero
+        for (int g = 0; g < NGROUP; ++g) {
+            gcnt[g].v = 0;
+            C->gdone[par][g].v = 0;
+            C->nnew[par][g].v = 0;
+            C->rfound[par][g].v = 0;
+        }
+        C->alldone[par] = 0;
+        C->qoverflow[par] = 0;
+        C->rebuild = level + 1;
+        return;
+    }
+    for (int g = 0; g <= NGROUP; ++g) C->gtile[par][g] = gt[g];
+    C->ctl[par] = c;
+    if (level < LOG_ROUNDS) C->log[level] = c;
+    C->acc.edges = before + (unsigned long long)c.E;
+    if (nfound) {
+        C->acc.vertices[0].v += nfound;
+        if (C->acc.levels < level + 1) C->acc.levels = level + 1;
+    }
+    // counters of the next round (their last users, the previous round and
+    // this round's readers of its queue, have finished)
+    for (int g = 0; g < NGROUP; ++g) {
+        C->gcnt[par ^ 1][g].v = 0;
+        C->gdone[par ^ 1][g].v = 0;
+        C->rfound[par ^ 1][g].v = 0;
+    }
+    C->alldone[par ^ 1] = 0;
+    C->qoverflow[par ^ 1] = 0;
 }
+
+// Rebuilds the queue of a round whose appends overflowed: the vertices of the
+// round's frontier bitmap (depth, visited and bitmap are already written) go
+// through the generic compaction, which writes the descriptor.
+struct PickBfsBits {
+    const uint32_t *bits;  // the round's frontier bitmap
+    PullRuleBfs rule;
+    using Pre = uint32_t;
+    __device__ Pre load(int v) const { return bits[v >> 5]; }  // the bitmap covers nblk * CHUNK vertices
+    __device__ bool pick(int v, Pre w, uint32_t &) const { return (w >> (v & 31)) & 1u; }
+    __device__ void on_new(int) const {}
+    __device__ int direction(const Ctl &c, unsigned long long seen, int prev) const {
+        return rule.decide(c, seen, prev);
+    }
+};
 
 // Row holding the first edge of every TILE-edge tile of the CSC.
 __global__ __launch_bounds__(WG) void k_tile_rows(int m, const int32_t *__restrict__ cp, int32_t *trows) {
@@ -328,8 +622,8 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
     }
 }
 
-// (fbits / fval: the frontier bitmap and pull snapshots of a traversal with
-// pull steps, written whole by every compaction but round 0's seed kernel)
+// (flags: BFS without a CSC; fbits: the three frontier bitmaps of BFS with
+// one; fval: SSSP pull snapshots)
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
                                              uint64_t *status, Control *C, uint32_t *bin_totals, uint32_t thr0,
@@ -339,10 +633,11 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
     const int64_t mpad = int64_t(nblk) * CHUNK;
     if (mode == MODE_BFS) {
         for (int64_t i = gtid; i < mpad / 16; i += stride) {
-            *(uint4 *)(flags + 16 * i) = make_uint4(0, 0, 0, 0);
+            if (flags) *(uint4 *)(flags + 16 * i) = make_uint4(0, 0, 0, 0);
             if ((i & 1) == 0) {  // one bitmap word per 32 vertices
-                visited[i >> 1] = 0u;
-                if (fbits) fbits[i >> 1] = 0u;
+                const int64_t w = i >> 1, words = mpad / 32;
+                visited[w] = 0u;
+                if (fbits) fbits[w] = fbits[words + w] = fbits[2 * words + w] = 0u;
             }
         }
         for (int64_t i = gtid; i < m; i += stride) depth[i] = -1;
@@ -356,12 +651,7 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
     if (bin_totals)
         for (int64_t i = gtid; i < 2 * MAX_BUCKETS; i += stride) bin_totals[i] = 0;
-    if (gtid == 0) {
-        Control z{};
-        z.thr = thr0;
-        z.dmin[0] = z.dmin[1] = ~0u;
-        *C = z;
-    }
+    control_clear(C, gtid, stride, thr0);
 }
 
 // ----------------------------------------------------------------------------
@@ -477,7 +767,7 @@ void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *s
     APSPIE_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     memset(stats, 0, sizeof(*stats));
     stats->edges = (int64_t)acc.edges;
-    for (int i = 0; i < SPREAD; ++i) stats->vertices += (int64_t)acc.vertices[i];
+    for (int i = 0; i < SPREAD; ++i) stats->vertices += (int64_t)acc.vertices[i].v;
     stats->iterations = acc.levels;
     stats->kernel_ms = ms;
     if (mode == MODE_BFS) {
@@ -521,7 +811,14 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     uint32_t *fval = nullptr;
     if (pull) {  // growing a slot leaves the others in place
         if (MODE == MODE_BFS) {
-            b.fbits = (uint32_t *)ctx->slot(S_FBITS, size_t(b.nblk) * CHUNK / 8);
+            b.fbw = int32_t(size_t(b.nblk) * CHUNK / 32);
+            b.fbits = (uint32_t *)ctx->slot(S_FBITS, 3 * size_t(b.fbw) * 4);
+            b.flags = nullptr;  // the rounds keep depth, visited and the frontier themselves
+            const size_t qn = size_t(NGROUP) * size_t(b.qcap);
+            b.queue1 = (int32_t *)ctx->slot(S_QUEUE1, qn * 4);
+            b.qstart1 = (int32_t *)ctx->slot(S_QSTART1, qn * 4);
+            b.qoff1 = (int32_t *)ctx->slot(S_QOFF1, qn * 4);
+            b.tiles1 = (int32_t *)ctx->slot(S_TILES1, size_t(NGROUP) * size_t(b.tcap) * 4);
         } else {
             fval = (uint32_t *)ctx->slot(S_FVAL, size_t(g->m) * 4);
             b.trows = (int32_t *)ctx->slot(S_TROWS, (size_t(g->nnz) / TILE + 2) * 4);
@@ -580,15 +877,37 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             if (delta > 0.0f) return launch_compact(ctx, b, g, near_far(pick_for(level, -1)), level, 0);
         launch_compact(ctx, b, g, pick_for(level, -1), level, 0);
     };
-    auto expand = [&](int level) {
-        if (MODE == MODE_BFS && pull) {
-            ctx->prof_begin(APSPIE_KCLASS_ROUND);
-            k_round_bfs<<<std::max(b.expand_grid, vert_grid), WG, 0, ctx->stream>>>(
-                OpBfs{b.visited, b.flags}, level, g->m, int(b.expand_grid), b.C, queue_of(b), g->col_ind,
-                g->csc_col_ptr, g->csc_row_ind, b.fbits);
-            ctx->prof_end();
-            return;
+    // BFS with a CSC: one k_round_bfs launch per round, queues by round parity
+    auto queue_par = [&](int p) {
+        Queue q = queue_of(b);
+        if (p) {
+            q.vert = b.queue1;
+            q.start = b.qstart1;
+            q.off = b.qoff1;
+            q.tiles = b.tiles1;
         }
+        return q;
+    };
+    const PullRuleBfs bfs_rule{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta};
+    const int bfs_cap = ctx->tune.bfs_group_cap > 0 ? std::min(ctx->tune.bfs_group_cap, b.qcap) : b.qcap;
+    auto bfs_round = [&](int level) {
+        ctx->prof_begin(APSPIE_KCLASS_ROUND);
+        const BfsRound rb{b.visited, depth, b.fbits, b.fbw, g->row_ptr, bfs_rule, bfs_cap};
+        k_round_bfs<<<std::max(b.expand_grid, vert_grid), WG, 0, ctx->stream>>>(
+            rb, level, g->m, int(b.expand_grid), b.C, queue_par((level - 1) & 1), queue_par(level & 1), g->col_ind,
+            g->csc_col_ptr, g->csc_row_ind);
+        ctx->prof_end();
+    };
+    // a round whose queue overflowed its group regions: the pass over its bitmap
+    auto bfs_rebuild = [&](int level) {
+        ctx->prof_begin(APSPIE_KCLASS_COMPACT);
+        k_compact<PickBfsBits><<<b.nblk, WGC, 0, ctx->stream>>>(
+            PickBfsBits{b.fbits + size_t(level % 3) * b.fbw, bfs_rule}, level, g->m, b.nblk, 0, g->row_ptr,
+            queue_par(level & 1), b.C);
+        ctx->prof_end();
+        APSPIE_HIP(hipMemsetAsync(&b.C->rebuild, 0, sizeof(int), ctx->stream));
+    };
+    auto expand = [&](int level) {
         if (MODE == MODE_BFS)
             launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
         else
@@ -627,18 +946,33 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     // traversals end within 6-12 rounds, and a round launched past the end
     // costs its kernels' launch
     int level = 1, chunk = ctx->tune.round_chunk, checks = 0;
-    volatile Ctl *host_ctl = (volatile Ctl *)ctx->pinned;
+    // the descriptors, the round counter and the rebuild mark lead the control block
+    static_assert(offsetof(Control, rebuild) + sizeof(int) <= 4096, "fits the pinned staging");
+    volatile Control *host_c = (volatile Control *)ctx->pinned;
+    const bool bfs_rounds = MODE == MODE_BFS && pull;
     for (;;) {
         for (int i = 0; i < chunk; ++i, ++level) {
+            if (bfs_rounds) {
+                bfs_round(level);
+                continue;
+            }
             expand(level);
             compact(level);
         }
         APSPIE_HIP(hipGetLastError());
-        APSPIE_HIP(hipMemcpyAsync((void *)host_ctl, &b.C->ctl[(level - 1) & 1], sizeof(Ctl), hipMemcpyDeviceToHost,
-                                  ctx->stream));
+        APSPIE_HIP(hipMemcpyAsync((void *)host_c, b.C, offsetof(Control, rebuild) + sizeof(int),
+                                  hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+        if (bfs_rounds && host_c->rebuild) {  // the later launches of the chunk were void
+            level = host_c->rebuild - 1;
+            bfs_rebuild(level);
+            ++level;
+            continue;
+        }
+        const Ctl last{host_c->ctl[(level - 1) & 1].nF, host_c->ctl[(level - 1) & 1].E,
+                       host_c->ctl[(level - 1) & 1].nTiles, host_c->ctl[(level - 1) & 1].pull};
         // done: an empty frontier with nothing waiting past a threshold
-        if (host_ctl->nF == 0 && host_ctl->pull != DIR_REPICK) break;
+        if (last.nF == 0 && last.pull != DIR_REPICK) break;
         // every round settles a vertex or, repicking, admits a waiting one
         APSPIE_CHECK(level <= 2 * int64_t(g->m) + 4, APSPIE_ENOCONV, "traversal did not converge");
         ++checks;

@@ -40,10 +40,20 @@ enum {
 // workgroups.  Readers sum the slots.
 constexpr int SPREAD = 16;
 
+// A counter alone in a 256-byte slot.  Device-scope atomics that fall in one
+// 64-byte line are serialised where memory performs them (about 12 ns each
+// on MI355X: 4096 adds to one address take 47 us, to 16 adjacent 8-byte
+// slots 28 us, to 16 slots 256 bytes apart 1.7 us), so the slots of a spread
+// counter must not share a line.
+template <class T>
+struct alignas(256) Padded {
+    T v;
+};
+
 // Totals over one call.
 struct Acc {
     unsigned long long edges;             // sum of frontier edge counts
-    unsigned long long vertices[SPREAD];  // vertices that entered a frontier
+    Padded<unsigned long long> vertices[SPREAD];  // vertices that entered a frontier
     int32_t levels;                       // rounds with a non-empty frontier
     int32_t pad;
 };
@@ -62,20 +72,29 @@ constexpr int NGROUP = SPREAD;
 
 struct Control {
     Ctl ctl[2];                 // by round parity: totals over the groups
+    int cur_level;  // device-driven rounds: the round the next compaction runs
+    // BFS rounds that queue their own frontier (k_round_bfs): round + 1 of a
+    // round whose queue overflowed a group region and is rebuilt by a pass
+    // over its bitmap before the next round runs (0: none); rounds exit at
+    // once while it is set
+    int rebuild;
     // per round parity and group: (entries << 32) | edges, and chunks done
-    unsigned long long gcnt[2][NGROUP];
-    unsigned int gdone[2][NGROUP];
+    Padded<unsigned long long> gcnt[2][NGROUP];
+    Padded<unsigned int> gdone[2][NGROUP];
     unsigned int alldone[2];    // groups whose chunks are all done
     int32_t gtile[2][NGROUP + 1];  // first tile of each group in the round's tile numbering
-    unsigned int nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
+    Padded<unsigned int> nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
     Acc acc;
-    int cur_level;  // device-driven rounds: the round the next compaction runs
     // SSSP near/far: a changed vertex enters the frontier only while its
     // distance is below thr (float bits; all ones: no threshold); the others
     // wait, counted (ndefer) with their smallest distance (dmin) by parity
     uint32_t thr;
     unsigned int ndefer[2];
     uint32_t dmin[2];
+    // k_round_bfs, by round parity: vertices the round reached (by slot), and
+    // whether an append found its group region full
+    Padded<unsigned int> rfound[2][SPREAD];
+    unsigned int qoverflow[2];
     Ctl log[LOG_ROUNDS];  // the descriptor each compaction wrote, by round
 };
 
@@ -85,10 +104,17 @@ struct Bufs {
     uint8_t *flags = nullptr;
     uint32_t *visited = nullptr;
     uint32_t *prev = nullptr;  // SSSP: distance bits at the start of the round
-    uint32_t *fbits = nullptr;  // frontier bitmap written by each compaction (pull steps only)
+    // BFS with a CSC: three frontier bitmaps of fbw words each, the frontier
+    // of round L in bitmap L mod 3 (round L reads bitmap L - 1, writes L and
+    // clears L + 1)
+    uint32_t *fbits = nullptr;
+    int32_t fbw = 0;
     int32_t *trows = nullptr;   // SSSP pull: CSC row holding the first edge of each tile
     int32_t *queue = nullptr, *qstart = nullptr, *qoff = nullptr, *tiles = nullptr;
     int32_t qcap = 0, tcap = 0;  // entries and tiles per group region
+    // BFS with a CSC: the queue of odd rounds (vertex, row start, edge offset,
+    // tiles); even rounds use the arrays above
+    int32_t *queue1 = nullptr, *qstart1 = nullptr, *qoff1 = nullptr, *tiles1 = nullptr;
     float *qval = nullptr;
     uint64_t *status = nullptr;
     Control *C = nullptr;

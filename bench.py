@@ -95,34 +95,37 @@ def sssp_class_bytes(rounds, m, nnz):
 
 
 def bfs_class_bytes(rounds, m):
-    """Algorithmic bytes per traversal of each BFS kernel class (round 0 is
-    the seed kernel's, every later one a compaction's, and one more compaction
-    finds the frontier empty).
-    push expansion: 8 B per expanded edge (column, visited-bitmap probe),
-      8 B per frontier vertex (row start, edge offset), 1 B per new vertex;
-    bottom-up step: the m/8-byte visited bitmap, 12 B per unvisited vertex
-      (column-pointer pair, first in-edge), 1 B per new vertex;
-    compaction: 1 B per vertex (touched flag), the visited bitmap read and
-      written and the frontier bitmap written (3m/8 B), 29 B per new vertex
-      (flag clear, depth, row-pointer pair, 16-byte queue entry)."""
+    """Algorithmic bytes per traversal of each BFS kernel class with a CSC
+    (round 0 is the seed kernel's frontier, round i + 1 expands round i's;
+    every round also clears the bitmap of the round after next, m/8 B).
+    top-down round: 8 B per expanded edge (column, visited-word probe), 8 B per
+      frontier vertex (row start, edge offset), 20 B per new vertex (claim on
+      its visited word, frontier-bitmap bit, depth, row-pointer pair);
+    bottom-up round: the visited bitmap read and the frontier bitmap written
+      (m/4 B), 12 B per unvisited vertex (column-pointer pair, first in-edge),
+      12 B per new vertex (depth, row-pointer pair);
+    compaction, only before a top-down round: the frontier bitmap (m/8 B) and
+      24 B per new vertex (row-pointer pair, 16-byte queue entry); the others
+      read the round's tallies, write its descriptor and move nothing else."""
     out = {"expand": 0.0, "pull": 0.0, "compact": 0.0}
     seen = 0
     for i, (nf, e, pull) in enumerate(rounds):
         seen += nf
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
         if pull:
-            out["pull"] += m / 8.0 + 12.0 * (m - seen) + nxt
+            out["pull"] += 3.0 * m / 8.0 + 12.0 * (m - seen) + 12.0 * nxt
         else:
-            out["expand"] += 8.0 * e + 8.0 * nf + nxt
-        out["compact"] += m + 3.0 * m / 8.0 + 29.0 * nxt
-    # with a CSC, one launch per round runs either form (k_round_bfs)
+            out["expand"] += m / 8.0 + 8.0 * e + 8.0 * nf + 20.0 * nxt
+        if nxt > 0 and not rounds[i + 1][2]:
+            out["compact"] += m / 8.0 + 24.0 * nxt
+    # one launch per round runs either form (k_round_bfs)
     out["round"] = out["expand"] + out["pull"]
     return out
 
 
 KERNEL_NAMES = {
     ("sssp", "compact"): "k_compact<PickSssp>",
-    ("bfs", "compact"): "k_compact<PickBfs>",
+    ("bfs", "compact"): "k_compact<PickBfsBits>",
     ("sssp", "expand"): "k_expand<OpRelax>",
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",

@@ -154,6 +154,10 @@ typedef struct {
     int32_t spgemm_passes;    /* SpGEMM rows of the LDS tables: 1 = one pass into a staging area
                                  (default; two when their products reach 2^31), 2 = a counting pass,
                                  then the product pass */
+    int32_t bfs_group_cap;    /* BFS with a CSC: entries one round may append to a queue group region
+                                 (16 regions); a round whose new frontier does not fit, and whose
+                                 successor runs top-down, is queued by a pass over the vertex range
+                                 instead.  0 = the whole region (default), else 1..2^30 */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

@@ -43,6 +43,35 @@ def with_csc(ctx, h, und):
     return ctx.upload(h, csc="symmetric" if und and h.val is None else A.transpose(h))
 
 
+@pytest.mark.parametrize("cap", [1, 16, 700])
+@pytest.mark.parametrize("scale,und,seed", [(12, True, 1212), (14, False, 2214)])
+def test_bfs_rounds_whose_queue_overflows_are_rebuilt(tuned, cap, scale, und, seed):
+    """BFS with a CSC queues each round's frontier inside the round; with a
+    group region capped at `cap` entries the larger frontiers overflow, and a
+    top-down successor's queue is rebuilt by a pass over the round's bitmap
+    (k_compact launches).  alpha = 1 keeps most rounds top-down.  Depths and
+    counters equal the oracle's either way."""
+    ctx = tuned
+    ctx.set_tuning(bfs_alpha=1, bfs_beta=1 << 20, bfs_group_cap=cap)
+    h = A.rmat(scale, 16, seed, undirected=und)
+    g = with_csc(ctx, h, und)
+    rebuilt = 0
+    for src in [0, h.argmax_degree(), h.m // 3]:
+        ctx.set_profiling(True)
+        ctx.profile_reset()
+        depth, st = ctx.bfs(g, int(src))
+        rebuilt += ctx.profile()["compact"][1] - 1  # less the seed kernel's launch
+        ctx.set_profiling(False)
+        want, sd, e_exp, vr = O.bfs(ocsr(h), int(src))
+        assert np.array_equal(depth, want), (scale, und, src, cap)
+        assert (st.edges, st.vertices, st.iterations) == (e_exp, vr, sd)
+        # the round log is the oracle's level by level
+        rounds = ctx.rounds()
+        assert len(rounds) == sd
+    if cap <= 16:
+        assert rebuilt > 0  # the capped regions overflowed
+
+
 @pytest.mark.parametrize("mode", ["default", "always"])
 @pytest.mark.parametrize("scale,und,seed", [(10, True, 1010), (12, False, 2012), (14, True, 1014),
                                             (15, False, 2015)])
@@ -192,8 +221,9 @@ def test_push_and_pull_agree_and_thresholds_round_trip(tuned):
 
 def test_pull_rounds_are_timed_separately(tuned):
     """BFS with a CSC runs one launch per round (class "round": the push tiles
-    or the bottom-up step); SSSP with pull rounds launches the expansion and
-    the pull step side by side, each timed in its own class."""
+    or the bottom-up step, which also queue the next frontier) and no
+    compaction after the seed's; SSSP with pull rounds launches the expansion
+    and the pull step side by side, each timed in its own class."""
     ctx = tuned
     ctx.set_tuning(**ALWAYS)
     h = A.rmat(14, 16, 77, undirected=True, weight_seed=5)
@@ -209,6 +239,7 @@ def test_pull_rounds_are_timed_separately(tuned):
     finally:
         ctx.set_profiling(False)
     assert prof_bfs["round"][1] > 0 and prof_bfs["expand"][1] == 0 and prof_bfs["pull"][1] == 0
-    assert prof_bfs["round"][1] == prof_bfs["compact"][1] - 1  # one per round after the seed's compaction
+    assert prof_bfs["compact"][1] == 1  # the seed kernel
+    assert prof_bfs["round"][1] >= st.iterations  # every level, and the launches past the end
     assert prof_sssp["pull"][1] > 0 and prof_sssp["expand"][1] == prof_sssp["pull"][1]
     assert prof_sssp["round"][1] == 0
