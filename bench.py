@@ -95,30 +95,28 @@ def sssp_class_bytes(rounds, m, nnz):
 
 
 def bfs_class_bytes(rounds, m):
-    """Algorithmic bytes per traversal of each BFS kernel class with a CSC
-    (round 0 is the seed kernel's frontier, round i + 1 expands round i's;
-    every round also clears the bitmap of the round after next, m/8 B).
+    """Algorithmic bytes per traversal of each BFS kernel class with a CSC:
+    one k_round_bfs launch per round, which also queues the vertices it
+    reaches (round 0 is the seed kernel's frontier, round i + 1 expands round
+    i's; every round clears the bitmap of the round after next, m/8 B).
     top-down round: 8 B per expanded edge (column, visited-word probe), 8 B per
-      frontier vertex (row start, edge offset), 20 B per new vertex (claim on
-      its visited word, frontier-bitmap bit, depth, row-pointer pair);
+      frontier vertex (row start, edge offset), 32 B per new vertex (claim on
+      its visited word, frontier-bitmap bit, depth, row-pointer pair, 12-byte
+      queue entry);
     bottom-up round: the visited bitmap read and the frontier bitmap written
       (m/4 B), 12 B per unvisited vertex (column-pointer pair, first in-edge),
-      12 B per new vertex (depth, row-pointer pair);
-    compaction, only before a top-down round: the frontier bitmap (m/8 B) and
-      24 B per new vertex (row-pointer pair, 16-byte queue entry); the others
-      read the round's tallies, write its descriptor and move nothing else."""
+      24 B per new vertex (depth, row-pointer pair, 12-byte queue entry).
+    A compaction runs only to rebuild a queue that overflowed its region
+    (never at these sizes); its class stays 0."""
     out = {"expand": 0.0, "pull": 0.0, "compact": 0.0}
     seen = 0
     for i, (nf, e, pull) in enumerate(rounds):
         seen += nf
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
         if pull:
-            out["pull"] += 3.0 * m / 8.0 + 12.0 * (m - seen) + 12.0 * nxt
+            out["pull"] += 3.0 * m / 8.0 + 12.0 * (m - seen) + 24.0 * nxt
         else:
-            out["expand"] += m / 8.0 + 8.0 * e + 8.0 * nf + 20.0 * nxt
-        if nxt > 0 and not rounds[i + 1][2]:
-            out["compact"] += m / 8.0 + 24.0 * nxt
-    # one launch per round runs either form (k_round_bfs)
+            out["expand"] += m / 8.0 + 8.0 * e + 8.0 * nf + 32.0 * nxt
     out["round"] = out["expand"] + out["pull"]
     return out
 
