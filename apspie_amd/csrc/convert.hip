@@ -18,6 +18,7 @@
 // values are gathered through the original entry positions.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "convert.hpp"
 #include "device.hpp"
@@ -271,6 +272,112 @@ __global__ __launch_bounds__(T) void k_scatter(const int32_t *__restrict__ key, 
     }
 }
 
+// Segmented passes (seg_radix_kv): tiles of at most TILE_N items that never
+// cross a segment, tile j given as (t0, t1, hbase, hstride): its count of
+// digit d sits at hist[hbase + d * hstride].  hist[...] = items of the tile
+// with digit d, counted per wave as in k_hist.
+__global__ __launch_bounds__(T) void k_seg_hist(const int32_t *__restrict__ key, const int4 *__restrict__ tiles,
+                                                int ntiles, int shift, int32_t *hist) {
+    constexpr int NW = T / WAVE;
+    __shared__ int32_t s_h[NW][RADIX];
+    const int tid = threadIdx.x, wid = tid / WAVE;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int4 tl = tiles[tile];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_h[w][tid] = 0;
+        __syncthreads();
+        for (int t = tl.x + tid; t < tl.y; t += T) atomicAdd(&s_h[wid][(uint32_t(key[t]) >> shift) & (RADIX - 1)], 1);
+        __syncthreads();
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) c += s_h[w][tid];
+        hist[tl.z + tid * tl.w] = c;
+        __syncthreads();
+    }
+}
+
+// The stable scatter of k_scatter over the segmented tiles, moving an FP64
+// value with each key (staged in LDS beside it) instead of an index.
+__global__ __launch_bounds__(T) void k_seg_scatter(const int32_t *__restrict__ key, const double *__restrict__ val,
+                                                   const int4 *__restrict__ tiles, int ntiles, int shift,
+                                                   const int32_t *__restrict__ off, int32_t *key_out,
+                                                   double *val_out) {
+    constexpr int NW = T / WAVE, PER = TILE_N / NW, R = PER / WAVE;
+    static_assert(RADIX == T, "one thread per digit");
+    __shared__ int32_t s_cnt[NW][RADIX];
+    __shared__ int32_t s_goff[RADIX];  // global position of the tile's digit-d run, less its tile-local start
+    __shared__ int32_t s_wsum[NW];
+    __shared__ int32_t s_k[TILE_N];
+    __shared__ double s_v[TILE_N];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int4 tl = tiles[tile];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
+        __syncthreads();
+        const int items = tl.y - tl.x;
+        const int wb = tl.x + wid * PER;
+        int k[R];
+        double v[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int t = wb + r * WAVE + lane;
+            k[r] = t < tl.y ? key[t] : -1;
+            v[r] = t < tl.y ? val[t] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (k[r] >= 0) atomicAdd(&s_cnt[wid][(uint32_t(k[r]) >> shift) & (RADIX - 1)], 1);
+        __syncthreads();
+        {  // thread d: digit d's tile-local start, then each wave's start
+            int tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) tot += s_cnt[w][tid];
+            const int incl = int(wave_incl_scan32(uint32_t(tot), lane));
+            if (lane == WAVE - 1) s_wsum[wid] = incl;
+            __syncthreads();
+            int run = incl - tot;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) run += w < wid ? s_wsum[w] : 0;
+            s_goff[tid] = off[tl.z + tid * tl.w] - run;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                const int c = s_cnt[w][tid];
+                s_cnt[w][tid] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const bool valid = k[r] >= 0;
+            const int d = valid ? (uint32_t(k[r]) >> shift) & (RADIX - 1) : 0;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const uint64_t ones = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? ones : ~ones;
+            }
+            const int rank = __popcll(peers & lt);
+            const int pos = valid ? s_cnt[wid][d] + rank : 0;
+            if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
+            if (valid) {
+                s_k[pos] = k[r];
+                s_v[pos] = v[r];
+            }
+        }
+        __syncthreads();
+        for (int q = tid; q < items; q += T) {
+            const int kk = s_k[q];
+            const int g = s_goff[(uint32_t(kk) >> shift) & (RADIX - 1)] + q;
+            key_out[g] = kk;
+            val_out[g] = s_v[q];
+        }
+        __syncthreads();  // the next tile clears the rows and refills the stage
+    }
+}
+
 // rowid[e] = the row holding entry e (one wave per row)
 __global__ __launch_bounds__(256) void k_row_ids(int m, const int32_t *__restrict__ rp, int32_t *rowid) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -331,6 +438,43 @@ int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, in
         scan32(ctx, hist, offs, nh);
         k_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], idx[cur], ident && shift == 0, n, shift, ntiles, offs,
                                              key[cur ^ 1], idx[cur ^ 1]);
+    }
+    APSPIE_HIP(hipGetLastError());
+    return cur;
+}
+
+int seg_radix_kv(apspie_ctx *ctx, int32_t *key[2], double *val[2], int64_t n, const std::vector<int32_t> &seg,
+                 int bits, std::vector<int32_t> &tiles) {
+    int cur = 0;
+    if (n == 0 || bits == 0) return cur;
+    // the tile table: tiles of a segment are consecutive, histogram slots of
+    // a segment are digit-major over its tiles
+    tiles.clear();
+    const int nseg = int(seg.size()) - 1;
+    int64_t nh = 0;
+    for (int r = 0; r < nseg; ++r) {
+        const int64_t a = seg[r], b = seg[size_t(r) + 1];
+        const int nt = int(ceil_div(b - a, TILE_N));
+        for (int j = 0; j < nt; ++j) {
+            tiles.push_back(int32_t(a + int64_t(j) * TILE_N));
+            tiles.push_back(int32_t(std::min<int64_t>(b, a + int64_t(j + 1) * TILE_N)));
+            tiles.push_back(int32_t(nh + j));
+            tiles.push_back(nt);
+        }
+        nh += int64_t(nt) * RADIX;
+    }
+    const int ntiles = int(tiles.size() / 4);
+    APSPIE_CHECK(nh < (int64_t(1) << 31), APSPIE_ELIMIT, "segmented sort histogram too large");
+    int4 *dt = (int4 *)ctx->slot(S_AUX6, tiles.size() * 4);
+    APSPIE_HIP(hipMemcpyAsync(dt, tiles.data(), tiles.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
+    int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
+    const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
+    for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
+        k_seg_hist<<<gt, T, 0, ctx->stream>>>(key[cur], dt, ntiles, shift, hist);
+        scan32(ctx, hist, offs, nh);
+        k_seg_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], val[cur], dt, ntiles, shift, offs, key[cur ^ 1],
+                                                 val[cur ^ 1]);
     }
     APSPIE_HIP(hipGetLastError());
     return cur;

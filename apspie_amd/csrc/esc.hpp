@@ -1,7 +1,7 @@
-// esc.hpp -- expand-sort-compress steps shared by the outer-product SpGEMM
-// (outer.hip) and the longest rows of the row-wise SpGEMM (spgemm.hip):
-// products sorted stably by column and then by row (convert.hpp's radix
-// pairs) are grouped into runs of equal (row, column).
+// esc.hpp -- expand-sort-compress steps of the outer-product SpGEMM
+// (outer.hip): products sorted stably by column and then by row
+// (convert.hpp's radix pairs) are grouped into runs of equal (row, column).
+// bits_for also sizes the row-wise SpGEMM's long-row sort (spgemm.hip).
 #pragma once
 
 #include <cstdint>

@@ -11,9 +11,11 @@
 //   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
 //      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
 //      once) and write the sorted row to a staging area at the prefix of the
-//      rows' ub, with its nnz.  Long rows: their products are written once,
-//      sorted stably by column and then by row (radix), and each run of equal
-//      columns is one entry (FP64 sum in k order); nnz = the runs.
+//      rows' ub, with its nnz.  Long rows: their products are written once
+//      in row order, sorted stably by column inside each row's range (a
+//      segmented radix sort that moves the FP64 values with the keys), and
+//      each run of equal columns is one entry (FP64 sum in k order); nnz =
+//      the runs.
 //   4. row_ptr(C) = exclusive scan (decoupled look-back).
 //   5. the staged rows are copied into C; the long rows' runs are summed
 //      into C.
@@ -762,15 +764,14 @@ __global__ __launch_bounds__(256) void k_rows_wave(const int32_t *__restrict__ r
 // The rows with the most products, by expand-sort-compress instead of
 // global hash tables (whose inserts are device-scope atomics, a few G/s):
 // row r of the list writes its products at poff[r] in the order of A's
-// entries, then of B's row: (column, a * b in FP64); key = column (B.n for a
-// column outside B, dropped later as the hash tables skip it); the first
-// sort pass takes each product's position as its index.  Stable sorts by
-// column and then by row leave each output entry's products in increasing k,
+// entries, then of B's row: key = column (B.n for a column outside B,
+// dropped later as the hash tables skip it), value a * b in FP64.  A stable
+// sort by column inside each row's range (cvt::seg_radix_kv, the values
+// moving with the keys) leaves each output entry's products in increasing k,
 // so the FP64 run sums are the tables' and Gustavson's.
 template <int T>
 __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
-                                                   const int32_t *__restrict__ poff, int32_t *pcol, double *pval,
-                                                   int32_t *key) {
+                                                   const int32_t *__restrict__ poff, int32_t *key, double *pval) {
     __shared__ int32_t s_bs[T];
     __shared__ int32_t s_pre[T];
     __shared__ float s_a[T];
@@ -809,10 +810,8 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
                 }
                 const int q = s_bs[lo] + (t - s_pre[lo]);
                 const int c = B.col[q];
-                const bool ok = (unsigned)c < (unsigned)B.n;
                 const int o = base + t;
-                pcol[o] = ok ? c : 0;
-                key[o] = ok ? c : B.n;
+                key[o] = (unsigned)c < (unsigned)B.n ? c : B.n;
                 pval[o] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
             }
             base += tot;
@@ -821,63 +820,55 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
     }
 }
 
-// Sort keys of the row pass: the list row of product perm[s] (the last r
-// with poff[r] <= perm[s]: brow[b] is that row for product b * 2048, the
-// offsets are read from LDS when they fit), or nrows
-// for a product outside B's columns (column key B.n), which sorts after
-// every row and is dropped.
-__global__ __launch_bounds__(256) void k_long_rowkey(int64_t n, int nrows, int ncol, const int32_t *__restrict__ poff,
-                                                     const int32_t *__restrict__ brow,
-                                                     const int32_t *__restrict__ ckey,
-                                                     const int32_t *__restrict__ perm, int32_t *key, int32_t *idx) {
-    constexpr int MAXL = 16384;
-    __shared__ int32_t s_off[MAXL];
-    const bool lds = nrows + 1 <= MAXL;
-    if (lds)
-        for (int r = threadIdx.x; r <= nrows; r += 256) s_off[r] = poff[r];
-    __syncthreads();
-    const int32_t *off = lds ? s_off : poff;
+// flag[s] = 1 where a run of equal columns starts inside a long row, after
+// the sort: s is its row's first product or its column differs from s - 1's.
+// Products outside B's columns (key ncol, sorted last in their row) start
+// none.  Row starts are marked by k_long_rowstart afterwards.
+__global__ __launch_bounds__(256) void k_long_flags(int64_t n, const int32_t *__restrict__ key, int ncol,
+                                                    int32_t *flag) {
     for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
-        const int p = perm[s];
-        int r = nrows;
-        if (ckey[s] != ncol) {
-            // the row holding product (p >> 11) << 11, then forward: every
-            // row holds over 2048 products, so at most one step
-            int lo = brow[p >> 11];
-            while (lo + 1 < nrows && off[lo + 1] <= p) ++lo;
-            r = lo;
-        }
-        key[s] = r;
-        idx[s] = p;
+        const int k = key[s];
+        flag[s] = k < ncol && (s == 0 || k != key[s - 1]) ? 1 : 0;
+    }
+}
+__global__ __launch_bounds__(256) void k_long_rowstart(int nrows, const int32_t *__restrict__ poff,
+                                                       const int32_t *__restrict__ key, int ncol, int32_t *flag) {
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nrows; r += gridDim.x * 256) {
+        const int s = poff[r];
+        if (s < poff[r + 1]) flag[s] = key[s] < ncol ? 1 : 0;
     }
 }
 
-// nnz(C(i,:)) of the long rows: the runs between their sorted row starts
+// nnz(C(i,:)) of the long rows: the runs inside their product ranges
 __global__ __launch_bounds__(256) void k_long_nnz(int nrows, const int32_t *__restrict__ rows,
-                                                  const int32_t *__restrict__ rstart,
+                                                  const int32_t *__restrict__ poff,
                                                   const int32_t *__restrict__ runpos, int32_t *nnzc) {
     for (int r = blockIdx.x * 256 + threadIdx.x; r < nrows; r += gridDim.x * 256)
-        nnzc[rows[r]] = runpos[rstart[r + 1]] - runpos[rstart[r]];
+        nnzc[rows[r]] = runpos[poff[r + 1]] - runpos[poff[r]];
 }
 
-// one output entry per run of a long row: its products summed in FP64 in
-// sorted order, written at the row's offset in C
+// one output entry per run of a long row: its products (consecutive after
+// the sort, in increasing k) summed in FP64 in that order, written at the
+// row's offset in C.  The row of a run start s: the row holding product
+// (s >> 11) << 11 (brow), then forward (every row holds over 2048 products,
+// so at most one step).
 __global__ __launch_bounds__(256) void k_long_runs(int64_t n, int nrows, const int32_t *__restrict__ rows,
-                                                   const int32_t *__restrict__ rstart,
+                                                   const int32_t *__restrict__ poff,
+                                                   const int32_t *__restrict__ brow,
                                                    const int32_t *__restrict__ flag,
                                                    const int32_t *__restrict__ runpos,
-                                                   const int32_t *__restrict__ srow, const int32_t *__restrict__ perm,
-                                                   const int32_t *__restrict__ pcol,
-                                                   const double *__restrict__ pval, const int32_t *__restrict__ crp,
-                                                   int32_t *ccol, float *cval) {
+                                                   const int32_t *__restrict__ key, const double *__restrict__ val,
+                                                   const int32_t *__restrict__ crp, int32_t *ccol, float *cval) {
     for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
         if (!flag[s]) continue;
-        const int r = srow[s];
-        if (r >= nrows) continue;  // products outside B's columns
-        double acc = pval[perm[s]];
-        for (int64_t x = s + 1; x < n && !flag[x]; ++x) acc += pval[perm[x]];
-        const int j = crp[rows[r]] + runpos[s] - runpos[rstart[r]];
-        ccol[j] = pcol[perm[s]];
+        int r = brow[s >> 11];
+        while (r + 1 < nrows && poff[r + 1] <= s) ++r;
+        const int k = key[s];
+        const int64_t end = poff[r + 1];
+        double acc = val[s];
+        for (int64_t x = s + 1; x < end && key[x] == k; ++x) acc += val[x];
+        const int j = crp[rows[r]] + runpos[s] - runpos[poff[r]];
+        ccol[j] = k;
         cval[j] = float(acc);
     }
 }
@@ -936,12 +927,13 @@ struct LongRows {
     int nrows = 0;
     int64_t n = 0;  // products
     const int32_t *rows = nullptr;
-    int32_t *rstart = nullptr, *flag = nullptr, *runpos = nullptr, *srow = nullptr, *perm = nullptr, *pcol = nullptr;
-    double *pval = nullptr;
-    std::vector<int32_t> hpo, hbrow;  // host sources of the offset tables, kept until the stream syncs
+    int32_t *poff = nullptr, *brow = nullptr, *key = nullptr, *flag = nullptr, *runpos = nullptr;
+    double *val = nullptr;
+    // host sources of the uploaded tables, kept until the stream syncs
+    std::vector<int32_t> hpo, hbrow, htiles;
 };
 
-// Symbolic phase of the long rows: products, sorts, runs, nnz(C(i,:)).
+// Symbolic phase of the long rows: products, the sort, runs, nnz(C(i,:)).
 // gub[r] = the products of row r of the list (their sum < 2^31).
 LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const CsrD &A,
                        const CsrD &B, int32_t *nnzc) {
@@ -962,44 +954,34 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     L.n = n;
     L.rows = rows;
     auto S = [&](int k, int64_t bytes) { return ctx->slot(S_OUTER0 + k, size_t(std::max<int64_t>(bytes, 16))); };
-    int32_t *poff = (int32_t *)S(0, (int64_t(ng) + 1) * 4);
-    L.pcol = (int32_t *)S(2, n * 4);
-    L.pval = (double *)S(3, n * 8);
-    int32_t *key[2] = {(int32_t *)S(4, (n + 1) * 4), (int32_t *)S(5, (n + 1) * 4)};
-    int32_t *idx[2] = {(int32_t *)S(6, (n + 1) * 4), (int32_t *)S(7, (n + 1) * 4)};
-    L.rstart = (int32_t *)S(8, (int64_t(ng) + 2) * 4);
-    int32_t *dbrow = (int32_t *)S(9, int64_t(brow.size()) * 4);
-    APSPIE_HIP(hipMemcpyAsync(poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
-    APSPIE_HIP(hipMemcpyAsync(dbrow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    L.poff = (int32_t *)S(0, (int64_t(ng) + 1) * 4);
+    L.brow = (int32_t *)S(1, int64_t(brow.size()) * 4);
+    int32_t *key[2] = {(int32_t *)S(2, n * 4), (int32_t *)S(3, n * 4)};
+    double *val[2] = {(double *)S(4, n * 8), (double *)S(5, n * 8)};
+    APSPIE_HIP(hipMemcpyAsync(L.poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    APSPIE_HIP(hipMemcpyAsync(L.brow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
-                          ctx->stream>>>(rows, ng, A, B, poff, L.pcol, L.pval, key[0]);
+                          ctx->stream>>>(rows, ng, A, B, L.poff, key[0], val[0]);
     APSPIE_HIP(hipGetLastError());
-    const int c1 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(B.n) + 1), 0, true);
+    const int c = cvt::seg_radix_kv(ctx, key, val, n, po, esc::bits_for(int64_t(B.n) + 1), L.htiles);
+    L.key = key[c];
+    L.val = val[c];
+    L.flag = key[c ^ 1];  // free after the last pass
+    L.runpos = (int32_t *)S(6, (n + 1) * 4);
     const unsigned g = cvt::grid_for(ctx, n, 256);
-    // the row keys and the permutation go to the other buffer pair, where
-    // the row pass starts
-    // (each workgroup stages the row offsets in LDS once: a few per CU)
-    k_long_rowkey<<<std::min(g, unsigned(ctx->cu_count) * 4), 256, 0, ctx->stream>>>(n, ng, B.n, poff, dbrow, key[c1],
-                                                                                     idx[c1], key[c1 ^ 1],
-                                                                                     idx[c1 ^ 1]);
-    const int c2 = cvt::radix_pairs(ctx, key, idx, n, esc::bits_for(int64_t(ng) + 1), c1 ^ 1);
-    L.srow = key[c2];
-    L.perm = idx[c2];
-    L.flag = key[c2 ^ 1];
-    L.runpos = idx[c2 ^ 1];
-    esc::k_runflag<<<g, 256, 0, ctx->stream>>>(n, L.srow, L.perm, L.pcol, L.flag);
+    k_long_flags<<<g, 256, 0, ctx->stream>>>(n, L.key, B.n, L.flag);
+    const unsigned gr = std::max(1u, std::min(ceil_div(ng, 256), 1024u));
+    k_long_rowstart<<<gr, 256, 0, ctx->stream>>>(ng, L.poff, L.key, B.n, L.flag);
     cvt::scan32(ctx, L.flag, L.runpos, n);
-    cvt::compress(ctx, L.srow, n, ng + 1, L.rstart);  // sorted row starts, rows 0 .. ng (the dropped ones)
-    k_long_nnz<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(ng, rows, L.rstart,
-                                                                                         L.runpos, nnzc);
+    k_long_nnz<<<gr, 256, 0, ctx->stream>>>(ng, rows, L.poff, L.runpos, nnzc);
     APSPIE_HIP(hipGetLastError());
     return L;
 }
 
 void long_numeric(apspie_ctx *ctx, const LongRows &L, const int32_t *crp, int32_t *ccol, float *cval) {
-    k_long_runs<<<cvt::grid_for(ctx, L.n, 256), 256, 0, ctx->stream>>>(L.n, L.nrows, L.rows, L.rstart, L.flag,
-                                                                       L.runpos, L.srow, L.perm, L.pcol, L.pval,
-                                                                       crp, ccol, cval);
+    k_long_runs<<<cvt::grid_for(ctx, L.n, 256), 256, 0, ctx->stream>>>(L.n, L.nrows, L.rows, L.poff, L.brow,
+                                                                       L.flag, L.runpos, L.key, L.val, crp, ccol,
+                                                                       cval);
     APSPIE_HIP(hipGetLastError());
 }
 
