@@ -84,6 +84,7 @@ class Tuning(C.Structure):
         ("sssp_delta", C.c_int32),
         ("spgemm_long_rows", C.c_int32),
         ("spgemm_passes", C.c_int32),
+        ("spgemm_esc_min", C.c_int32),
         ("bfs_group_cap", C.c_int32),
     ]
 

@@ -103,6 +103,7 @@ struct apspie_ctx {
         int spgemm_long_rows = 1;
         int spgemm_passes = 1;
         int bfs_group_cap = 0;
+        int spgemm_esc_min = 1024;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

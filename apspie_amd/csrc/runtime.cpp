@@ -150,7 +150,9 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
         t->batch_chunk < 1 || t->batch_chunk > 16 ||
         (t->sssp_batch_lanes != 0 && t->sssp_batch_lanes != 16 && t->sssp_batch_lanes != 64) ||
         t->sssp_delta < 0 || t->sssp_delta > (1 << 24) || (t->spgemm_long_rows != 0 && t->spgemm_long_rows != 1) ||
-        (t->spgemm_passes != 1 && t->spgemm_passes != 2) || t->bfs_group_cap < 0 || t->bfs_group_cap > (1 << 30)) {
+        (t->spgemm_passes != 1 && t->spgemm_passes != 2) || t->bfs_group_cap < 0 || t->bfs_group_cap > (1 << 30) ||
+        (t->spgemm_esc_min != 256 && t->spgemm_esc_min != 512 && t->spgemm_esc_min != 1024 &&
+         t->spgemm_esc_min != 2048)) {
         ctx->err = "tuning knob out of range";
         return APSPIE_EINVAL;
     }
@@ -167,6 +169,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.spgemm_long_rows = t->spgemm_long_rows;
     ctx->tune.spgemm_passes = t->spgemm_passes;
     ctx->tune.bfs_group_cap = t->bfs_group_cap;
+    ctx->tune.spgemm_esc_min = t->spgemm_esc_min;
     return APSPIE_OK;
 }
 
@@ -185,6 +188,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->spgemm_long_rows = ctx->tune.spgemm_long_rows;
     t->spgemm_passes = ctx->tune.spgemm_passes;
     t->bfs_group_cap = ctx->tune.bfs_group_cap;
+    t->spgemm_esc_min = ctx->tune.spgemm_esc_min;
     return APSPIE_OK;
 }
 

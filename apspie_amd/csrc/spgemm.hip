@@ -1,5 +1,6 @@
 // spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned LDS
-// hash tables for rows up to 2048 products, expand-sort-compress beyond.
+// hash tables for rows up to 1024 products (spgemm_esc_min; the tables reach
+// 2048), expand-sort-compress beyond.
 //
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
@@ -7,7 +8,9 @@
 //   2. bins    rows by ub: <= 16 (four rows a wave) and <= 64 (a wave per
 //              row), both in registers; <= 128, <= 256, <= 512, <= 1024,
 //              <= 2048 (LDS tables of 512 .. 8192 slots, 128 .. 1024 threads
-//              per row); > 2048 (the long rows).
+//              per row); > 2048 (the long rows, joined by the bins above
+//              spgemm_esc_min, 1024 by default: expand-sort-compress moves a
+//              product faster than the largest tables).
 //   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
 //      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
 //      once) and write the sorted row to a staging area at the prefix of the
@@ -850,8 +853,8 @@ __global__ __launch_bounds__(256) void k_long_nnz(int nrows, const int32_t *__re
 // one output entry per run of a long row: its products (consecutive after
 // the sort, in increasing k) summed in FP64 in that order, written at the
 // row's offset in C.  The row of a run start s: the row holding product
-// (s >> 11) << 11 (brow), then forward (every row holds over 2048 products,
-// so at most one step).
+// (s >> 11) << 11 (brow), then forward (every row holds over 256 products,
+// so at most eight steps; two with the default threshold of 1024).
 __global__ __launch_bounds__(256) void k_long_runs(int64_t n, int nrows, const int32_t *__restrict__ rows,
                                                    const int32_t *__restrict__ poff,
                                                    const int32_t *__restrict__ brow,
@@ -873,11 +876,10 @@ __global__ __launch_bounds__(256) void k_long_runs(int64_t n, int nrows, const i
     }
 }
 
-// ubs[i] = the products of row i when the LDS tables take it, else 0: the
-// rows' room in the staging area of the one-pass product
-__global__ __launch_bounds__(256) void k_stage_ub(int m, const int32_t *__restrict__ ub, int32_t *ubs) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256)
-        ubs[i] = ub[i] <= BIN_UB[NBINS - 2] ? ub[i] : 0;
+// ubs[i] = the products of row i when the LDS tables take it (ub <= lub),
+// else 0: the rows' room in the staging area of the one-pass product
+__global__ __launch_bounds__(256) void k_stage_ub(int m, const int32_t *__restrict__ ub, int lub, int32_t *ubs) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) ubs[i] = ub[i] <= lub ? ub[i] : 0;
 }
 
 // C(i,:) of the rows of the LDS and register bins from the staging area.
@@ -885,7 +887,7 @@ __global__ __launch_bounds__(256) void k_stage_ub(int m, const int32_t *__restri
 // are contiguous, so its threads stride over them (coalesced writes) and
 // find each output's row by a binary search over the block's row pointers
 // in LDS; the outputs of long rows (written by their own path) are skipped.
-__global__ __launch_bounds__(256) void k_unstage(int m, const int32_t *__restrict__ ub,
+__global__ __launch_bounds__(256) void k_unstage(int m, const int32_t *__restrict__ ub, int lub,
                                                  const int32_t *__restrict__ ubpre, const int32_t *__restrict__ crp,
                                                  const int32_t *__restrict__ scol, const float *__restrict__ sval,
                                                  int32_t *ccol, float *cval) {
@@ -896,7 +898,7 @@ __global__ __launch_bounds__(256) void k_unstage(int m, const int32_t *__restric
         for (int x = threadIdx.x; x <= nr; x += 256) s_rp[x] = crp[r0 + x];
         if (threadIdx.x < nr) {
             const int i = r0 + threadIdx.x;
-            s_src[threadIdx.x] = ub[i] > BIN_UB[NBINS - 2] ? -1 : ubpre[i];
+            s_src[threadIdx.x] = ub[i] > lub ? -1 : ubpre[i];
         }
         __syncthreads();
         const int hi = s_rp[nr];
@@ -1117,6 +1119,19 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         APSPIE_CHECK(hbad == 0, APSPIE_ELIMIT, "a row of C has 2^31 or more products");
         for (int k = 0; k < NBINS; ++k) P.counts[k] = hcounts[k];
+        // rows beyond lub products leave the LDS tables for the long rows'
+        // path: the bins above spgemm_esc_min join the long list (each bin's
+        // list region holds m + 1 rows, so the long region has room)
+        int lub = BIN_UB[NBINS - 2];
+        if (ctx->tune.spgemm_long_rows == 1)
+            for (int k = LONG - 1; k > 0 && BIN_UB[k - 1] >= ctx->tune.spgemm_esc_min; --k) {
+                lub = BIN_UB[k - 1];
+                if (!P.counts[k]) continue;
+                APSPIE_HIP(hipMemcpyAsync(lists + LONG * int64_t(m) + P.counts[LONG], lists + k * int64_t(m),
+                                          size_t(P.counts[k]) * 4, hipMemcpyDeviceToDevice, ctx->stream));
+                P.counts[LONG] += P.counts[k];
+                P.counts[k] = 0;
+            }
         // global tables for the largest rows: 2 * pow2ceil(ub) slots each
         int64_t *goff = nullptr;
         int32_t *gslots = nullptr, *gkeys = nullptr;
@@ -1175,7 +1190,7 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             int32_t *ubs = (int32_t *)ctx->slot(S_OUTER0 + 13, size_t(std::max(m, 1)) * 4);
             scol = (int32_t *)ctx->slot(S_OUTER0 + 10, size_t(std::max<int64_t>(staged, 1)) * 4);
             sval = (float *)ctx->slot(S_OUTER0 + 11, size_t(std::max<int64_t>(staged, 1)) * 4);
-            k_stage_ub<<<g256, 256, 0, ctx->stream>>>(m, ub, ubs);
+            k_stage_ub<<<g256, 256, 0, ctx->stream>>>(m, ub, lub, ubs);
             cvt::scan32(ctx, ubs, ubpre, m);
             launch_bins<true>(ctx, P03, A, B, nnzc, ubpre, scol, sval, nullptr, nullptr, nullptr, nullptr);
         } else {
@@ -1201,7 +1216,7 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         // 5. numeric
         if (one_pass)
             k_unstage<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 256), (unsigned)ctx->cu_count * 8)), 256, 0,
-                        ctx->stream>>>(m, ub, ubpre, c->row_ptr, scol, sval, c->col_ind, c->val);
+                        ctx->stream>>>(m, ub, lub, ubpre, c->row_ptr, scol, sval, c->col_ind, c->val);
         else
             launch_bins<true>(ctx, P03, A, B, nnzc, c->row_ptr, c->col_ind, c->val, nullptr, nullptr, nullptr,
                               nullptr);

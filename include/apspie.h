@@ -154,6 +154,9 @@ typedef struct {
     int32_t spgemm_passes;    /* SpGEMM rows of the LDS tables: 1 = one pass into a staging area
                                  (default; two when their products reach 2^31), 2 = a counting pass,
                                  then the product pass */
+    int32_t spgemm_esc_min;   /* SpGEMM with spgemm_long_rows = 1: rows with more products than this go
+                                 to expand-sort-compress instead of the LDS tables; 256, 512, 1024
+                                 (default) or 2048 */
     int32_t bfs_group_cap;    /* BFS with a CSC: entries one round may append to a queue group region
                                  (16 regions); a round whose new frontier does not fit, and whose
                                  successor runs top-down, is queued by a pass over the vertex range

@@ -118,6 +118,28 @@ def test_spgemm_long_rows_both_ways(ctx, m, maxlen, seed):
         assert np.array_equal(c.row_ptr, ref.row_ptr) and np.array_equal(c.col_ind, ref.col_ind)
 
 
+@pytest.mark.parametrize("esc_min", [256, 512, 2048])
+def test_spgemm_esc_threshold_moves_rows_off_the_lds_tables(ctx, esc_min):
+    """spgemm_esc_min moves the LDS-table bins above it to the
+    expand-sort-compress rows (and out of the staging area): the product
+    matches the oracle at every threshold, and equals the default split
+    (1024) bit for bit."""
+    h = A.powerlaw(30000, 30000, 8.0, 4096, 5)
+    lp = long_row_products(h, h)
+    lo, hi = min(esc_min, 1024), max(esc_min, 1024)
+    assert ((lp > lo) & (lp <= hi)).sum() > 10  # rows that change side
+    assert ctx.tuning()["spgemm_esc_min"] == 1024
+    saved = ctx.tuning()
+    try:
+        base, _ = run(ctx, h, h)
+        ctx.set_tuning(spgemm_esc_min=esc_min)
+        got, _ = run(ctx, h, h)
+    finally:
+        ctx.set_tuning(**saved)
+    assert np.array_equal(got.row_ptr, base.row_ptr) and np.array_equal(got.col_ind, base.col_ind)
+    assert np.array_equal(got.val, base.val)
+
+
 def test_spgemm_long_rows_with_columns_outside_b(ctx):
     """A long row whose B rows hold columns outside B's range: those products
     are dropped by both paths (validate_csr does not check column ranges)."""

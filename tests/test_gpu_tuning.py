@@ -71,7 +71,7 @@ def test_out_of_range_knobs_are_refused(tuned):
     for bad in (dict(expand_wg_per_cu=0), dict(round_chunk=33), dict(batch_chunk=0), dict(sssp_batch_lanes=32),
                 dict(sssp_delta=-1), dict(sssp_delta=(1 << 24) + 1), dict(spgemm_long_rows=2),
                 dict(spgemm_long_rows=-1), dict(spgemm_passes=0), dict(spgemm_passes=3), dict(bfs_group_cap=-1),
-                dict(bfs_group_cap=(1 << 30) + 1)):
+                dict(bfs_group_cap=(1 << 30) + 1), dict(spgemm_esc_min=300), dict(spgemm_esc_min=4096)):
         with pytest.raises(A.ApspieError):
             ctx.set_tuning(**bad)
         assert ctx.tuning() == before
