@@ -69,7 +69,8 @@ enum Slot : int {
     S_AUX6,
     S_AUX7,
     S_OUTER0,                   // outer-product SpGEMM working arrays: S_OUTER0 + k, k < 40
-    S_NSLOTS = S_OUTER0 + 40
+    S_TC0 = S_OUTER0 + 40,      // triangle counting's partner arrays: S_TC0 + k, k < 12
+    S_NSLOTS = S_TC0 + 12
 };
 
 }  // namespace apspie

@@ -379,8 +379,10 @@ int apspie_spgemm_outer(apspie_ctx *ctx, const apspie_csr *a, const apspie_csr *
 
 /* Triangle count of an undirected graph from its strict lower triangle L
  * (row i holds the sorted columns j < i): *count = sum over (i,k) in L of
- * |L(i,:) & L(k,:)|, exact int64.  stats: edges = wedges checked,
- * vertices = nnz(L).  Replaces the TC pipeline of test/tc/main.cu:148-261
+ * |L(i,:) & L(k,:)|, exact int64.  stats: edges = wedges of that sum (sum
+ * over (i,k) of nnz(L(k,:))), vertices = nnz(L), improved = set probes made
+ * (each edge intersects the shorter of its two rows with the longer, so
+ * sum over (i,k) of min(nnz(L(i,:)), nnz(L(k,:)))).  Replaces the TC pipeline of test/tc/main.cu:148-261
  * (cuSPARSE L*L^T, ewiseMultTc src/tc.cuh:168-199, float mgpu::Reduce). */
 int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, apspie_stats *stats);
 
