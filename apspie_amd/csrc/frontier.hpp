@@ -84,7 +84,8 @@ __device__ __forceinline__ void control_clear(Control *C, int64_t gtid, int64_t 
 // host helpers shared by the frontier algorithms (traverse.hip)
 Queue queue_of(const Bufs &b);
 unsigned init_grid(const Bufs &b, int m);
-void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *stats);
+// (host_acc: the totals already copied to the host; else they are copied here)
+void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *stats, const Acc *host_acc = nullptr);
 
 // Compaction.  Wave w of workgroup b owns vertices [b*CHUNK + w*SEGV, +SEGV)
 // and walks them in ROUNDS rounds of 64 consecutive vertices; in-wave ranks

@@ -37,6 +37,8 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::apspie::Error((code), (msg));    \
     } while (0)
 
+constexpr size_t PINNED_BYTES = 16384;  // pinned host staging per context
+
 // Named slots of the grow-only scratch pool.  A slot keeps the largest size
 // asked of it; memory is released only when the context is destroyed.
 enum Slot : int {
@@ -83,7 +85,7 @@ struct apspie_ctx {
     std::string err;
     void *slot_ptr[apspie::S_NSLOTS] = {};
     size_t slot_bytes[apspie::S_NSLOTS] = {};
-    void *pinned = nullptr;  // 4 KiB pinned host staging for counters
+    void *pinned = nullptr;  // PINNED_BYTES of pinned host staging for counters
     std::mutex mu;
 
     // Per-kernel-class event timing (off unless enabled).

@@ -102,7 +102,7 @@ int apspie_ctx_create(int device, apspie_ctx **out) {
         APSPIE_HIP(hipEventCreate(&ctx->ev0));
         APSPIE_HIP(hipEventCreate(&ctx->ev1));
         APSPIE_HIP(hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device));
-        APSPIE_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        APSPIE_HIP(hipHostMalloc(&ctx->pinned, PINNED_BYTES, hipHostMallocDefault));
         trav::noalloc_setup(device);
     });
     if (rc != APSPIE_OK) {

@@ -757,10 +757,14 @@ unsigned init_grid(const Bufs &b, int m) {
     return std::max(1u, std::min(ceil_div(std::max<int64_t>(m, int64_t(b.nblk) * CHUNK / 16), WG), 4096u));
 }
 
-void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *stats) {
+void fill_stats(apspie_ctx *ctx, const Bufs &b, int mode, int m, apspie_stats *stats, const Acc *host_acc) {
     Acc acc;
-    APSPIE_HIP(hipMemcpyAsync(&acc, &b.C->acc, sizeof(Acc), hipMemcpyDeviceToHost, ctx->stream));
-    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    if (host_acc) {
+        acc = *host_acc;
+    } else {
+        APSPIE_HIP(hipMemcpyAsync(&acc, &b.C->acc, sizeof(Acc), hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    }
     ctx->prof_collect();
     if (!stats) return;
     float ms = 0.f;
@@ -946,8 +950,11 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     // traversals end within 6-12 rounds, and a round launched past the end
     // costs its kernels' launch
     int level = 1, chunk = ctx->tune.round_chunk, checks = 0;
-    // the descriptors, the round counter and the rebuild mark lead the control block
-    static_assert(offsetof(Control, rebuild) + sizeof(int) <= 4096, "fits the pinned staging");
+    // the descriptors, the round counter, the rebuild mark and the totals lead
+    // the control block: one copy after each chunk of rounds reads them all
+    // (the totals of the last chunk are the call's, with no further copy)
+    constexpr size_t LEAD = offsetof(Control, acc) + sizeof(Acc);
+    static_assert(LEAD <= PINNED_BYTES, "fits the pinned staging");
     volatile Control *host_c = (volatile Control *)ctx->pinned;
     const bool bfs_rounds = MODE == MODE_BFS && pull;
     for (;;) {
@@ -960,8 +967,8 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             compact(level);
         }
         APSPIE_HIP(hipGetLastError());
-        APSPIE_HIP(hipMemcpyAsync((void *)host_c, b.C, offsetof(Control, rebuild) + sizeof(int),
-                                  hipMemcpyDeviceToHost, ctx->stream));
+        APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        APSPIE_HIP(hipMemcpyAsync((void *)host_c, b.C, LEAD, hipMemcpyDeviceToHost, ctx->stream));
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         if (bfs_rounds && host_c->rebuild) {  // the later launches of the chunk were void
             level = host_c->rebuild - 1;
@@ -978,8 +985,9 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         ++checks;
         chunk = checks <= 2 ? 2 : std::min(chunk * 2, 32);
     }
-    APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    fill_stats(ctx, b, MODE, g->m, stats);
+    Acc acc;
+    memcpy(&acc, (const void *)&host_c->acc, sizeof(Acc));
+    fill_stats(ctx, b, MODE, g->m, stats, &acc);
 }
 
 void validate_traversal(const apspie_csr *g, int32_t src) {

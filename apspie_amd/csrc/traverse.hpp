@@ -78,13 +78,13 @@ struct Control {
     // over its bitmap before the next round runs (0: none); rounds exit at
     // once while it is set
     int rebuild;
+    Acc acc;  // read with the descriptors once a traversal ends
     // per round parity and group: (entries << 32) | edges, and chunks done
     Padded<unsigned long long> gcnt[2][NGROUP];
     Padded<unsigned int> gdone[2][NGROUP];
     unsigned int alldone[2];    // groups whose chunks are all done
     int32_t gtile[2][NGROUP + 1];  // first tile of each group in the round's tile numbering
     Padded<unsigned int> nnew[2][SPREAD];  // vertices picked by the compaction, by round parity
-    Acc acc;
     // SSSP near/far: a changed vertex enters the frontier only while its
     // distance is below thr (float bits; all ones: no threshold); the others
     // wait, counted (ndefer) with their smallest distance (dmin) by parity
