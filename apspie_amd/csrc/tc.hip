@@ -13,10 +13,14 @@
 //   1. ew[e] = nnz(L(y,:)) for every partner entry e = (x,y) of P (the wedges
 //      it opens), and gpre = its int64 exclusive scan (single-pass look-back);
 //      row x owns wedges [gpre[prp[x]], gpre[prp[x+1]]).
-//   2. Work items (i, c) cover wedges [c*CW, (c+1)*CW) of row i and are binned
-//      by nnz(L(i,:)): <= 256 (one wave per item, 512-slot LDS set), <= 4096
-//      (256 threads, 8192-slot LDS set), larger (binary search of the sorted
-//      row in global memory).
+//   2. Work items (i, c) cover wedges [c*CW, (c+1)*CW) of row i (CW by bin:
+//      an item rebuilds its row's set, so bins of larger sets take larger
+//      items) and are binned by nnz(L(i,:)): <= 256 (one wave per item,
+//      512-slot LDS set), <= 1024 / 2048 / 4096 (256 threads, 2048 / 4096 /
+//      8192 slots; the smaller the set, the more workgroups share a CU, and
+//      the faster they probe), larger (binary search of the sorted row in
+//      global memory: a 16384-slot set leaves one workgroup a CU and probed
+//      slower).
 //   3. Per item: the mask row L(i,:) goes into an LDS hash set behind a bit
 //      filter (most probes are of absent columns and stop there); the item's
 //      first edge comes from a binary search of gpre; its wedges (k, j) are
@@ -40,9 +44,10 @@ namespace tcx {
 
 using namespace dev;
 
-constexpr int CW = 16384;  // wedges per work item
-constexpr int NB = 3;
-constexpr int BIN_DEG[NB - 1] = {256, 4096};
+// wedges per work item, by bin
+__host__ __device__ constexpr int cw_of(int bin) { return bin == 3 ? 65536 : bin == 1 || bin == 2 ? 32768 : 16384; }
+constexpr int NB = 5;
+constexpr int BIN_DEG[NB - 1] = {256, 1024, 2048, 4096};
 
 __global__ __launch_bounds__(256) void k_setup(int64_t nnz, const int32_t *__restrict__ rp,
                                                const int32_t *__restrict__ col, int32_t *ew, int nscan,
@@ -200,8 +205,8 @@ __global__ __launch_bounds__(256) void k_items(int pass, int m, const int32_t *_
             if (wi >= (int64_t(1) << 31)) {  // per-row wedge offsets are int32
                 *bad = 1;
             } else if (wi > 0) {
-                n = int((wi + CW - 1) / CW);
                 b = bin_of(rp[i + 1] - rp[i]);
+                n = int((wi + cw_of(b) - 1) / cw_of(b));
             }
         }
 #pragma unroll
@@ -229,7 +234,7 @@ __device__ __forceinline__ uint32_t fslot(int j, int shift) { return (uint32_t(j
 
 // Item (i, c): the wedges [c * CW, (c + 1) * CW) of row i's partners (prp,
 // pcol), each partner k's L row scanned against the set of L(i,:) (rp, col).
-template <int T, int S, bool LDSMASK>
+template <int T, int S, bool LDSMASK, int CW>
 __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int nitems, const int32_t *__restrict__ rp,
                                              const int32_t *__restrict__ col, const int32_t *__restrict__ prp,
                                              const int32_t *__restrict__ pcol, const int64_t *__restrict__ gpre,
@@ -435,14 +440,20 @@ extern "C" int apspie_tc(apspie_ctx *ctx, const apspie_csr *l, int64_t *count, a
         k_items<<<gm, 256, 0, ctx->stream>>>(1, m, prp, l->row_ptr, gpre, counts, lists, bin_base, bad);
         const unsigned cu = (unsigned)ctx->cu_count;
         if (hc[0])
-            k_count<64, 512, true><<<std::min((unsigned)hc[0], cu * 16), 64, 0, ctx->stream>>>(
+            k_count<64, 512, true, cw_of(0)><<<std::min((unsigned)hc[0], cu * 16), 64, 0, ctx->stream>>>(
                 lists + hb[0], hc[0], l->row_ptr, l->col_ind, prp, pcol, gpre, acc);
         if (hc[1])
-            k_count<256, 8192, true><<<std::min((unsigned)hc[1], cu * 4), 256, 0, ctx->stream>>>(
+            k_count<256, 2048, true, cw_of(1)><<<std::min((unsigned)hc[1], cu * 8), 256, 0, ctx->stream>>>(
                 lists + hb[1], hc[1], l->row_ptr, l->col_ind, prp, pcol, gpre, acc);
         if (hc[2])
-            k_count<256, 1, false><<<std::min((unsigned)hc[2], cu * 8), 256, 0, ctx->stream>>>(
+            k_count<256, 4096, true, cw_of(2)><<<std::min((unsigned)hc[2], cu * 8), 256, 0, ctx->stream>>>(
                 lists + hb[2], hc[2], l->row_ptr, l->col_ind, prp, pcol, gpre, acc);
+        if (hc[3])
+            k_count<256, 8192, true, cw_of(3)><<<std::min((unsigned)hc[3], cu * 4), 256, 0, ctx->stream>>>(
+                lists + hb[3], hc[3], l->row_ptr, l->col_ind, prp, pcol, gpre, acc);
+        if (hc[4])
+            k_count<256, 1, false, cw_of(4)><<<std::min((unsigned)hc[4], cu * 8), 256, 0, ctx->stream>>>(
+                lists + hb[4], hc[4], l->row_ptr, l->col_ind, prp, pcol, gpre, acc);
         APSPIE_HIP(hipGetLastError());
         APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         unsigned long long tri = 0;
