@@ -227,9 +227,10 @@ __global__ __launch_bounds__(256) void k_items(int pass, int m, const int32_t *_
 __device__ __forceinline__ uint32_t hslot(int j, uint32_t mask) { return (uint32_t(j) * 2654435761u) & mask; }
 
 // Bit filter in front of the LDS hash set: a row whose set has sz slots sets
-// one of 32 sz bits per entry (index from the high bits of another
-// multiplicative hash), so at most 1 in 64 probes of an absent column passes
-// it; most wedges do not close a triangle and end at one LDS read.
+// one of FB sz bits per entry (index from the high bits of another
+// multiplicative hash; FB = 32, or 16 for the largest sets), so at most 1 in
+// 2 FB probes of an absent column passes it; most wedges do not close a
+// triangle and end at one LDS read.
 __device__ __forceinline__ uint32_t fslot(int j, int shift) { return (uint32_t(j) * 0x85ebca6bu) >> shift; }
 
 // Item (i, c): the wedges [c * CW, (c + 1) * CW) of row i's partners (prp,
@@ -240,7 +241,10 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
                                              const int32_t *__restrict__ pcol, const int64_t *__restrict__ gpre,
                                              unsigned long long *total) {
     __shared__ int32_t s_set[LDSMASK ? S : 1];
-    __shared__ uint32_t s_filt[LDSMASK ? S : 1];
+    // the filter: FB bits per set slot (16 for the sets of 4096 slots and
+    // more, whose LDS footprint sets how many workgroups share a CU)
+    constexpr int FB = S >= 4096 ? 16 : 32;
+    __shared__ uint32_t s_filt[LDSMASK ? S * FB / 32 : 1];
     __shared__ int32_t s_bs[T];
     __shared__ int32_t s_pre[T];
     __shared__ int32_t s_tmp[T / WAVE + 1];
@@ -254,14 +258,14 @@ __global__ __launch_bounds__(T) void k_count(const int2 *__restrict__ items, int
         const int wi = int(gpre[ie] - g0);
         const int lo = c * CW, hi = min(lo + CW, wi);
         uint32_t mask = 0;
-        int fshift = 0;  // filter of 32 sz bits: index = top log2(32 sz) bits of fslot's product
+        int fshift = 0;  // filter of FB sz bits: index = top log2(FB sz) bits of fslot's product
         if (LDSMASK) {
             int sz = 64;
             while (sz < 2 * deg) sz <<= 1;
             mask = uint32_t(sz - 1);
-            fshift = __clz(sz) - 4;  // 32 - log2(32 sz)
+            fshift = __clz(sz) - (FB == 32 ? 4 : 3);  // 32 - log2(FB sz)
             for (int s = tid; s < sz; s += T) s_set[s] = -1;
-            for (int s = tid; s < sz; s += T) s_filt[s] = 0u;
+            for (int s = tid; s < sz * FB / 32; s += T) s_filt[s] = 0u;
             __syncthreads();
             for (int p = mb + tid; p < me; p += T) {
                 const int j = col[p];
