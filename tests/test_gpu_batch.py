@@ -59,6 +59,35 @@ def test_sssp_batch_rows_match_single(ctx, k):
         assert np.array_equal(dist[s], want), (k, s)
 
 
+def path_graph(m):
+    """Undirected path 0 - 1 - ... - (m - 1)."""
+    a = np.arange(m - 1, dtype=np.int64)
+    u = np.concatenate([a, a + 1])
+    v = np.concatenate([a + 1, a])
+    order = np.lexsort((v, u))
+    u, v = u[order], v[order]
+    rp = np.zeros(m + 1, np.int32)
+    np.add.at(rp, u + 1, 1)
+    rp = np.cumsum(rp).astype(np.int32)
+    return A.HostCsr(m, m, rp, v.astype(np.int32), np.ones(len(v), np.float32))
+
+
+@pytest.mark.parametrize("k", [3, 70, 200])
+def test_bfs_batch_on_a_long_path(ctx, k):
+    """A traversal of hundreds of rounds (many captured chunks, the last one
+    partly idle): a 700-vertex path from its ends and spread vertices,
+    depths up to 699 (past what one byte holds)."""
+    h = path_graph(700)
+    g = ctx.upload(h)
+    srcs = np.linspace(0, 699, k).astype(np.int32)
+    srcs[0], srcs[-1] = 0, 699
+    depth, st = ctx.bfs_batch(g, srcs)
+    o = ocsr(h)
+    for s, v in enumerate(srcs):
+        assert np.array_equal(depth[s], O.bfs(o, int(v))[0]), (k, s)
+    assert depth.max() == 699
+
+
 def test_batch_limits(ctx):
     h = A.rmat(8, 4, 1, undirected=True)
     g = ctx.upload(h)
