@@ -86,6 +86,7 @@ class Tuning(C.Structure):
         ("spgemm_passes", C.c_int32),
         ("spgemm_esc_min", C.c_int32),
         ("bfs_group_cap", C.c_int32),
+        ("sssp_hint", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -1,27 +1,33 @@
-// binned.hpp -- binned relaxation of dense SSSP rounds (propagation blocking:
-// Beamer, Asanovic, Patterson, "Reducing PageRank Communication via
-// Propagation Blocking", IPDPS 2017), the gfx950 replacement for the random
-// 4-byte distance probes and atomics a push round makes per edge.
+// binned.hpp -- the distance bound codes of the SSSP rounds, and the binned
+// relaxation of dense SSSP rounds (propagation blocking: Beamer, Asanovic,
+// Patterson, "Reducing PageRank Communication via Propagation Blocking",
+// IPDPS 2017), which replaces the random 4-byte distance probes and atomics
+// a push round makes per edge with streams.
 //
-//   k_bin_relax    a persistent grid of G workgroups; workgroup w owns the
-//                  frontier tiles [w T / G, (w + 1) T / G) and the same stretch
-//                  of the entry buffer.  Pass 1 counts its candidates per
-//                  bucket (target >> BUCKET_SHIFT) in LDS; pass 2 writes every
-//                  candidate (target, d_u + w) into its bucket's run inside the
-//                  stretch.  One row of G x (nb + 1) run offsets per
-//                  workgroup; per-bucket totals for the apply's split.  All
-//                  global traffic is streams (the column ids are read twice).
-//   k_apply_relax  one workgroup per (bucket, part): the bucket's slice of the
+//   k_bin_expand   G workgroups over the round's tiles (t = w, w + G, ...).
+//                  Each edge's candidate d_u + w is kept when it is below its
+//                  target's bound code (all are kept without codes); the kept
+//                  (target, candidate) pairs are appended, wave-compacted, to
+//                  the workgroup's own region of the candidate buffer (no
+//                  global atomics), and counted per bucket (target >>
+//                  BUCKET_SHIFT) in LDS; the counts go to a bucket-major
+//                  table row.
+//   k_bin_scan     one workgroup per bucket: exclusive scan of the bucket's
+//                  counts over the G workgroups, and the bucket's total.
+//   k_bin_scatter  the G regions again, in chunks sorted by bucket in LDS,
+//                  written to their places in the bucket-ordered buffer (one
+//                  contiguous run per bucket a chunk).
+//   k_bin_apply    one workgroup per (bucket, part): the bucket's slice of the
 //                  distance array (BUCKET vertices, 64 KB) sits in LDS, the
-//                  bucket's runs of the part's bin workgroups are folded in
-//                  with LDS atomicMin, and the improved distances are written
-//                  back (atomicMin when other parts share the bucket).  Heavy
-//                  buckets (the RMAT hubs) are split into parts of about
-//                  PART_ENTRIES candidates.
+//                  part's candidates are folded in with LDS atomicMin, and
+//                  the improved distances are written back (atomicMin when
+//                  other parts share the bucket).  Heavy buckets (the RMAT
+//                  hubs) are split into parts of about PART_ENTRIES.
 //
 // The round computes exactly what the push round does (every candidate of the
-// frontier snapshot, min-reduced into the live distances), so the schedule,
-// the counters and the result are the same.
+// frontier snapshot, min-reduced into the distances; a candidate at or past
+// its target's bound could not lower it), so the schedule, the counters and
+// the result are the same.
 #pragma once
 
 #include "frontier.hpp"
@@ -29,87 +35,81 @@
 namespace apspie {
 namespace trav {
 
+// Distance bound codes: a byte per vertex whose decoded value bounds the
+// vertex's distance from above.  The compaction writes the code of a
+// vertex's distance whenever it dropped, and distances only fall during a
+// round, so at every moment of a round the bound holds.  An edge whose
+// candidate is not below its target's bound cannot lower the target's
+// distance and needs no 4-byte distance probe (push) or candidate entry
+// (binned); the byte array is a quarter of the distances' size, so its random
+// probes hit the caches far more often.  Codes: 0 -> 0; c = 1..127 -> c;
+// c = 128..254 -> the float whose bits are those of 127.0 plus (c - 127) <<
+// 17 (128.0 in steps of 2 to 256.0, then steps of 4 to 504.0); 255 -> no
+// bound.  Integer distances up to 128 are coded exactly.
+constexpr uint32_t HINT_F127 = 0x42fe0000u;  // bits of 127.0f
+__device__ __forceinline__ uint32_t hint_code(uint32_t bits) {
+    if (bits == 0u) return 0u;
+    if (bits <= HINT_F127) return uint32_t(ceilf(__uint_as_float(bits)));  // 1..127, at least the distance
+    const uint32_t q = (bits - HINT_F127 + 0x1ffffu) >> 17;               // rounds up
+    return q <= 127u ? 127u + q : 255u;
+}
+__device__ __forceinline__ uint32_t hint_bound(uint32_t c) {
+    if (c <= 127u) return __float_as_uint(float(c));  // 0 -> bits 0
+    return c == 255u ? ~0u : HINT_F127 + ((c - 127u) << 17);
+}
+
 constexpr int BUCKET_SHIFT = 14;
 constexpr int BUCKET = 1 << BUCKET_SHIFT;  // vertices per bucket (64 KB of distances)
 constexpr int MAX_BUCKETS = 512;           // binned rounds need m <= MAX_BUCKETS * BUCKET
 constexpr int WGA = 1024;                  // threads per apply workgroup
 constexpr int PART_ENTRIES = 1 << 18;      // candidates per apply workgroup (heavy buckets split)
+constexpr int SCH = 2048;                  // entries per scatter chunk (8 per thread)
 
-// Frontier tiles of bin workgroup w among G: [tile_lo(w), tile_lo(w + 1)).
-__device__ __forceinline__ int bin_tile_lo(int w, int G, int ntiles) { return int(int64_t(w) * ntiles / G); }
+// Entries of each bin workgroup's region for a round of ntiles tiles over G
+// workgroups: a workgroup expands at most ceil(ntiles / G) tiles.  Region w
+// starts at w * bin_cap.
+__device__ __forceinline__ int64_t bin_cap(int ntiles, int G) { return int64_t((ntiles + G - 1) / G) * TILE; }
 
-// Entry buffer: >= ntiles * TILE uint2 (target, candidate bits); table: G *
-// (nb + 1) int32 run offsets; totals: 2 parities * MAX_BUCKETS.
-__global__ __launch_bounds__(WG) void k_bin_relax(int level_arg, int m, int nb, const Control *C, Queue q,
-                                                  const int32_t *__restrict__ col_ind,
-                                                  const float *__restrict__ val, uint2 *__restrict__ out,
-                                                  int32_t *__restrict__ tbl, uint32_t *totals) {
+// Exclusive scan of x[0..n) (n <= MAX_BUCKETS) by one full wave into dst;
+// returns the total.  dst may be written with `add[b]` added (null: none).
+__device__ __forceinline__ uint32_t bucket_scan(const int32_t *x, int n, uint32_t *dst, int lane) {
+    uint32_t carry = 0;
+    for (int b0 = 0; b0 < n; b0 += WAVE) {
+        const int b = b0 + lane;
+        const uint32_t h = b < n ? uint32_t(x[b]) : 0u;
+        const uint32_t incl = wave_incl_scan32(h, lane);
+        if (b < n) dst[b] = carry + incl - h;
+        carry += __shfl(incl, WAVE - 1, WAVE);
+    }
+    return carry;
+}
+
+// Candidate regions: G * bin_cap(nTiles, G) uint2 (target, candidate bits);
+// table: nb * G int32 per-bucket counts (bucket-major); wcount: G int32.
+template <bool HINT>
+__global__ __launch_bounds__(WG) void k_bin_expand(int level_arg, int m, int nb, const Control *C, Queue q,
+                                                   const int32_t *__restrict__ col_ind,
+                                                   const float *__restrict__ val, const uint8_t *__restrict__ hint,
+                                                   uint2 *__restrict__ out, int32_t *__restrict__ tbl,
+                                                   int32_t *__restrict__ wcount) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];
     __shared__ uint32_t s_val[TILE + 1];
     __shared__ int32_t s_vert[1];
     __shared__ int32_t s_wmax[NW];
-    __shared__ uint32_t s_hist[MAX_BUCKETS];  // counts, then fill cursors
-    __shared__ uint32_t s_off[MAX_BUCKETS + 1];
-    __shared__ uint32_t s_tcnt[MAX_BUCKETS];   // pass 2: this tile's candidates per bucket
-    __shared__ uint32_t s_toff[MAX_BUCKETS];   // pass 2: their offset in the tile's sorted order
-    __shared__ uint32_t s_gpos[MAX_BUCKETS];   // pass 2: their position in the entry buffer
-    __shared__ uint2 s_sorted[TILE];           // pass 2: the tile's candidates sorted by bucket
-    __shared__ uint32_t s_tn;
+    __shared__ uint32_t s_hist[MAX_BUCKETS];
+    __shared__ uint32_t s_wcnt[NW];
     const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-    // every round clears the other parity's totals for the next round (their
-    // last reader, an earlier apply, has finished); the first two rounds'
-    // totals start cleared by the traversal's initialisation
-    if (blockIdx.x == 0)
-        for (int b = tid; b < MAX_BUCKETS; b += WG) totals[((level & 1) ^ 1) * MAX_BUCKETS + b] = 0;
     const Round cur = round_of(C, level);
     if (cur.pull != DIR_BINNED) return;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int G = gridDim.x, w = blockIdx.x;
-    const int t0 = bin_tile_lo(w, G, cur.nTiles), t1 = bin_tile_lo(w + 1, G, cur.nTiles);
+    uint2 *reg = out + size_t(w) * size_t(bin_cap(cur.nTiles, G));
     for (int b = tid; b < nb; b += WG) s_hist[b] = 0;
-    // pass 1: count
-    for (int t = t0; t < t1; ++t) {
-        int own[IPT];
-        const int elim = tile_owners<false, false>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
-        int col[IPT];
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) {
-            const int s = wid * SEG + r * WAVE + lane;
-            col[r] = s < elim ? col_ind[s_base[own[r]] + s] : -1;
-        }
-#pragma unroll
-        for (int r = 0; r < IPT; ++r)
-            if ((unsigned)col[r] < (unsigned)m) atomicAdd(&s_hist[col[r] >> BUCKET_SHIFT], 1u);
-        __syncthreads();  // the next tile_owners rewrites the staging
-    }
     __syncthreads();
-    if (wid == 0) {
-        uint32_t carry = 0;
-        for (int b0 = 0; b0 < nb; b0 += WAVE) {
-            const int b = b0 + lane;
-            const uint32_t h = b < nb ? s_hist[b] : 0u;
-            const uint32_t incl = wave_incl_scan32(h, lane);
-            if (b < nb) s_off[b] = carry + incl - h;
-            carry += __shfl(incl, WAVE - 1, WAVE);
-        }
-        if (lane == 0) s_off[nb] = carry;
-    }
-    __syncthreads();
-    int32_t *row = tbl + size_t(w) * (nb + 1);
-    uint32_t *tot = totals + (level & 1) * MAX_BUCKETS;
-    for (int b = tid; b <= nb; b += WG) row[b] = int32_t(s_off[b]);
-    for (int b = tid; b < nb; b += WG) {
-        if (s_hist[b]) atomicAdd(&tot[b], s_hist[b]);
-        s_hist[b] = 0;
-    }
-    __syncthreads();
-    // pass 2: sort each tile's candidates by bucket in LDS, then write them
-    // out in that order, so a wave's stores land as runs of its buckets
-    uint2 *dst = out + size_t(t0) * TILE;
-    for (int b = tid; b < nb; b += WG) s_tcnt[b] = 0;
-    for (int t = t0; t < t1; ++t) {
+    int cursor = 0;  // the same in every thread
+    for (int t = w; t < cur.nTiles; t += G) {
         int own[IPT];
         const int elim = tile_owners<true, false>(t, cur, q, s_own, s_base, s_val, s_vert, s_wmax, own);
         int pos[IPT], col[IPT];
@@ -121,127 +121,175 @@ __global__ __launch_bounds__(WG) void k_bin_relax(int level_arg, int m, int nb, 
 #pragma unroll
         for (int r = 0; r < IPT; ++r) col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
         float wt[IPT];
+        uint32_t hc[IPT];
 #pragma unroll
-        for (int r = 0; r < IPT; ++r) wt[r] = (val && (unsigned)col[r] < (unsigned)m) ? val[pos[r]] : 1.0f;
-        uint32_t cand[IPT], rank[IPT];
+        for (int r = 0; r < IPT; ++r) {
+            const bool ok = (unsigned)col[r] < (unsigned)m;
+            wt[r] = (val && ok) ? val[pos[r]] : 1.0f;
+            hc[r] = (HINT && ok) ? uint32_t(hint[col[r]]) : 255u;
+        }
+        uint32_t cand[IPT];
+        uint32_t keep = 0;
 #pragma unroll
-        for (int r = 0; r < IPT; ++r)
-            if ((unsigned)col[r] < (unsigned)m) {
-                cand[r] = __float_as_uint(__uint_as_float(s_val[own[r]]) + wt[r]);
-                rank[r] = atomicAdd(&s_tcnt[col[r] >> BUCKET_SHIFT], 1u);
-            }
+        for (int r = 0; r < IPT; ++r) {
+            cand[r] = __float_as_uint(__uint_as_float(s_val[own[r]]) + wt[r]);
+            if ((unsigned)col[r] < (unsigned)m && cand[r] < hint_bound(hc[r])) keep |= 1u << r;
+        }
+        const uint32_t wn = wave_total32(uint32_t(__popc(keep)));
+        if (lane == 0) s_wcnt[wid] = wn;
         __syncthreads();
-        if (wid == 0) {  // tile offsets and reserved buffer positions per bucket
-            uint32_t carry = 0;
-            for (int b0 = 0; b0 < nb; b0 += WAVE) {
-                const int b = b0 + lane;
-                const uint32_t h = b < nb ? s_tcnt[b] : 0u;
-                const uint32_t incl = wave_incl_scan32(h, lane);
-                if (b < nb) {
-                    s_toff[b] = carry + incl - h;
-                    s_gpos[b] = s_off[b] + s_hist[b];
-                    s_hist[b] += h;
-                    s_tcnt[b] = 0;  // ready for the next tile
-                }
-                carry += __shfl(incl, WAVE - 1, WAVE);
+        int wbase = 0, tn = 0;
+#pragma unroll
+        for (int x = 0; x < NW; ++x) {
+            if (x < wid) wbase += int(s_wcnt[x]);
+            tn += int(s_wcnt[x]);
+        }
+        int at = cursor + wbase;
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) {
+            const bool k = (keep >> r) & 1u;
+            const uint64_t ball = __ballot(k);
+            if (k) {
+                const int rank =
+                    __builtin_amdgcn_mbcnt_hi(uint32_t(ball >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(ball), 0));
+                reg[at + rank] = make_uint2(uint32_t(col[r]), cand[r]);
+                atomicAdd(&s_hist[col[r] >> BUCKET_SHIFT], 1u);
             }
-            if (lane == 0) s_tn = carry;
+            at += __popcll(ball);
+        }
+        cursor += tn;
+        __syncthreads();  // s_wcnt and the tile staging are rewritten next tile
+    }
+    for (int b = tid; b < nb; b += WG) tbl[size_t(b) * G + w] = int32_t(s_hist[b]);
+    if (tid == 0) wcount[w] = cursor;
+}
+
+// Bucket b's counts over the G workgroups become their exclusive offsets
+// inside the bucket; tot[b] gets the bucket's total.  Grid: nb workgroups.
+__global__ __launch_bounds__(256) void k_bin_scan(int level_arg, int nb, int G, const Control *C,
+                                                  int32_t *__restrict__ tbl, int32_t *__restrict__ tot) {
+    __shared__ uint32_t s_w[4];
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    if (C->ctl[(level - 1) & 1].pull != DIR_BINNED) return;
+    const int b = blockIdx.x;
+    if (b >= nb) return;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    int32_t *row = tbl + size_t(b) * G;
+    uint32_t carry = 0;
+    for (int i0 = 0; i0 < G; i0 += 256) {
+        const int i = i0 + tid;
+        const uint32_t x = i < G ? uint32_t(row[i]) : 0u;
+        const uint32_t incl = wave_incl_scan32(x, lane);
+        if (lane == WAVE - 1) s_w[wid] = incl;
+        __syncthreads();
+        uint32_t wpre = 0, all = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < wid) wpre += s_w[k];
+            all += s_w[k];
+        }
+        if (i < G) row[i] = int32_t(carry + wpre + incl - x);
+        carry += all;
+        __syncthreads();
+    }
+    if (tid == 0) tot[b] = int32_t(carry);
+}
+
+// Region w's candidates, SCH at a time, sorted by bucket in LDS and written
+// to bucket b's run at (first entry of b) + (w's offset inside b) + (what w
+// already wrote to b).  Grid: the G workgroups of k_bin_expand.
+__global__ __launch_bounds__(WG) void k_bin_scatter(int level_arg, int nb, const Control *C,
+                                                    const uint2 *__restrict__ out, const int32_t *__restrict__ tbl,
+                                                    const int32_t *__restrict__ tot,
+                                                    const int32_t *__restrict__ wcount, uint2 *__restrict__ sorted) {
+    constexpr int K = SCH / WG;
+    __shared__ uint32_t s_cur[MAX_BUCKETS];  // where this workgroup's next entry of each bucket goes
+    __shared__ uint32_t s_cnt[MAX_BUCKETS];  // this chunk's entries per bucket
+    __shared__ uint32_t s_off[MAX_BUCKETS];  // their first place in the chunk's sorted order
+    __shared__ uint2 s_sorted[SCH];
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    const Ctl cur = C->ctl[(level - 1) & 1];
+    if (cur.pull != DIR_BINNED) return;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int G = gridDim.x, w = blockIdx.x;
+    if (wid == 0) bucket_scan(tot, nb, s_cur, lane);
+    for (int b = tid; b < nb; b += WG) s_cnt[b] = 0;
+    __syncthreads();
+    for (int b = tid; b < nb; b += WG) s_cur[b] += uint32_t(tbl[size_t(b) * G + w]);
+    const int n = wcount[w];
+    const uint2 *reg = out + size_t(w) * size_t(bin_cap(cur.nTiles, G));
+    for (int c0 = 0; c0 < n; c0 += SCH) {
+        const int cn = min(SCH, n - c0);
+        uint2 e[K];
+        uint32_t rk[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = k * WG + tid;
+            if (i < cn) {
+                e[k] = reg[c0 + i];
+                rk[k] = atomicAdd(&s_cnt[e[k].x >> BUCKET_SHIFT], 1u);
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int r = 0; r < IPT; ++r)
-            if ((unsigned)col[r] < (unsigned)m)
-                s_sorted[s_toff[col[r] >> BUCKET_SHIFT] + rank[r]] = make_uint2(uint32_t(col[r]), cand[r]);
+        if (wid == 0) bucket_scan((const int32_t *)s_cnt, nb, s_off, lane);
         __syncthreads();
-        const int tn = int(s_tn);
-        for (int i = tid; i < tn; i += WG) {
-            const uint2 e = s_sorted[i];
-            const int b = int(e.x >> BUCKET_SHIFT);
-            dst[s_gpos[b] + (uint32_t(i) - s_toff[b])] = e;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int i = k * WG + tid;
+            if (i < cn) s_sorted[s_off[e[k].x >> BUCKET_SHIFT] + rk[k]] = e[k];
+        }
+        __syncthreads();
+        for (int i = tid; i < cn; i += WG) {
+            const uint2 x = s_sorted[i];
+            const int b = int(x.x >> BUCKET_SHIFT);
+            sorted[s_cur[b] + (uint32_t(i) - s_off[b])] = x;
+        }
+        __syncthreads();
+        for (int b = tid; b < nb; b += WG) {
+            s_cur[b] += s_cnt[b];
+            s_cnt[b] = 0;
         }
         __syncthreads();
     }
 }
 
-// Workgroup (bucket b, part p of S_b): S_b = ceil(total_b / PART_ENTRIES),
-// parts split the G bin workgroups' runs evenly.  Launch at least
-// nb + nnz / PART_ENTRIES + 1 workgroups; the surplus exits.
-__global__ __launch_bounds__(WGA) void k_apply_relax(int level_arg, int m, int nb, int G, const Control *C,
-                                                     const uint2 *__restrict__ out,
-                                                     const int32_t *__restrict__ tbl,
-                                                     const uint32_t *__restrict__ totals, uint32_t *dist) {
-    constexpr int NWA = WGA / WAVE;
+// Workgroup (bucket b, part p of S_b): S_b = ceil(tot[b] / PART_ENTRIES),
+// parts split the bucket's run evenly.  Launch at least nb + nnz /
+// PART_ENTRIES + 1 workgroups; the surplus exits.
+__global__ __launch_bounds__(WGA) void k_bin_apply(int level_arg, int m, int nb, const Control *C,
+                                                   const uint2 *__restrict__ sorted, const int32_t *__restrict__ tot,
+                                                   uint32_t *dist) {
     __shared__ uint32_t s_d[BUCKET];
-    __shared__ int32_t s_first[MAX_BUCKETS + 1];  // first workgroup of each bucket
-    __shared__ int32_t s_pre[WGA + 1];            // exclusive prefix of the part's run lengths
-    __shared__ int64_t s_at[WGA];                 // the runs' first entries
-    __shared__ int32_t s_wsum[NWA];
+    __shared__ int32_t s_parts[MAX_BUCKETS];
+    __shared__ uint32_t s_first[MAX_BUCKETS + 1];  // first workgroup of each bucket
+    __shared__ uint32_t s_base[MAX_BUCKETS + 1];   // first entry of each bucket
     const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    const Ctl cur = C->ctl[(level - 1) & 1];
-    if (cur.pull != DIR_BINNED) return;
+    if (C->ctl[(level - 1) & 1].pull != DIR_BINNED) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
-    const uint32_t *tot = totals + (level & 1) * MAX_BUCKETS;
-    if (wid == 0) {
-        int carry = 0;
-        for (int b0 = 0; b0 < nb; b0 += WAVE) {
-            const int b = b0 + lane;
-            const uint32_t t = b < nb ? tot[b] : 0u;
-            const uint32_t parts = (t + PART_ENTRIES - 1) / PART_ENTRIES;
-            const uint32_t incl = wave_incl_scan32(parts, lane);
-            if (b < nb) s_first[b] = carry + int(incl - parts);
-            carry += int(__shfl(incl, WAVE - 1, WAVE));
-        }
-        if (lane == 0) s_first[nb] = carry;
-    }
+    for (int b = tid; b < nb; b += WGA) s_parts[b] = (tot[b] + PART_ENTRIES - 1) / PART_ENTRIES;
+    __syncthreads();
+    if (wid == 0) s_first[nb] = bucket_scan(s_parts, nb, s_first, lane);
+    if (wid == 1) s_base[nb] = bucket_scan(tot, nb, s_base, lane);
     __syncthreads();
     const int item = blockIdx.x;
-    if (item >= s_first[nb]) return;  // uniform
-    int lo = 0, hi = nb;              // last bucket whose first workgroup <= item
+    if (item >= int(s_first[nb])) return;  // uniform
+    int lo = 0, hi = nb;                   // last bucket whose first workgroup <= item
     while (hi - lo > 1) {
         const int mid = (lo + hi) / 2;
-        if (s_first[mid] <= item) lo = mid;
+        if (int(s_first[mid]) <= item) lo = mid;
         else hi = mid;
     }
     const int b = lo;
-    const int nparts = s_first[b + 1] - s_first[b];
-    const int part = item - s_first[b];
-    const int w0 = int(int64_t(part) * G / nparts), w1 = int(int64_t(part + 1) * G / nparts);
+    const int nparts = s_parts[b];
+    const int part = item - int(s_first[b]);
+    const int64_t tb = tot[b];
+    const int64_t e0 = s_base[b] + tb * part / nparts, e1 = s_base[b] + tb * (part + 1) / nparts;
     const int v0 = b << BUCKET_SHIFT;
     const int vn = min(BUCKET, m - v0);
     for (int i = tid; i < vn; i += WGA) s_d[i] = dist[v0 + i];
     __syncthreads();
-    // thread i takes the run of bin workgroup w0 + i (G <= WGA); the
-    // workgroup walks the concatenation of the runs, a candidate finding its
-    // run by binary search over the runs' prefix
-    const int nr = w1 - w0;
-    int len = 0;
-    int64_t at = 0;
-    if (tid < nr) {
-        const int32_t *row = tbl + size_t(w0 + tid) * (nb + 1);
-        const int r0 = row[b];
-        len = row[b + 1] - r0;
-        at = int64_t(bin_tile_lo(w0 + tid, G, cur.nTiles)) * TILE + r0;
-    }
-    const uint32_t incl = wave_incl_scan32(uint32_t(len), lane);
-    if (lane == WAVE - 1) s_wsum[wid] = int(incl);
-    __syncthreads();
-    int wpre = 0, total = 0;
-#pragma unroll
-    for (int x = 0; x < NWA; ++x) {
-        if (x < wid) wpre += s_wsum[x];
-        total += s_wsum[x];
-    }
-    s_pre[tid] = wpre + int(incl) - len;
-    s_at[tid] = at;
-    __syncthreads();
-    for (int j = tid; j < total; j += WGA) {
-        int a = 0, z = nr - 1;  // last run whose prefix <= j
-        while (a < z) {
-            const int mid = (a + z + 1) >> 1;
-            if (s_pre[mid] <= j) a = mid;
-            else z = mid - 1;
-        }
-        const uint2 e = out[s_at[a] + (j - s_pre[a])];
+    for (int64_t j = e0 + tid; j < e1; j += WGA) {
+        const uint2 e = sorted[j];
         uint32_t *slot = &s_d[e.x - uint32_t(v0)];
         if (e.y < *slot) atomicMin(slot, e.y);
     }

@@ -140,6 +140,10 @@ struct has_direction : std::false_type {};
 template <class P>
 struct has_direction<P, std::void_t<decltype(&P::direction)>> : std::true_type {};
 template <class P, class = void>
+struct has_mid : std::false_type {};
+template <class P>
+struct has_mid<P, std::void_t<decltype(&P::mid)>> : std::true_type {};
+template <class P, class = void>
 struct has_defer : std::false_type {};
 template <class P>
 struct has_defer<P, std::void_t<decltype(&P::defer_key)>> : std::true_type {};
@@ -445,6 +449,11 @@ __device__ __forceinline__ void slot_max_scan(int (&x)[IPT], int32_t *s_wmax) {
 //   Pre load(int c, int p, uint32_t payload, int v)   issue loads for edge
 //                                                  (v -> c) at col_ind[p]
 //   void apply(int c, int p, uint32_t payload, int v, const Pre &)
+//   optionally
+//   void mid(int c, int p, uint32_t payload, int v, Pre &)
+//                                                  loads that depend on the
+//                                                  first ones, issued for all
+//                                                  slots before any apply
 //   static constexpr bool kVert                    needs the entry's vertex id
 //   static constexpr bool kVal                     needs the entry's payload
 // Columns outside [0, m) are skipped.
@@ -534,6 +543,13 @@ __device__ __forceinline__ void expand_tiles(const Op &op, int m, const Round &c
                 if ((unsigned)col[r0 + r] < (unsigned)m)
                     pre[r] = op.load(col[r0 + r], pos[r0 + r], Op::kVal ? s_val[own[r0 + r]] : 0u,
                                      Op::kVert ? s_vert[own[r0 + r]] : 0);
+            if constexpr (has_mid<Op>::value) {
+#pragma unroll
+                for (int r = 0; r < B; ++r)
+                    if ((unsigned)col[r0 + r] < (unsigned)m)
+                        op.mid(col[r0 + r], pos[r0 + r], Op::kVal ? s_val[own[r0 + r]] : 0u,
+                               Op::kVert ? s_vert[own[r0 + r]] : 0, pre[r]);
+            }
 #pragma unroll
             for (int r = 0; r < B; ++r)
                 if ((unsigned)col[r0 + r] < (unsigned)m)

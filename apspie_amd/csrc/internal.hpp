@@ -55,13 +55,15 @@ enum Slot : int {
     S_FBITS,       // uint32 bitmap: the frontier a compaction produced (pull steps)
     S_TROWS,       // int32 per CSC edge tile: row holding the tile's first edge
     S_FVAL,        // uint32 per vertex: SSSP frontier snapshot for pull steps
-    S_BINOUT,      // uint2 per frontier edge: binned SSSP candidates
-    S_BINTBL,      // int32 per (bin workgroup, bucket): offsets of the binned runs
-    S_BINTOT,      // uint32 per (parity, bucket): binned entries per bucket
+    S_BINOUT,      // uint2 per frontier edge: binned SSSP candidates, by bin workgroup
+    S_BINTBL,      // int32 per (bucket, bin workgroup): candidate counts, then offsets inside the bucket
+    S_BINTOT,      // int32 per bucket: binned candidates per bucket
     S_QUEUE1,      // BFS with a CSC: the queue arrays of odd rounds
     S_QSTART1,
     S_QOFF1,
     S_TILES1,
+    S_HINT,        // uint8 per vertex: SSSP distance bound codes
+    S_BINSORT,     // uint2 per candidate: binned SSSP candidates ordered by bucket
     S_AUX0,
     S_AUX1,
     S_AUX2,
@@ -107,6 +109,7 @@ struct apspie_ctx {
         int spgemm_passes = 1;
         int bfs_group_cap = 0;
         int spgemm_esc_min = 1024;
+        int sssp_hint = 1;
     } tune;
 
     // Instantiated hipGraphs of captured round chunks, keyed by everything

@@ -152,7 +152,8 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
         t->sssp_delta < 0 || t->sssp_delta > (1 << 24) || (t->spgemm_long_rows != 0 && t->spgemm_long_rows != 1) ||
         (t->spgemm_passes != 1 && t->spgemm_passes != 2) || t->bfs_group_cap < 0 || t->bfs_group_cap > (1 << 30) ||
         (t->spgemm_esc_min != 256 && t->spgemm_esc_min != 512 && t->spgemm_esc_min != 1024 &&
-         t->spgemm_esc_min != 2048)) {
+         t->spgemm_esc_min != 2048) ||
+        (t->sssp_hint != 0 && t->sssp_hint != 1)) {
         ctx->err = "tuning knob out of range";
         return APSPIE_EINVAL;
     }
@@ -170,6 +171,7 @@ int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t) {
     ctx->tune.spgemm_passes = t->spgemm_passes;
     ctx->tune.bfs_group_cap = t->bfs_group_cap;
     ctx->tune.spgemm_esc_min = t->spgemm_esc_min;
+    ctx->tune.sssp_hint = t->sssp_hint;
     return APSPIE_OK;
 }
 
@@ -189,6 +191,7 @@ int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t) {
     t->spgemm_passes = ctx->tune.spgemm_passes;
     t->bfs_group_cap = ctx->tune.bfs_group_cap;
     t->spgemm_esc_min = ctx->tune.spgemm_esc_min;
+    t->sssp_hint = ctx->tune.sssp_hint;
     return APSPIE_OK;
 }
 

@@ -85,6 +85,7 @@ constexpr uint32_t NOT_IN_FRONTIER = 0xffffffffu;
 
 struct PickSssp {
     uint32_t *dist, *prev;
+    uint8_t *hint;  // distance bound codes, rewritten when a distance dropped (null: none)
     int m, seed;
     // null: no CSC, push only.  Else fval[v] = the distance snapshot of v if
     // v entered the frontier, NOT_IN_FRONTIER otherwise (what a pull step reads)
@@ -106,6 +107,7 @@ struct PickSssp {
         payload = d;
         const bool isnew = d < x.p;
         if (isnew) prev[v] = d;
+        if (hint && isnew) hint[v] = uint8_t(hint_code(d));
         if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
         return isnew;
     }
@@ -133,8 +135,9 @@ struct PickSsspNF : PickSssp {
             dist[v] = 0u;
         }
         payload = d;
-        const bool isnew = d < x.p && d < threshold();
+        const bool changed = d < x.p, isnew = changed && d < threshold();
         if (isnew) prev[v] = d;
+        if (hint && changed) hint[v] = uint8_t(hint_code(d));  // a waiting vertex's bound falls too
         if (fval) fval[v] = isnew ? d : NOT_IN_FRONTIER;
         return isnew;
     }
@@ -185,6 +188,30 @@ struct OpRelax {
     const float *val;
     uint32_t *dist;
     __device__ Pre load(int c, int p, uint32_t, int) const { return Pre{val ? val[p] : 1.0f, dist[c]}; }
+    __device__ void apply(int c, int, uint32_t du, int, const Pre &pr) const {
+        const uint32_t nb = __float_as_uint(__uint_as_float(du) + pr.w);
+        if (nb < pr.d) __hip_atomic_fetch_min(&dist[c], nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// OpRelax behind the distance bound codes: the loads fetch the weight and the
+// target's code, the middle phase probes the distance of the targets whose
+// bound the candidate is below (the others get 0, which no candidate is
+// below), and the apply issues the atomic as OpRelax does.
+struct OpRelaxHint {
+    static constexpr bool kVert = false, kVal = true;
+    struct Pre {
+        float w;
+        uint32_t h, d;
+    };
+    const float *val;
+    uint32_t *dist;
+    const uint8_t *hint;
+    __device__ Pre load(int c, int p, uint32_t, int) const { return Pre{val ? val[p] : 1.0f, hint[c], 0u}; }
+    __device__ void mid(int c, int, uint32_t du, int, Pre &pr) const {
+        const uint32_t nb = __float_as_uint(__uint_as_float(du) + pr.w);
+        pr.d = nb < hint_bound(pr.h) ? dist[c] : 0u;
+    }
     __device__ void apply(int c, int, uint32_t du, int, const Pre &pr) const {
         const uint32_t nb = __float_as_uint(__uint_as_float(du) + pr.w);
         if (nb < pr.d) __hip_atomic_fetch_min(&dist[c], nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -626,8 +653,8 @@ __global__ __launch_bounds__(WG) void k_pull_relax(int level_arg, int m, int nnz
 // one; fval: SSSP pull snapshots)
 __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t *depth, uint32_t *dist,
                                              uint32_t *prev, uint8_t *flags, uint32_t *visited,
-                                             uint64_t *status, Control *C, uint32_t *bin_totals, uint32_t thr0,
-                                             uint32_t *fbits, uint32_t *fval) {
+                                             uint64_t *status, Control *C, uint32_t thr0,
+                                             uint32_t *fbits, uint32_t *fval, uint8_t *hint) {
     const int64_t gtid = int64_t(blockIdx.x) * WG + threadIdx.x;
     const int64_t stride = int64_t(gridDim.x) * WG;
     const int64_t mpad = int64_t(nblk) * CHUNK;
@@ -647,10 +674,11 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
             if (prev) prev[i] = INF_BITS;
             if (fval) fval[i] = NOT_IN_FRONTIER;
         }
+        // no bound yet (the slot holds nblk * CHUNK bytes, 16-byte aligned)
+        if (hint)
+            for (int64_t i = gtid; i < mpad / 16; i += stride) *(uint4 *)(hint + 16 * i) = make_uint4(~0u, ~0u, ~0u, ~0u);
     }
     for (int64_t i = gtid; i < 2 * int64_t(nblk); i += stride) status[i] = 0;
-    if (bin_totals)
-        for (int64_t i = gtid; i < 2 * MAX_BUCKETS; i += stride) bin_totals[i] = 0;
     control_clear(C, gtid, stride, thr0);
 }
 
@@ -828,21 +856,24 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             b.trows = (int32_t *)ctx->slot(S_TROWS, (size_t(g->nnz) / TILE + 2) * 4);
         }
     }
-    // binned relaxation of dense SSSP rounds: entry buffer, per-unit offset
-    // table and per-bucket totals
+    // binned relaxation of dense SSSP rounds: the bin workgroups' candidate
+    // regions (a round has at most max_tiles + NGROUP tiles, a workgroup
+    // ceil(tiles / G) of them), the bucket-ordered candidates (at most one per
+    // frontier edge), the per-(bucket, workgroup) table with the workgroups'
+    // counts after it, and the bucket totals
     const int nb = (int)ceil_div(g->m, BUCKET);
     const bool binned =
         !fixed && MODE == MODE_SSSP && ctx->tune.sssp_bin_ratio > 0 && g->nnz > 0 && nb <= MAX_BUCKETS;
-    uint2 *bin_out = nullptr;
-    int32_t *bin_tbl = nullptr;
-    uint32_t *bin_tot = nullptr;
+    uint2 *bin_out = nullptr, *bin_sorted = nullptr;
+    int32_t *bin_tbl = nullptr, *bin_wcnt = nullptr, *bin_tot = nullptr;
     const int max_tiles = (int)ceil_div(g->nnz, TILE);
-    // (the apply gives each bin workgroup's run one thread: bin_grid <= WGA)
-    const int bin_grid = std::max(1, std::min({max_tiles, ctx->cu_count * 3, WGA}));
+    const int bin_grid = std::max(1, std::min(max_tiles, ctx->cu_count * ctx->tune.expand_wg_per_cu));
     if (binned) {
-        bin_out = (uint2 *)ctx->slot(S_BINOUT, size_t(max_tiles) * TILE * sizeof(uint2));
-        bin_tbl = (int32_t *)ctx->slot(S_BINTBL, size_t(bin_grid) * (nb + 1) * sizeof(int32_t));
-        bin_tot = (uint32_t *)ctx->slot(S_BINTOT, 2 * MAX_BUCKETS * sizeof(uint32_t));
+        bin_out = (uint2 *)ctx->slot(S_BINOUT, size_t(max_tiles + NGROUP + bin_grid) * TILE * sizeof(uint2));
+        bin_sorted = (uint2 *)ctx->slot(S_BINSORT, size_t(g->nnz) * sizeof(uint2));
+        bin_tbl = (int32_t *)ctx->slot(S_BINTBL, (size_t(nb) + 1) * bin_grid * sizeof(int32_t));
+        bin_wcnt = bin_tbl + size_t(nb) * bin_grid;
+        bin_tot = (int32_t *)ctx->slot(S_BINTOT, MAX_BUCKETS * sizeof(int32_t));
     }
     uint32_t *dist = (uint32_t *)dist_f;
     const int nnz32 = int(g->nnz);
@@ -855,9 +886,13 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     const float delta = MODE == MODE_SSSP ? float(ctx->tune.sssp_delta) : 0.0f;
     uint32_t thr0 = ~0u;
     if (delta > 0.0f) memcpy(&thr0, &delta, 4);
+    // distance bound codes (push rounds of a traversal with its own buffers)
+    uint8_t *hint = nullptr;
+    if (MODE == MODE_SSSP && !fixed && ctx->tune.sssp_hint)
+        hint = (uint8_t *)ctx->slot(S_HINT, size_t(b.nblk) * CHUNK);
     k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE, g->m, b.nblk, depth, dist, b.prev, b.flags,
-                                                       b.visited, b.status, b.C, bin_tot, thr0,
-                                                       MODE == MODE_BFS ? b.fbits : nullptr, fval);
+                                                       b.visited, b.status, b.C, thr0,
+                                                       MODE == MODE_BFS ? b.fbits : nullptr, fval, hint);
     if (pull && MODE == MODE_SSSP)
         k_tile_rows<<<vert_grid, WG, 0, ctx->stream>>>(g->m, g->csc_col_ptr, b.trows);
     ctx->prof_end();
@@ -866,7 +901,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
             return PickBfs{b.flags, b.visited, depth, level, seed,
                            PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}};
         else
-            return PickSssp{dist, b.prev, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
+            return PickSssp{dist, b.prev, hint, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
                             binned ? ctx->tune.sssp_bin_ratio : 0};
     };
     auto near_far = [&](const PickSssp &pk) {
@@ -914,16 +949,25 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     auto expand = [&](int level) {
         if (MODE == MODE_BFS)
             launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
+        else if (hint)
+            launch_expand(ctx, b, g, OpRelaxHint{g->val, dist, hint}, level);
         else
             launch_expand(ctx, b, g, OpRelax{g->val, dist}, level);
         if (binned) {
             ctx->prof_begin(APSPIE_KCLASS_BIN);
-            k_bin_relax<<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b), g->col_ind, g->val,
-                                                          bin_out, bin_tbl, bin_tot);
+            if (hint)
+                k_bin_expand<true><<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b), g->col_ind,
+                                                                     g->val, hint, bin_out, bin_tbl, bin_wcnt);
+            else
+                k_bin_expand<false><<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b),
+                                                                      g->col_ind, g->val, nullptr, bin_out, bin_tbl,
+                                                                      bin_wcnt);
             ctx->prof_end();
             ctx->prof_begin(APSPIE_KCLASS_APPLY);
-            k_apply_relax<<<apply_grid, WGA, 0, ctx->stream>>>(level, g->m, nb, bin_grid, b.C, bin_out, bin_tbl,
-                                                               bin_tot, dist);
+            k_bin_scan<<<nb, 256, 0, ctx->stream>>>(level, nb, bin_grid, b.C, bin_tbl, bin_tot);
+            k_bin_scatter<<<bin_grid, WG, 0, ctx->stream>>>(level, nb, b.C, bin_out, bin_tbl, bin_tot, bin_wcnt,
+                                                            bin_sorted);
+            k_bin_apply<<<apply_grid, WGA, 0, ctx->stream>>>(level, g->m, nb, b.C, bin_sorted, bin_tot, dist);
             ctx->prof_end();
         }
         if (!pull) return;
@@ -1069,8 +1113,8 @@ int apspie_sssp_step(apspie_ctx *ctx, const apspie_csr *g, const float *d_x, flo
         APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
         ctx->prof_begin(APSPIE_KCLASS_OTHER);
         k_init<<<init_grid(b, g->m), WG, 0, ctx->stream>>>(MODE_STEP, g->m, b.nblk, nullptr, y, nullptr,
-                                                           b.flags, b.visited, b.status, b.C, nullptr, ~0u,
-                                                           nullptr, nullptr);
+                                                           b.flags, b.visited, b.status, b.C, ~0u,
+                                                           nullptr, nullptr, nullptr);
         ctx->prof_end();
         launch_compact(ctx, b, g, PickStep{d_x, g->m}, 0, 1);
         launch_expand(ctx, b, g, OpRelax{g->val, y}, 1);

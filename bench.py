@@ -74,10 +74,12 @@ def sssp_class_bytes(rounds, m, nnz):
     pull step: 4 B per CSC edge (source row) + 4 B per vertex (column
       pointer) + the m/8-byte frontier bitmap, 8 B per in-edge from the
       frontier (weight, source distance), 4 B per improved vertex;
-    binned round, bin kernel: 8 B per relaxed edge (column, weight) + 12 B
-      per frontier vertex + 8 B per candidate written;
-    binned round, apply kernel: 8 B per candidate read + 4 B per vertex
-      (distance slices) + 4 B per improved vertex;
+    binned round, k_bin_expand: the push expansion's 12 B per relaxed edge
+      (column, weight, the target's distance, which it reads as its 1-byte
+      bound code) + 12 B per frontier vertex (the candidates it writes are
+      the binning's own traffic, not the algorithm's);
+    binned round, k_bin_scan + k_bin_scatter + k_bin_apply: 4 B per improved
+      vertex (distance write);
     compaction: 8 B per vertex (distance and round-start copy) + 28 B per
       queued vertex (copy update, row-pointer pair, 16-byte queue entry)."""
     out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0, "compact": 0.0}
@@ -86,8 +88,8 @@ def sssp_class_bytes(rounds, m, nnz):
         if mode == 1:
             out["pull"] += 4.0 * nnz + 4.0 * (m + 1) + m / 8.0 + 8.0 * e + 4.0 * nxt
         elif mode == 2:
-            out["bin"] += 16.0 * e + 12.0 * nf
-            out["apply"] += 8.0 * e + 4.0 * m + 4.0 * nxt
+            out["bin"] += 12.0 * e + 12.0 * nf
+            out["apply"] += 4.0 * nxt
         else:
             out["expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
         out["compact"] += 8.0 * m + 28.0 * nxt
@@ -124,13 +126,22 @@ def bfs_class_bytes(rounds, m):
 KERNEL_NAMES = {
     ("sssp", "compact"): "k_compact<PickSssp>",
     ("bfs", "compact"): "k_compact<PickBfsBits>",
-    ("sssp", "expand"): "k_expand<OpRelax>",
+    ("sssp", "expand"): "k_expand<OpRelaxHint>",
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",
     ("bfs", "round"): "k_round_bfs",
-    ("sssp", "bin"): "k_bin_relax",
-    ("sssp", "apply"): "k_apply_relax",
+    ("sssp", "bin"): "k_bin_expand<true>",
+    ("sssp", "apply"): "k_bin_scatter+k_bin_apply",
 }
+
+
+def kernel_name(ctx, algo, cls):
+    """The kernel a class launches under the context's knobs (SSSP without
+    the distance bound codes runs the plain relaxation)."""
+    n = KERNEL_NAMES[(algo, cls)]
+    if algo == "sssp" and not ctx.tuning()["sssp_hint"]:
+        n = {"k_expand<OpRelaxHint>": "k_expand<OpRelax>", "k_bin_expand<true>": "k_bin_expand<false>"}.get(n, n)
+    return n
 
 
 def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
@@ -153,7 +164,7 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
     for c, b in bytes_by_class.items():  # every class that ran, the same way
         if prof[c][1] and prof[c][0] > 0 and (algo, c) in KERNEL_NAMES:
             a = b * steps / (prof[c][0] * 1e-3) / 1e9
-            classes[KERNEL_NAMES[(algo, c)]] = {"achieved": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4),
+            classes[kernel_name(ctx, algo, c)] = {"achieved": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4),
                                                 "avg_launch_us": round(prof[c][0] / prof[c][1] * 1e3, 3),
                                                 "launches_per_step": prof[c][1] / steps,
                                                 "bytes_per_launch": int(b * steps / prof[c][1])}
@@ -164,7 +175,7 @@ def kernel_roofline(ctx, fn, steps, bytes_by_class, algo):
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
-        "kernel": KERNEL_NAMES[(algo, cls)],
+        "kernel": kernel_name(ctx, algo, cls),
         "avg_launch_us": round(avg_ms * 1e3, 3),
         "launches_per_step": launches / steps,
         "bytes_per_launch": int(bytes_per_launch),

@@ -161,6 +161,9 @@ typedef struct {
                                  (16 regions); a round whose new frontier does not fit, and whose
                                  successor runs top-down, is queued by a pass over the vertex range
                                  instead.  0 = the whole region (default), else 1..2^30 */
+    int32_t sssp_hint;        /* SSSP push rounds: 1 = keep a byte per vertex whose code bounds the vertex's
+                                 distance from above and probe a target's 4-byte distance only when an
+                                 edge's candidate is below that bound (default), 0 = probe every target */
 } apspie_tuning;
 int apspie_ctx_set_tuning(apspie_ctx *ctx, const apspie_tuning *t);
 int apspie_ctx_get_tuning(const apspie_ctx *ctx, apspie_tuning *t);

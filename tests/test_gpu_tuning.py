@@ -40,7 +40,7 @@ def sources(h, k, seed):
     dict(batch_chunk=1), dict(batch_chunk=16), dict(sssp_batch_lanes=64), dict(sssp_batch_lanes=16),
     dict(bfs_alpha=1, bfs_beta=1000), dict(sssp_pull_ratio=2, sssp_bin_ratio=0), dict(sssp_bin_ratio=2),
     dict(sssp_delta=1), dict(sssp_delta=40), dict(sssp_delta=40, sssp_pull_ratio=2),
-    dict(sssp_delta=40, sssp_bin_ratio=2),
+    dict(sssp_delta=40, sssp_bin_ratio=2), dict(sssp_hint=0), dict(sssp_hint=1, sssp_delta=40),
 ])
 def test_every_knob_setting_is_exact(tuned, graph, knobs):
     ctx = tuned
@@ -71,7 +71,8 @@ def test_out_of_range_knobs_are_refused(tuned):
     for bad in (dict(expand_wg_per_cu=0), dict(round_chunk=33), dict(batch_chunk=0), dict(sssp_batch_lanes=32),
                 dict(sssp_delta=-1), dict(sssp_delta=(1 << 24) + 1), dict(spgemm_long_rows=2),
                 dict(spgemm_long_rows=-1), dict(spgemm_passes=0), dict(spgemm_passes=3), dict(bfs_group_cap=-1),
-                dict(bfs_group_cap=(1 << 30) + 1), dict(spgemm_esc_min=300), dict(spgemm_esc_min=4096)):
+                dict(bfs_group_cap=(1 << 30) + 1), dict(spgemm_esc_min=300), dict(spgemm_esc_min=4096),
+                dict(sssp_hint=-1), dict(sssp_hint=2)):
         with pytest.raises(A.ApspieError):
             ctx.set_tuning(**bad)
         assert ctx.tuning() == before
