@@ -63,7 +63,7 @@ constexpr int BUCKET = 1 << BUCKET_SHIFT;  // vertices per bucket (64 KB of dist
 constexpr int MAX_BUCKETS = 512;           // binned rounds need m <= MAX_BUCKETS * BUCKET
 constexpr int WGA = 1024;                  // threads per apply workgroup
 constexpr int PART_ENTRIES = 1 << 18;      // candidates per apply workgroup (heavy buckets split)
-constexpr int SCH = 2048;                  // entries per scatter chunk (8 per thread)
+constexpr int SCH = 4096;                  // entries per scatter chunk (16 per thread)
 
 // Entries of each bin workgroup's region for a round of ntiles tiles over G
 // workgroups: a workgroup expands at most ceil(ntiles / G) tiles.  Region w
@@ -71,15 +71,15 @@ constexpr int SCH = 2048;                  // entries per scatter chunk (8 per t
 __device__ __forceinline__ int64_t bin_cap(int ntiles, int G) { return int64_t((ntiles + G - 1) / G) * TILE; }
 
 // Exclusive scan of x[0..n) (n <= MAX_BUCKETS) by one full wave into dst;
-// returns the total.  dst may be written with `add[b]` added (null: none).
+// returns the total.
 __device__ __forceinline__ uint32_t bucket_scan(const int32_t *x, int n, uint32_t *dst, int lane) {
     uint32_t carry = 0;
     for (int b0 = 0; b0 < n; b0 += WAVE) {
         const int b = b0 + lane;
         const uint32_t h = b < n ? uint32_t(x[b]) : 0u;
-        const uint32_t incl = wave_incl_scan32(h, lane);
+        const uint32_t incl = wave_incl_sum_dpp(h);
         if (b < n) dst[b] = carry + incl - h;
-        carry += __shfl(incl, WAVE - 1, WAVE);
+        carry += uint32_t(wave_last(int(incl)));
     }
     return carry;
 }
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_bin_scan(int level_arg, int nb, int G, 
     for (int i0 = 0; i0 < G; i0 += 256) {
         const int i = i0 + tid;
         const uint32_t x = i < G ? uint32_t(row[i]) : 0u;
-        const uint32_t incl = wave_incl_scan32(x, lane);
+        const uint32_t incl = wave_incl_sum_dpp(x);
         if (lane == WAVE - 1) s_w[wid] = incl;
         __syncthreads();
         uint32_t wpre = 0, all = 0;
@@ -288,10 +288,20 @@ __global__ __launch_bounds__(WGA) void k_bin_apply(int level_arg, int m, int nb,
     const int vn = min(BUCKET, m - v0);
     for (int i = tid; i < vn; i += WGA) s_d[i] = dist[v0 + i];
     __syncthreads();
-    for (int64_t j = e0 + tid; j < e1; j += WGA) {
-        const uint2 e = sorted[j];
-        uint32_t *slot = &s_d[e.x - uint32_t(v0)];
-        if (e.y < *slot) atomicMin(slot, e.y);
+    // eight loads in flight per thread before their LDS updates
+    constexpr int U = 8;
+    for (int64_t j0 = e0; j0 < e1; j0 += int64_t(U) * WGA) {
+        uint2 e[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int64_t j = j0 + k * WGA + tid;
+            e[k] = j < e1 ? sorted[j] : make_uint2(uint32_t(v0), ~0u);  // ~0u lowers nothing
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            uint32_t *slot = &s_d[e[k].x - uint32_t(v0)];
+            if (e[k].y < *slot) atomicMin(slot, e[k].y);
+        }
     }
     __syncthreads();
     for (int i = tid; i < vn; i += WGA) {
