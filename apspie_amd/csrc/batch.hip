@@ -170,7 +170,7 @@ __global__ __launch_bounds__(WG) void k_expand_ms_sssp(int m, const Control *C, 
                                                        const int32_t *__restrict__ col_ind,
                                                        const float *__restrict__ val, uint32_t *dist,
                                                        const uint64_t *__restrict__ fr, uint64_t *chg) {
-    constexpr int NW = WG / WAVE, SEG = TILE / NW, E = WAVE / K, U = K == 64 ? 8 : 4;
+    constexpr int NW = WG / WAVE, SEG = TILE / NW, E = WAVE / K, U = K == 64 ? 16 : 8;
     static_assert(WAVE % K == 0, "K divides the wave");
     __shared__ int32_t s_own[TILE];
     __shared__ int32_t s_base[TILE + 1];

@@ -138,3 +138,17 @@ def test_binned_rounds_rmat_s18(tuned, hint, ratio):
     dist0, st0 = tuned.sssp(g, src)
     assert [r[:2] for r in tuned.rounds()] == [r[:2] for r in rounds]
     assert (st0.edges, st0.vertices, st0.iterations) == (st.edges, st.vertices, st.iterations)
+
+
+@pytest.mark.parametrize("kind", ["integer", "stepped", "wide", "fractional", "zero"])
+@pytest.mark.parametrize("k", [16, 64])
+def test_batched_rows_exact_for_weight_kinds(tuned, kind, k):
+    """Batched SSSP (16- and 64-lane forms) on the same weight kinds: every
+    row equals the oracle's single-source distances."""
+    h = weighted(kind)
+    g = tuned.upload(h)
+    cand = np.nonzero(h.degrees() > 0)[0]
+    srcs = np.random.RandomState(k).choice(cand, k, replace=False).astype(np.int32)
+    want = np.stack([oracle_dist(h, int(s)) for s in srcs])
+    dist, _ = tuned.sssp_batch(g, srcs)
+    assert np.array_equal(dist, want), (kind, k)
