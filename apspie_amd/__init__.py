@@ -181,6 +181,7 @@ def lib() -> C.CDLL:
         "apspie_server_drain": [vp],
         "apspie_server_stats": [vp, vp],
         "apspie_server_reset_stats": [vp],
+        "apspie_server_set_tuning": [vp, P(Tuning)],
         "apspie_server_destroy": [vp],
     }
     for name, args in sig.items():
@@ -828,6 +829,17 @@ class Server:
 
     def reset_stats(self) -> None:
         lib().apspie_server_reset_stats(self.h)
+
+    def set_tuning(self, tuning: dict) -> None:
+        """Knobs of every lane's context: every knob, as Context.tuning()
+        returns them (apspie_server_set_tuning)."""
+        names = {k for k, _ in Tuning._fields_}
+        if set(tuning) != names:
+            raise KeyError(f"tuning must name exactly the knobs {sorted(names)}")
+        t = Tuning(**{k: int(v) for k, v in tuning.items()})
+        rc = lib().apspie_server_set_tuning(self.h, C.byref(t))
+        if rc != 0:
+            raise ApspieError(rc, "server_set_tuning: " + lib().apspie_server_last_error(self.h).decode())
 
     def close(self) -> None:
         if self.h:

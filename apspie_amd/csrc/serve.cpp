@@ -422,6 +422,19 @@ int apspie_server_reset_stats(apspie_server *s) {
     return APSPIE_OK;
 }
 
+int apspie_server_set_tuning(apspie_server *s, const apspie_tuning *t) {
+    if (!s || !t) return APSPIE_EINVAL;
+    for (apspie_ctx *c : s->ctxs) {  // each waits for its lane's current call
+        const int rc = apspie_ctx_set_tuning(c, t);
+        if (rc != APSPIE_OK) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->err = apspie_last_error(c);
+            return rc;
+        }
+    }
+    return APSPIE_OK;
+}
+
 const char *apspie_server_last_error(apspie_server *s) { return s ? s->err.c_str() : "null server"; }
 
 int apspie_server_destroy(apspie_server *s) {

@@ -430,7 +430,13 @@ def bench_served(ctx, n):
     log(f"serving pool of {len(graphs)} graphs ready ({time.time() - t:.1f}s)")
     reqs = serving.make_stream(ms, n=n, rate=2000.0)
     res = {"pool_scales": serving.POOL_SCALES, "requests": n, "rate_per_s": 2000.0}
-    with A.Server(graphs, lanes=4) as srv:
+    # six lanes: measured 30.2-30.8 K q/s saturated, against 28.1-28.3 K with
+    # three, 28.3-28.6 K with four and 29.8-30.1 K with eight
+    # served lanes and plain launches run the same knobs (the context's)
+    knobs = ctx.tuning()
+    res["tuning"] = knobs
+    with A.Server(graphs, lanes=6) as srv:
+        srv.set_tuning(knobs)
         serving.run_served(srv, reqs[:500], open_loop=False)  # warm-up: contexts, buffers
         res["served_open_loop"] = serving.run_served(srv, reqs, open_loop=True)
         log(f"served open-loop: {res['served_open_loop']}")

@@ -471,6 +471,10 @@ int apspie_server_wait(apspie_server *s, uint64_t ticket, double *latency_ms);
 int apspie_server_drain(apspie_server *s);
 int apspie_server_stats(apspie_server *s, apspie_server_report *st);
 int apspie_server_reset_stats(apspie_server *s);
+/* Sets the traversal knobs of every lane's context (apspie_ctx_set_tuning;
+ * a lane running a batch takes them when the batch ends).  Results never
+ * depend on them. */
+int apspie_server_set_tuning(apspie_server *s, const apspie_tuning *t);
 const char *apspie_server_last_error(apspie_server *s);
 int apspie_server_destroy(apspie_server *s);
 

@@ -87,3 +87,26 @@ def test_admission_by_footprint(ctx, pool):
             srv.submit(A.REQ_BFS, 0, 0)
         assert e.value.code == -7
         assert srv.report()["rejected"] == 1
+
+
+def test_lane_tuning_keeps_results(ctx, pool):
+    """apspie_server_set_tuning: the lanes take any valid knob setting with
+    results unchanged, refuse an out-of-range one, and the Python side
+    insists on every knob."""
+    hosts, graphs = pool
+    reqs = serving.make_stream([h.m for h in hosts], n=200, rate=50000.0, seed=11)
+    outs = {i: np.empty(hosts[r.graph].m, np.int32 if r.kind == A.REQ_BFS else np.float32)
+            for i, r in enumerate(reqs)}
+    knobs = ctx.tuning()
+    knobs.update(sssp_bin_ratio=2, sssp_hint=0, expand_wg_per_cu=3, round_chunk=1, batch_chunk=2, bfs_alpha=1)
+    with A.Server(graphs, lanes=2, plain_bfs_upto=-1, plain_sssp_upto=2) as srv:
+        srv.set_tuning(knobs)
+        rep = serving.run_served(srv, reqs, open_loop=False, outs=outs)
+        bad = dict(knobs, round_chunk=0)
+        with pytest.raises(A.ApspieError):
+            srv.set_tuning(bad)
+        with pytest.raises(KeyError):
+            srv.set_tuning({"round_chunk": 4})
+    assert rep["completed"] == len(reqs)
+    for i, r in enumerate(reqs):
+        assert np.array_equal(outs[i], solo(ctx, graphs, r)), i
