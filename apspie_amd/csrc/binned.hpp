@@ -3,7 +3,8 @@
 // Propagation Blocking", IPDPS 2017), which replaces the random 4-byte
 // distance probes and atomics a push round makes per edge with streams.
 //
-//   k_bin_expand   G workgroups over the round's tiles (t = w, w + G, ...).
+//   bin_tiles      (in k_relax, traverse.hip, the round's one relaxation
+//                  launch) G workgroups over the round's tiles (t = w, w + G, ...).
 //                  Each edge's candidate d_u + w is kept when it is below its
 //                  target's bound code (all are kept without codes); the kept
 //                  (target, candidate) pairs are appended, wave-compacted, to
@@ -61,25 +62,20 @@ __device__ __forceinline__ uint32_t bucket_scan(const int32_t *x, int n, uint32_
     return carry;
 }
 
-// Candidate regions: G * bin_cap(nTiles, G) uint2 (target, candidate bits);
-// table: nb * G int32 per-bucket counts (bucket-major); wcount: G int32.
+// The binning expansion of a binned round by workgroup w = blockIdx.x of G =
+// gridDim.x.  Candidate regions: G * bin_cap(nTiles, G) uint2 (target,
+// candidate bits); table: nb * G int32 per-bucket counts (bucket-major);
+// wcount: G int32.  LDS: the tile staging (s_own[TILE], s_base[TILE + 1],
+// s_val[TILE + 1], s_vert[1], s_wmax[WG / WAVE]), s_hist[MAX_BUCKETS] and
+// s_wcnt[WG / WAVE].
 template <bool HINT>
-__global__ __launch_bounds__(WG) void k_bin_expand(int level_arg, int m, int nb, const Control *C, Queue q,
-                                                   const int32_t *__restrict__ col_ind,
-                                                   const float *__restrict__ val, const uint8_t *__restrict__ hint,
-                                                   uint2 *__restrict__ out, int32_t *__restrict__ tbl,
-                                                   int32_t *__restrict__ wcount) {
+__device__ __forceinline__ void bin_tiles(const Round &cur, int m, int nb, const Queue &q,
+                                          const int32_t *__restrict__ col_ind, const float *__restrict__ val,
+                                          const uint8_t *__restrict__ hint, uint2 *__restrict__ out,
+                                          int32_t *__restrict__ tbl, int32_t *__restrict__ wcount, int32_t *s_own,
+                                          int32_t *s_base, uint32_t *s_val, int32_t *s_vert, int32_t *s_wmax,
+                                          uint32_t *s_hist, uint32_t *s_wcnt) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
-    __shared__ int32_t s_own[TILE];
-    __shared__ int32_t s_base[TILE + 1];
-    __shared__ uint32_t s_val[TILE + 1];
-    __shared__ int32_t s_vert[1];
-    __shared__ int32_t s_wmax[NW];
-    __shared__ uint32_t s_hist[MAX_BUCKETS];
-    __shared__ uint32_t s_wcnt[NW];
-    const int level = level_arg >= 0 ? level_arg : C->cur_level;
-    const Round cur = round_of(C, level);
-    if (cur.pull != DIR_BINNED) return;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     const int G = gridDim.x, w = blockIdx.x;
     uint2 *reg = out + size_t(w) * size_t(bin_cap(cur.nTiles, G));
@@ -174,7 +170,7 @@ __global__ __launch_bounds__(256) void k_bin_scan(int level_arg, int nb, int G, 
 
 // Region w's candidates, SCH at a time, sorted by bucket in LDS and written
 // to bucket b's run at (first entry of b) + (w's offset inside b) + (what w
-// already wrote to b).  Grid: the G workgroups of k_bin_expand.
+// already wrote to b).  Grid: the G workgroups of the round's k_relax.
 __global__ __launch_bounds__(WG) void k_bin_scatter(int level_arg, int nb, const Control *C,
                                                     const uint2 *__restrict__ out, const int32_t *__restrict__ tbl,
                                                     const int32_t *__restrict__ tot,

@@ -512,15 +512,15 @@ struct op_batch<T, std::void_t<decltype(T::kBatch)>> {
 
 // The tiles t = bid, bid + nblk, ... of the frontier `cur` (block-uniform
 // arguments; every thread of the workgroup calls it).
+// (the tile staging in LDS given by the caller: s_own[TILE], s_base[TILE +
+// 1], s_val[TILE + 1] when Op::kVal, s_vert[TILE + 1] when Op::kVert,
+// s_wmax[WG / WAVE])
 template <class Op>
-__device__ __forceinline__ void expand_tiles(const Op &op, int m, const Round &cur, const Queue &q,
-                                             const int32_t *__restrict__ col_ind, int bid, int nblk) {
+__device__ __forceinline__ void expand_tiles_in(const Op &op, int m, const Round &cur, const Queue &q,
+                                                const int32_t *__restrict__ col_ind, int bid, int nblk,
+                                                int32_t *s_own, int32_t *s_base, uint32_t *s_val, int32_t *s_vert,
+                                                int32_t *s_wmax) {
     constexpr int NW = WG / WAVE, SEG = TILE / NW;
-    __shared__ int32_t s_own[TILE];
-    __shared__ int32_t s_base[TILE + 1];  // start - (off - e0): col_ind index of slot 0
-    __shared__ uint32_t s_val[Op::kVal ? TILE + 1 : 1];
-    __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
-    __shared__ int32_t s_wmax[NW];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
     for (int t = bid; t < cur.nTiles; t += nblk) {
         int own[IPT];
@@ -558,6 +558,18 @@ __device__ __forceinline__ void expand_tiles(const Op &op, int m, const Round &c
         }
         __syncthreads();
     }
+}
+
+template <class Op>
+__device__ __forceinline__ void expand_tiles(const Op &op, int m, const Round &cur, const Queue &q,
+                                             const int32_t *__restrict__ col_ind, int bid, int nblk) {
+    constexpr int NW = WG / WAVE;
+    __shared__ int32_t s_own[TILE];
+    __shared__ int32_t s_base[TILE + 1];  // start - (off - e0): col_ind index of slot 0
+    __shared__ uint32_t s_val[Op::kVal ? TILE + 1 : 1];
+    __shared__ int32_t s_vert[Op::kVert ? TILE + 1 : 1];
+    __shared__ int32_t s_wmax[NW];
+    expand_tiles_in(op, m, cur, q, col_ind, bid, nblk, s_own, s_base, s_val, s_vert, s_wmax);
 }
 
 template <class Op>

@@ -218,6 +218,34 @@ struct OpRelaxHint {
     }
 };
 
+// The relaxation of an SSSP round with binned rounds enabled, in one launch
+// whichever way the round runs: the push tiles with Op, or the binning
+// expansion of a binned round (bin_tiles), both on one LDS staging (a push
+// expansion and a binning one as separate kernels cost a launch that exits
+// at once in every round).  Pull rounds exit.
+template <class Op, bool HINT>
+__global__ __launch_bounds__(WG) void k_relax(Op op, int level_arg, int m, int nb, const Control *C, Queue q,
+                                              const int32_t *__restrict__ col_ind, const float *__restrict__ val,
+                                              const uint8_t *__restrict__ hint, uint2 *__restrict__ out,
+                                              int32_t *__restrict__ tbl, int32_t *__restrict__ wcount) {
+    constexpr int NW = WG / WAVE;
+    static_assert(Op::kVal && !Op::kVert, "the staging holds payloads, not vertex ids");
+    __shared__ int32_t s_own[TILE];
+    __shared__ int32_t s_base[TILE + 1];
+    __shared__ uint32_t s_val[TILE + 1];
+    __shared__ int32_t s_vert[1];
+    __shared__ int32_t s_wmax[NW];
+    __shared__ uint32_t s_hist[MAX_BUCKETS];
+    __shared__ uint32_t s_wcnt[NW];
+    const int level = level_arg >= 0 ? level_arg : C->cur_level;
+    const Round cur = round_of(C, level);
+    if (cur.pull == DIR_PUSH)
+        expand_tiles_in(op, m, cur, q, col_ind, blockIdx.x, gridDim.x, s_own, s_base, s_val, s_vert, s_wmax);
+    else if (cur.pull == DIR_BINNED)
+        bin_tiles<HINT>(cur, m, nb, q, col_ind, val, hint, out, tbl, wcount, s_own, s_base, s_val, s_vert, s_wmax,
+                        s_hist, s_wcnt);
+}
+
 // ----------------------------------------------------------------------------
 // pull steps (rounds whose compaction set Ctl::pull), over the CSC: cp[m+1]
 // column pointers, ri[nnz] source rows, cv[nnz] values
@@ -669,8 +697,18 @@ __global__ __launch_bounds__(WG) void k_init(int mode, int m, int nblk, int32_t 
         }
         for (int64_t i = gtid; i < m; i += stride) depth[i] = -1;
     } else {
-        for (int64_t i = gtid; i < m; i += stride) {
-            dist[i] = INF_BITS;
+        // 16-byte stores where the arrays allow (the caller's distances may
+        // be only 4-byte aligned; prev and fval are pool slots)
+        const int64_t m4 = m / 4;
+        const uint4 inf4 = make_uint4(INF_BITS, INF_BITS, INF_BITS, INF_BITS);
+        const bool dvec = (reinterpret_cast<uintptr_t>(dist) & 15) == 0;
+        for (int64_t i = gtid; i < m4; i += stride) {
+            if (dvec) ((uint4 *)dist)[i] = inf4;
+            if (prev) ((uint4 *)prev)[i] = inf4;
+            if (fval) ((uint4 *)fval)[i] = make_uint4(NOT_IN_FRONTIER, NOT_IN_FRONTIER, NOT_IN_FRONTIER, NOT_IN_FRONTIER);
+        }
+        for (int64_t i = (dvec ? 4 * m4 : 0) + gtid; i < m; i += stride) dist[i] = INF_BITS;
+        for (int64_t i = 4 * m4 + gtid; i < m; i += stride) {
             if (prev) prev[i] = INF_BITS;
             if (fval) fval[i] = NOT_IN_FRONTIER;
         }
@@ -867,7 +905,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     uint2 *bin_out = nullptr, *bin_sorted = nullptr;
     int32_t *bin_tbl = nullptr, *bin_wcnt = nullptr, *bin_tot = nullptr;
     const int max_tiles = (int)ceil_div(g->nnz, TILE);
-    const int bin_grid = std::max(1, std::min(max_tiles, ctx->cu_count * ctx->tune.expand_wg_per_cu));
+    const int bin_grid = int(b.expand_grid);  // k_relax's grid: the push expansion's
     if (binned) {
         bin_out = (uint2 *)ctx->slot(S_BINOUT, size_t(max_tiles + NGROUP + bin_grid) * TILE * sizeof(uint2));
         bin_sorted = (uint2 *)ctx->slot(S_BINSORT, size_t(g->nnz) * sizeof(uint2));
@@ -947,21 +985,18 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         APSPIE_HIP(hipMemsetAsync(&b.C->rebuild, 0, sizeof(int), ctx->stream));
     };
     auto expand = [&](int level) {
-        if (MODE == MODE_BFS)
-            launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
-        else if (hint)
-            launch_expand(ctx, b, g, OpRelaxHint{g->val, dist, hint}, level);
-        else
-            launch_expand(ctx, b, g, OpRelax{g->val, dist}, level);
         if (binned) {
+            // one relaxation launch a round (push or binning), then the
+            // binned round's scan, scatter and apply (which exit otherwise)
             ctx->prof_begin(APSPIE_KCLASS_BIN);
             if (hint)
-                k_bin_expand<true><<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b), g->col_ind,
-                                                                     g->val, hint, bin_out, bin_tbl, bin_wcnt);
+                k_relax<OpRelaxHint, true><<<b.expand_grid, WG, 0, ctx->stream>>>(
+                    OpRelaxHint{g->val, dist, hint}, level, g->m, nb, b.C, queue_of(b), g->col_ind, g->val, hint,
+                    bin_out, bin_tbl, bin_wcnt);
             else
-                k_bin_expand<false><<<bin_grid, WG, 0, ctx->stream>>>(level, g->m, nb, b.C, queue_of(b),
-                                                                      g->col_ind, g->val, nullptr, bin_out, bin_tbl,
-                                                                      bin_wcnt);
+                k_relax<OpRelax, false><<<b.expand_grid, WG, 0, ctx->stream>>>(
+                    OpRelax{g->val, dist}, level, g->m, nb, b.C, queue_of(b), g->col_ind, g->val, nullptr, bin_out,
+                    bin_tbl, bin_wcnt);
             ctx->prof_end();
             ctx->prof_begin(APSPIE_KCLASS_APPLY);
             k_bin_scan<<<nb, 256, 0, ctx->stream>>>(level, nb, bin_grid, b.C, bin_tbl, bin_tot);
@@ -969,6 +1004,12 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                                                             bin_sorted);
             k_bin_apply<<<apply_grid, WGA, 0, ctx->stream>>>(level, g->m, nb, b.C, bin_sorted, bin_tot, dist);
             ctx->prof_end();
+        } else if (MODE == MODE_BFS) {
+            launch_expand(ctx, b, g, OpBfs{b.visited, b.flags}, level);
+        } else if (hint) {
+            launch_expand(ctx, b, g, OpRelaxHint{g->val, dist, hint}, level);
+        } else {
+            launch_expand(ctx, b, g, OpRelax{g->val, dist}, level);
         }
         if (!pull) return;
         ctx->prof_begin(APSPIE_KCLASS_PULL);

@@ -30,7 +30,7 @@ struct Ctl {
 enum {
     DIR_PUSH = 0,    // k_expand over the queued frontier
     DIR_PULL = 1,    // pull step over the CSC
-    DIR_BINNED = 2,  // SSSP: binned relaxation (k_bin_expand, k_bin_scan, k_bin_scatter, k_bin_apply)
+    DIR_BINNED = 2,  // SSSP: binned relaxation (k_relax's bin_tiles, k_bin_scan, k_bin_scatter, k_bin_apply)
     DIR_REPICK = 3,  // SSSP near/far: nothing to expand; changed vertices wait at or past the
                      // threshold, which was raised, and the next compaction picks again
 };

@@ -10,6 +10,7 @@ measured beside it.  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
+import re
 import os
 import platform
 import statistics
@@ -63,7 +64,7 @@ def time_steps(ctx, fn, steps):
     return walls, kms, last
 
 
-def sssp_class_bytes(rounds, m, nnz):
+def sssp_class_bytes(rounds, m, nnz, relax_kernel=False):
     """Algorithmic bytes per traversal of each SSSP kernel class, from the
     round log (frontier i = (vertices, edges, pull) is expanded by round i+1;
     round 0 is the seed kernel's, every later one a compaction's, and one more
@@ -74,14 +75,16 @@ def sssp_class_bytes(rounds, m, nnz):
     pull step: 4 B per CSC edge (source row) + 4 B per vertex (column
       pointer) + the m/8-byte frontier bitmap, 8 B per in-edge from the
       frontier (weight, source distance), 4 B per improved vertex;
-    binned round, k_bin_expand: the push expansion's 12 B per relaxed edge
-      (column, weight, the target's distance, which it reads as its 1-byte
-      bound code) + 12 B per frontier vertex (the candidates it writes are
-      the binning's own traffic, not the algorithm's);
+    binned round, k_relax's binning expansion: the push expansion's 12 B per
+      relaxed edge (column, weight, the target's distance, which it reads as
+      its 1-byte bound code) + 12 B per frontier vertex (the candidates it
+      writes are the binning's own traffic, not the algorithm's);
     binned round, k_bin_scan + k_bin_scatter + k_bin_apply: 4 B per improved
       vertex (distance write);
     compaction: 8 B per vertex (distance and round-start copy) + 28 B per
-      queued vertex (copy update, row-pointer pair, 16-byte queue entry)."""
+      queued vertex (copy update, row-pointer pair, 16-byte queue entry).
+    relax_kernel: binned rounds are on, so one k_relax launch a round runs the
+    push rounds as well as the binning expansions (class "bin")."""
     out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0, "compact": 0.0}
     for i, (nf, e, mode) in enumerate(rounds):
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
@@ -91,9 +94,16 @@ def sssp_class_bytes(rounds, m, nnz):
             out["bin"] += 12.0 * e + 12.0 * nf
             out["apply"] += 4.0 * nxt
         else:
-            out["expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
+            out["bin" if relax_kernel else "expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
         out["compact"] += 8.0 * m + 28.0 * nxt
     return out
+
+
+def relax_kernel(ctx, m):
+    """Whether SSSP on an m-vertex graph runs its rounds through k_relax
+    (binned rounds on: sssp_bin_ratio > 0 and m within the 512 buckets of
+    16384 vertices)."""
+    return ctx.tuning()["sssp_bin_ratio"] > 0 and m <= 512 * 16384
 
 
 def bfs_class_bytes(rounds, m):
@@ -130,7 +140,7 @@ KERNEL_NAMES = {
     ("sssp", "pull"): "k_pull_relax",
     ("bfs", "expand"): "k_expand<OpBfs>",
     ("bfs", "round"): "k_round_bfs",
-    ("sssp", "bin"): "k_bin_expand<true>",
+    ("sssp", "bin"): "k_relax<OpRelaxHint, true>",
     ("sssp", "apply"): "k_bin_scatter+k_bin_apply",
 }
 
@@ -140,7 +150,8 @@ def kernel_name(ctx, algo, cls):
     the distance bound codes runs the plain relaxation)."""
     n = KERNEL_NAMES[(algo, cls)]
     if algo == "sssp" and not ctx.tuning()["sssp_hint"]:
-        n = {"k_expand<OpRelaxHint>": "k_expand<OpRelax>", "k_bin_expand<true>": "k_bin_expand<false>"}.get(n, n)
+        n = {"k_expand<OpRelaxHint>": "k_expand<OpRelax>",
+             "k_relax<OpRelaxHint, true>": "k_relax<OpRelax, false>"}.get(n, n)
     return n
 
 
@@ -190,8 +201,7 @@ def attach_traffic(roof, workload):
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same dispatches
     (profiles/<kernel>_<workload>_traffic.json, written from `bench.py
     --profile-workload` runs), scaled by a copy kernel's counters."""
-    kname = roof["kernel"].split("<")[0] + ("_" + roof["kernel"].split("<")[1].rstrip(">") if "<" in roof["kernel"]
-                                            else "")
+    kname = re.sub(r"[<>, ]+", "_", roof["kernel"]).strip("_")  # k_relax<A, b> -> k_relax_A_b
     path = os.path.join(ROOT, "profiles", f"{kname}_{workload}_traffic.json")
     if not os.path.exists(path):
         return
@@ -485,7 +495,7 @@ def profile_workload(args):
             st = fn()
         rounds = ctx.rounds()
         info.update(m=h.m, nnz=h.nnz, rounds=rounds, device_ms=st.kernel_ms, stats=st.as_dict(),
-                    class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz) if w.startswith("sssp")
+                    class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz, relax_kernel(ctx, h.m)) if w.startswith("sssp")
                                  else bfs_class_bytes(rounds, h.m)))
     elif w in ("bfs_batch_s20_k256", "sssp_batch_s20_k64"):
         import ctypes as Ct
@@ -602,7 +612,8 @@ def main():
     log(f"sssp: {ms_step:.3f} ms/step, {gteps:.2f} GTEPS, iters={st.iterations} relax={st.edges}")
 
     rounds = ctx.rounds()
-    roof = kernel_roofline(ctx, run, max(3, args.steps // 2), sssp_class_bytes(rounds, h.m, h.nnz), "sssp")
+    roof = kernel_roofline(ctx, run, max(3, args.steps // 2),
+                           sssp_class_bytes(rounds, h.m, h.nnz, relax_kernel(ctx, h.m)), "sssp")
     if args.scale == SSSP_SCALE:
         attach_traffic(roof, "sssp_rmat_s22")
     log(f"roofline({roof['kernel']}): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak); "
