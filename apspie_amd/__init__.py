@@ -106,13 +106,13 @@ class Stats(C.Structure):
         ("vertices", C.c_int64),
         ("improved", C.c_int64),
         ("iterations", C.c_int32),
-        ("reserved", C.c_int32),
+        ("relax_rounds", C.c_int32),
         ("bytes_alg", C.c_double),
         ("kernel_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None

@@ -361,7 +361,10 @@ __global__ __launch_bounds__(WGC) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     c.nTiles = tile0;
     __hip_atomic_store(&C->acc.edges, before + (unsigned long long)c.E, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (picked && C->acc.levels < level + 1) C->acc.levels = level + 1;
-    if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, before + c.E, prev_pull);
+    if constexpr (has_direction<Pick>::value) {
+        c.pull = pk.direction(c, before + c.E, prev_pull);
+        if (c.pull == DIR_BINNED) C->acc.binned += 1;  // the round's one writer
+    }
     if constexpr (has_defer<Pick>::value) {
         // nothing left below the threshold: raise it past the smallest
         // waiting distance and pick again next round
@@ -414,6 +417,7 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
     c.nTiles = ntiles;
     c.pull = 0;
     if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, (unsigned long long)deg, C->ctl[1].pull);
+    if (c.pull == DIR_BINNED) C->acc.binned = 1;
     C->ctl[0] = c;
     C->log[0] = c;
 }

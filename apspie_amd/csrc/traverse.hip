@@ -918,6 +918,12 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
     const unsigned pull_grid = std::max(1u, std::min(ceil_div(g->nnz, TILE), (unsigned)ctx->cu_count * 8u));
     const unsigned vert_grid = std::max(1u, std::min(ceil_div(g->m, WG), (unsigned)ctx->cu_count * 16u));
     const unsigned apply_grid = unsigned(nb + g->nnz / PART_ENTRIES + 1);
+    // binned rounds may still come: the rounds launched now carry the binned
+    // kernels and their compactions may choose binned rounds.  Off once a push
+    // round follows the binned ones (frontiers rise, then fall): the later
+    // rounds then launch without them, and their compactions cannot choose them.
+    bool bin_on = binned;
+    int relax_rounds = -1;  // rounds launched with bin_on, once it turns off
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     ctx->prof_begin(APSPIE_KCLASS_OTHER);
     // near/far threshold: the first delta of distance (none for BFS or delta 0)
@@ -940,7 +946,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                            PullRuleBfs{b.fbits, g->nnz, g->m, ctx->tune.bfs_alpha, ctx->tune.bfs_beta}};
         else
             return PickSssp{dist, b.prev, hint, g->m, seed, fval, g->nnz, ctx->tune.sssp_pull_ratio,
-                            binned ? ctx->tune.sssp_bin_ratio : 0};
+                            bin_on ? ctx->tune.sssp_bin_ratio : 0};
     };
     auto near_far = [&](const PickSssp &pk) {
         PickSsspNF nf;
@@ -985,7 +991,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         APSPIE_HIP(hipMemsetAsync(&b.C->rebuild, 0, sizeof(int), ctx->stream));
     };
     auto expand = [&](int level) {
-        if (binned) {
+        if (bin_on) {
             // one relaxation launch a round (push or binning), then the
             // binned round's scan, scatter and apply (which exit otherwise)
             ctx->prof_begin(APSPIE_KCLASS_BIN);
@@ -1067,12 +1073,19 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
         if (last.nF == 0 && last.pull != DIR_REPICK) break;
         // every round settles a vertex or, repicking, admits a waiting one
         APSPIE_CHECK(level <= 2 * int64_t(g->m) + 4, APSPIE_ENOCONV, "traversal did not converge");
+        if (bin_on && host_c->acc.binned > 0 && last.pull != DIR_BINNED) {
+            bin_on = false;  // past the dense rounds: a fresh chunk of plain rounds
+            relax_rounds = level - 1;
+            chunk = std::max(4, ctx->tune.round_chunk);
+            continue;
+        }
         ++checks;
         chunk = checks <= 2 ? 2 : std::min(chunk * 2, 32);
     }
     Acc acc;
     memcpy(&acc, (const void *)&host_c->acc, sizeof(Acc));
     fill_stats(ctx, b, MODE, g->m, stats, &acc);
+    if (stats && binned) stats->relax_rounds = relax_rounds >= 0 ? relax_rounds : level - 1;
 }
 
 void validate_traversal(const apspie_csr *g, int32_t src) {

@@ -55,7 +55,7 @@ struct Acc {
     unsigned long long edges;             // sum of frontier edge counts
     Padded<unsigned long long> vertices[SPREAD];  // vertices that entered a frontier
     int32_t levels;                       // rounds with a non-empty frontier
-    int32_t pad;
+    int32_t binned;                       // rounds marked binned (SSSP)
 };
 
 constexpr int LOG_ROUNDS = 64;  // rounds whose frontier descriptor is kept for apspie_ctx_rounds

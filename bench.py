@@ -64,7 +64,7 @@ def time_steps(ctx, fn, steps):
     return walls, kms, last
 
 
-def sssp_class_bytes(rounds, m, nnz, relax_kernel=False):
+def sssp_class_bytes(rounds, m, nnz, relax_rounds=0):
     """Algorithmic bytes per traversal of each SSSP kernel class, from the
     round log (frontier i = (vertices, edges, pull) is expanded by round i+1;
     round 0 is the seed kernel's, every later one a compaction's, and one more
@@ -83,8 +83,9 @@ def sssp_class_bytes(rounds, m, nnz, relax_kernel=False):
       vertex (distance write);
     compaction: 8 B per vertex (distance and round-start copy) + 28 B per
       queued vertex (copy update, row-pointer pair, 16-byte queue entry).
-    relax_kernel: binned rounds are on, so one k_relax launch a round runs the
-    push rounds as well as the binning expansions (class "bin")."""
+    relax_rounds (apspie_stats.relax_rounds): with binned rounds on, the
+    leading rounds whose relaxation was one k_relax launch, push or binning
+    (class "bin"); the later push rounds ran k_expand."""
     out = {"expand": 0.0, "pull": 0.0, "bin": 0.0, "apply": 0.0, "compact": 0.0}
     for i, (nf, e, mode) in enumerate(rounds):
         nxt = rounds[i + 1][0] if i + 1 < len(rounds) else 0
@@ -94,16 +95,9 @@ def sssp_class_bytes(rounds, m, nnz, relax_kernel=False):
             out["bin"] += 12.0 * e + 12.0 * nf
             out["apply"] += 4.0 * nxt
         else:
-            out["bin" if relax_kernel else "expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
+            out["bin" if i < relax_rounds else "expand"] += 12.0 * e + 12.0 * nf + 4.0 * nxt
         out["compact"] += 8.0 * m + 28.0 * nxt
     return out
-
-
-def relax_kernel(ctx, m):
-    """Whether SSSP on an m-vertex graph runs its rounds through k_relax
-    (binned rounds on: sssp_bin_ratio > 0 and m within the 512 buckets of
-    16384 vertices)."""
-    return ctx.tuning()["sssp_bin_ratio"] > 0 and m <= 512 * 16384
 
 
 def bfs_class_bytes(rounds, m):
@@ -495,7 +489,7 @@ def profile_workload(args):
             st = fn()
         rounds = ctx.rounds()
         info.update(m=h.m, nnz=h.nnz, rounds=rounds, device_ms=st.kernel_ms, stats=st.as_dict(),
-                    class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz, relax_kernel(ctx, h.m)) if w.startswith("sssp")
+                    class_bytes=(sssp_class_bytes(rounds, h.m, h.nnz, st.relax_rounds) if w.startswith("sssp")
                                  else bfs_class_bytes(rounds, h.m)))
     elif w in ("bfs_batch_s20_k256", "sssp_batch_s20_k64"):
         import ctypes as Ct
@@ -613,7 +607,7 @@ def main():
 
     rounds = ctx.rounds()
     roof = kernel_roofline(ctx, run, max(3, args.steps // 2),
-                           sssp_class_bytes(rounds, h.m, h.nnz, relax_kernel(ctx, h.m)), "sssp")
+                           sssp_class_bytes(rounds, h.m, h.nnz, st.relax_rounds), "sssp")
     if args.scale == SSSP_SCALE:
         attach_traffic(roof, "sssp_rmat_s22")
     log(f"roofline({roof['kernel']}): {roof['achieved']} GB/s ({roof['frac'] * 100:.1f}% of peak); "

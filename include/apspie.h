@@ -95,7 +95,9 @@ typedef struct {
     int64_t vertices;
     int64_t improved;
     int32_t iterations;
-    int32_t reserved;
+    int32_t relax_rounds; /* SSSP with binned rounds on: the leading rounds whose relaxation ran in one
+                             k_relax launch each (push or binned); the later ones ran the plain push
+                             expansion.  0 otherwise */
     double bytes_alg;
     double kernel_ms;
 } apspie_stats;

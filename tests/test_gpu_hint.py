@@ -152,3 +152,27 @@ def test_batched_rows_exact_for_weight_kinds(tuned, kind, k):
     want = np.stack([oracle_dist(h, int(s)) for s in srcs])
     dist, _ = tuned.sssp_batch(g, srcs)
     assert np.array_equal(dist, want), (kind, k)
+
+
+@pytest.mark.parametrize("chunk", [1, 2, 3, 5, 12])
+def test_binned_launches_stop_after_the_dense_rounds(tuned, chunk):
+    """With binned rounds on, the binned kernels are launched until a push
+    round follows binned ones at a chunk boundary; the later rounds run the
+    plain push expansion.  Results and the schedule are the same for every
+    chunk length, and relax_rounds covers every binned round."""
+    h = A.rmat(18, 16, 7418, undirected=False, weight_seed=8418)
+    g = tuned.upload(h)
+    src = h.argmax_degree()
+    want = oracle_dist(h, src)
+    tuned.set_tuning(sssp_bin_ratio=0)
+    _, st0 = tuned.sssp(g, src)
+    r0 = tuned.rounds()
+    assert st0.relax_rounds == 0
+    tuned.set_tuning(sssp_bin_ratio=3, round_chunk=chunk)
+    dist, st = tuned.sssp(g, src)
+    rounds = tuned.rounds()
+    assert np.array_equal(dist, want)
+    assert [r[:2] for r in rounds] == [r[:2] for r in r0]
+    binned = [i for i, (_, _, p) in enumerate(rounds) if p == 2]
+    assert binned and st.relax_rounds >= binned[-1] + 1
+    assert st.relax_rounds <= len(rounds) + max(chunk, 4)
