@@ -200,7 +200,12 @@ def attach_traffic(roof, workload):
     if not os.path.exists(path):
         return
     tr = json.load(open(path))
-    roof["traffic"] = int(tr["traffic_bytes_per_launch"])
+    # the counters' bytes per traversal, over this run's launches per
+    # traversal (launches that exit at once move nothing, so a different
+    # round chunk only changes the divisor)
+    per_call = tr["traffic_bytes_per_launch"] * tr["launches_per_call"]
+    roof["traffic"] = int(per_call / max(roof["launches_per_step"], 1e-9))
+    roof["traffic_per_traversal"] = int(per_call)
     roof["traffic_source"] = os.path.relpath(path, ROOT)
     roof["traffic_over_algorithmic"] = round(roof["traffic"] / max(roof["bytes_per_launch"], 1), 3)
 
