@@ -444,7 +444,7 @@ int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, in
 }
 
 int seg_radix_kv(apspie_ctx *ctx, int32_t *key[2], double *val[2], int64_t n, const std::vector<int32_t> &seg,
-                 int bits, std::vector<int32_t> &tiles) {
+                 int bits, std::vector<int32_t> &tiles, int first_shift) {
     int cur = 0;
     if (n == 0 || bits == 0) return cur;
     // the tile table: tiles of a segment are consecutive, histogram slots of
@@ -470,7 +470,7 @@ int seg_radix_kv(apspie_ctx *ctx, int32_t *key[2], double *val[2], int64_t n, co
     int32_t *hist = (int32_t *)ctx->slot(S_AUX4, size_t(nh) * 4);
     int32_t *offs = (int32_t *)ctx->slot(S_AUX5, size_t(nh + 1) * 4);
     const unsigned gt = std::max(1u, std::min((unsigned)ntiles, unsigned(ctx->cu_count) * 8));
-    for (int shift = 0; shift < bits; shift += 8, cur ^= 1) {
+    for (int shift = std::max(0, first_shift); shift < bits; shift += 8, cur ^= 1) {
         k_seg_hist<<<gt, T, 0, ctx->stream>>>(key[cur], dt, ntiles, shift, hist);
         scan32(ctx, hist, offs, nh);
         k_seg_scatter<<<gt, T, 0, ctx->stream>>>(key[cur], val[cur], dt, ntiles, shift, offs, key[cur ^ 1],

@@ -36,15 +36,16 @@ int radix_pairs(apspie_ctx *ctx, int32_t *key[2], int32_t *idx[2], int64_t n, in
 
 // Stable LSD radix sort of (key[0][e], val[0][e]) by the low `bits` bits of
 // the key (non-negative keys) inside each segment [seg[r], seg[r + 1]) (host
-// offsets covering [0, n) in order), 8 bits a pass, each key's FP64 value
-// moving with it.  A pass's tiles never cross a segment, and the histogram of
+// offsets covering [0, n) in order), 8 bits a pass from bit first_shift up
+// (0: a full sort; bits - 8: one pass that groups each segment by the top 8
+// bits, stably), each key's FP64 value moving with it.  A pass's tiles never cross a segment, and the histogram of
 // tile j of segment r sits at (segment r's first slot) + digit * (tiles of
 // r) + (j - r's first tile), so one exclusive scan gives every (segment,
 // digit, tile) its place.  Returns the buffer (0 or 1) holding the result.
 // `tiles` receives the tile table uploaded from it: keep it until the stream
 // synchronises.  Uses the S_AUX4, S_AUX5, S_AUX6 and S_STATUS slots.
 int seg_radix_kv(apspie_ctx *ctx, int32_t *key[2], double *val[2], int64_t n, const std::vector<int32_t> &seg,
-                 int bits, std::vector<int32_t> &tiles);
+                 int bits, std::vector<int32_t> &tiles, int first_shift = 0);
 
 // The CSC of a (rows ascending in each column) into caller memory; cv may be
 // null.  Uses the S_AUX0..S_AUX5 and S_STATUS slots.

@@ -823,6 +823,209 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
     }
 }
 
+// One stable LSD radix pass in LDS over items [0, cnt) of a window: digit =
+// (key >> sh) & 127, from (ik, iv) to (ok, ov).  Wave w takes items [w *
+// PER, (w + 1) * PER) in rounds of 64; per-wave digit counts, a digit-major
+// scan over (digit, wave), then ranks inside a round from ballots over the
+// digit's 7 bits.  Contains barriers; every thread of the workgroup calls it.
+template <int T, int W>
+__device__ __forceinline__ void lds_radix_pass(const int32_t *ik, const double *iv, int32_t *ok, double *ov, int cnt,
+                                               int sh, int32_t (*s_cnt)[128], int32_t *s_tot) {
+    constexpr int NW = T / WAVE, PER = W / NW, RR = PER / WAVE;
+    static_assert(T >= 128 && PER % WAVE == 0, "pass shape");
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int x = tid; x < NW * 128; x += T) s_cnt[x / 128][x % 128] = 0;
+    __syncthreads();
+    int k[RR];
+    double v[RR];
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        const int t = wid * PER + r * WAVE + lane;
+        k[r] = t < cnt ? ik[t] : -1;
+        v[r] = t < cnt ? iv[t] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < RR; ++r)
+        if (k[r] >= 0) atomicAdd(&s_cnt[wid][(k[r] >> sh) & 127], 1);
+    __syncthreads();
+    // thread d < 128: digit d's start (a DPP scan over the two waves of
+    // digits), then each wave's start inside it
+    int tot = 0;
+    if (tid < 128) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) tot += s_cnt[w][tid];
+    }
+    const uint32_t incl = wave_incl_sum_dpp(uint32_t(tot));
+    if (lane == WAVE - 1 && wid < 2) s_tot[wid] = int(incl);
+    __syncthreads();
+    if (tid < 128) {
+        int run = int(incl) - tot + (wid == 1 ? s_tot[0] : 0);
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int c = s_cnt[w][tid];
+            s_cnt[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RR; ++r) {
+        const bool valid = k[r] >= 0;
+        const int d = valid ? (k[r] >> sh) & 127 : 0;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 7; ++b) {
+            const uint64_t ones = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? ones : ~ones;
+        }
+        const int rank = __popcll(peers & lt);
+        const int pos = valid ? s_cnt[wid][d] + rank : 0;
+        if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
+        if (valid) {
+            ok[pos] = k[r];
+            ov[pos] = v[r];
+        }
+    }
+    __syncthreads();
+}
+
+// After one segmented radix pass on the top 8 bits of the column key
+// (digit = key >> shift), a long row's products form groups of equal digit
+// in increasing digit order.  A workgroup at a time per row (rows taken from
+// a counter, largest first) sorts the row stably by the whole key, in place,
+// one window of up to W products at a time: a window ends at a group
+// boundary unless the row ends there, so the windows' keys are ordered among
+// themselves and sorting each window alone sorts the row.  A window is
+// sorted in LDS by LSD radix passes of 7 bits; its runs of equal columns
+// (never split between windows: a window ends where the digit changes) are
+// summed in FP64 in k order, and each run's column and sum are written to
+// the row's staging at poff[r] + the runs of its earlier windows, in order
+// (columns outside B, key ncol, are dropped); nnzc[rows[r]] gets the row's
+// run count.  A group larger than a window (columns clustered in one digit
+// range) raises *ovf and leaves the rest of its row undone: the caller then
+// sorts the long rows by the full radix passes instead.
+template <int T, int W>
+__global__ __launch_bounds__(T) void k_long_lsort(int nrows, const int32_t *__restrict__ order,
+                                                  const int32_t *__restrict__ rows, const int32_t *__restrict__ poff,
+                                                  int shift, int bits, int ncol, const int32_t *__restrict__ key,
+                                                  const double *__restrict__ val, int32_t *scol, float *sval,
+                                                  int32_t *nnzc, int32_t *ovf, unsigned *next) {
+    constexpr int NW = T / WAVE, R = W / T;
+    __shared__ int32_t s_k[2][W];
+    __shared__ double s_v[2][W];
+    __shared__ int32_t s_cnt[NW][128];
+    __shared__ int32_t s_tot[128];
+    __shared__ int32_t s_cut, s_row;
+    __shared__ int32_t s_wh[NW];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    for (;;) {
+        if (tid == 0) s_row = int(atomicAdd(next, 1u));
+        __syncthreads();
+        const int o = s_row;
+        __syncthreads();  // s_row is read before it is written again
+        if (o >= nrows) break;
+        const int r = order[o];
+        const int rb = poff[r], re = poff[r + 1];
+        int runs = 0;  // the row's runs so far (the same in every thread)
+        bool done = true;
+        for (int w0 = rb; w0 < re;) {
+            const int n = min(W, re - w0);
+            {  // every load issued before the stores
+                int lk[R];
+                double lv[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int i = q * T + tid;
+                    if (i < n) {
+                        lk[q] = key[w0 + i];
+                        lv[q] = val[w0 + i];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const int i = q * T + tid;
+                    if (i < n) {
+                        s_k[0][i] = lk[q];
+                        s_v[0][i] = lv[q];
+                    }
+                }
+            }
+            if (tid == 0) s_cut = w0 + n < re ? 0 : n;  // the row goes on: the last group's start
+            __syncthreads();
+            if (w0 + n < re) {
+                const int dl = s_k[0][n - 1] >> shift;
+                for (int i = tid; i < n; i += T)
+                    if (i > 0 && (s_k[0][i] >> shift) == dl && (s_k[0][i - 1] >> shift) != dl) s_cut = i;
+            }
+            __syncthreads();
+            const int cut = s_cut;
+            if (cut == 0) {  // uniform: one group fills the window and goes on
+                if (tid == 0) atomicOr(ovf, 1);
+                done = false;
+                break;
+            }
+            int cur = 0;
+            for (int sh = 0; sh < bits; sh += 7, cur ^= 1)
+                lds_radix_pass<T, W>(s_k[cur], s_v[cur], s_k[cur ^ 1], s_v[cur ^ 1], cut, sh, s_cnt, s_tot);
+            // runs: thread t takes the sorted items [t * R, t * R + R)
+            const int32_t *sk = s_k[cur];
+            const double *sv = s_v[cur];
+            uint32_t heads = 0;
+            int nh = 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int i = tid * R + q;
+                if (i < cut && sk[i] < ncol && (i == 0 || sk[i] != sk[i - 1])) {
+                    heads |= 1u << q;
+                    ++nh;
+                }
+            }
+            const uint32_t incl = wave_incl_sum_dpp(uint32_t(nh));
+            if (lane == WAVE - 1) s_wh[wid] = int(incl);
+            __syncthreads();
+            int pos = int(incl) - nh, tot = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                if (w < wid) pos += s_wh[w];
+                tot += s_wh[w];
+            }
+            const int64_t ob = int64_t(rb) + runs;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (!((heads >> q) & 1u)) continue;
+                const int i = tid * R + q;
+                const int kk = sk[i];
+                double acc = sv[i];
+                for (int j = i + 1; j < cut && sk[j] == kk; ++j) acc += sv[j];  // in k order
+                scol[ob + pos] = kk;
+                sval[ob + pos] = float(acc);
+                ++pos;
+            }
+            runs += tot;
+            __syncthreads();  // the next window refills the staging
+            w0 += cut;
+        }
+        if (done && tid == 0) nnzc[rows[r]] = runs;
+    }
+}
+
+// C(i,:) of the long rows from their staging (k_long_lsort): row r's runs
+// at poff[r] go to crp[rows[r]], one workgroup a row.
+__global__ __launch_bounds__(256) void k_long_copy(int nrows, const int32_t *__restrict__ rows,
+                                                   const int32_t *__restrict__ poff, const int32_t *__restrict__ crp,
+                                                   const int32_t *__restrict__ scol, const float *__restrict__ sval,
+                                                   int32_t *ccol, float *cval) {
+    for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int i = rows[r];
+        const int c0 = crp[i], nz = crp[i + 1] - c0, s0 = poff[r];
+        for (int x = threadIdx.x; x < nz; x += 256) {
+            ccol[c0 + x] = scol[s0 + x];
+            cval[c0 + x] = sval[s0 + x];
+        }
+    }
+}
+
 // flag[s] = 1 where a run of equal columns starts inside a long row, after
 // the sort: s is its row's first product or its column differs from s - 1's.
 // Products outside B's columns (key ncol, sorted last in their row) start
@@ -931,14 +1134,18 @@ struct LongRows {
     const int32_t *rows = nullptr;
     int32_t *poff = nullptr, *brow = nullptr, *key = nullptr, *flag = nullptr, *runpos = nullptr;
     double *val = nullptr;
+    // the windowed sort's staging (k_long_lsort): each row's runs at poff[r]
+    bool fused = false;
+    int32_t *scol = nullptr;
+    float *sval = nullptr;
     // host sources of the uploaded tables, kept until the stream syncs
-    std::vector<int32_t> hpo, hbrow, htiles;
+    std::vector<int32_t> hpo, hbrow, htiles, hord;
 };
 
 // Symbolic phase of the long rows: products, the sort, runs, nnz(C(i,:)).
 // gub[r] = the products of row r of the list (their sum < 2^31).
 LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const CsrD &A,
-                       const CsrD &B, int32_t *nnzc) {
+                       const CsrD &B, int32_t *nnzc, int32_t *ovf) {
     LongRows L;
     const int ng = int(gub.size());
     std::vector<int32_t> &po = L.hpo;
@@ -965,7 +1172,32 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
                           ctx->stream>>>(rows, ng, A, B, L.poff, key[0], val[0]);
     APSPIE_HIP(hipGetLastError());
-    const int c = cvt::seg_radix_kv(ctx, key, val, n, po, esc::bits_for(int64_t(B.n) + 1), L.htiles);
+    // ovf (device): one radix pass on the top 8 bits, then each row's digit
+    // groups sorted in LDS (k_long_lsort; a group too large for it raises
+    // *ovf); null: the full radix sort
+    const int bits = esc::bits_for(int64_t(B.n) + 1);
+    const bool msb = ovf != nullptr && bits > 8;
+    const int c = cvt::seg_radix_kv(ctx, key, val, n, po, bits, L.htiles, msb ? bits - 8 : 0);
+    if (msb) {
+        // rows by decreasing product count (a row is one workgroup's work)
+        std::vector<int32_t> &ord = L.hord;
+        ord.resize(size_t(ng));
+        for (int r = 0; r < ng; ++r) ord[r] = r;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return gub[x] > gub[y]; });
+        int32_t *dord = (int32_t *)S(7, int64_t(ng) * 4 + 256);
+        unsigned *next = (unsigned *)(dord + ((ng + 63) / 64) * 64);
+        APSPIE_HIP(hipMemcpyAsync(dord, ord.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
+        k_zero_i32<<<1, 64, 0, ctx->stream>>>((int32_t *)next, 1);
+        // the staging: the buffers the pass left free
+        L.fused = true;
+        L.scol = key[c ^ 1];
+        L.sval = (float *)val[c ^ 1];
+        k_long_lsort<256, 2048><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 3)), 256, 0,
+                                 ctx->stream>>>(ng, dord, rows, L.poff, bits - 8, bits, B.n, key[c], val[c], L.scol,
+                                                L.sval, nnzc, ovf, next);
+        APSPIE_HIP(hipGetLastError());
+        return L;
+    }
     L.key = key[c];
     L.val = val[c];
     L.flag = key[c ^ 1];  // free after the last pass
@@ -981,6 +1213,12 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
 }
 
 void long_numeric(apspie_ctx *ctx, const LongRows &L, const int32_t *crp, int32_t *ccol, float *cval) {
+    if (L.fused) {
+        k_long_copy<<<std::max(1u, std::min((unsigned)L.nrows, unsigned(ctx->cu_count) * 8)), 256, 0, ctx->stream>>>(
+            L.nrows, L.rows, L.poff, crp, L.scol, L.sval, ccol, cval);
+        APSPIE_HIP(hipGetLastError());
+        return;
+    }
     k_long_runs<<<cvt::grid_for(ctx, L.n, 256), 256, 0, ctx->stream>>>(L.n, L.nrows, L.rows, L.poff, L.brow,
                                                                        L.flag, L.runpos, L.key, L.val, crp, ccol,
                                                                        cval);
@@ -1197,15 +1435,26 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             launch_bins<false>(ctx, P03, A, B, nnzc, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         }
         LongRows lr;
-        if (esc_rows)
-            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc);
-        else if (P.counts[LONG])
+        int32_t *ovf = nullptr, hovf = 0;
+        if (esc_rows) {
+            ovf = (int32_t *)ctx->slot(S_OUTER0 + 8, 256);
+            k_zero_i32<<<1, 256, 0, ctx->stream>>>(ovf, 1);
+            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc, ovf);
+            APSPIE_HIP(hipMemcpyAsync(&hovf, ovf, 4, hipMemcpyDeviceToHost, ctx->stream));
+        } else if (P.counts[LONG]) {
             launch_bins<false>(ctx, P4, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
-        // 4. row pointers
+        }
+        // 4. row pointers (the scan's synchronisation also brings the long
+        // rows' overflow flag; on overflow they are sorted again by the full
+        // radix passes and the scan repeated)
         APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
         c->m = m;
         c->n = b->n;
-        const int32_t total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
+        int32_t total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
+        if (hovf) {
+            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc, nullptr);
+            total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
+        }
         c->nnz = total;
         if (hipMalloc(&c->col_ind, size_t(std::max(total, 1)) * 4) != hipSuccess ||
             hipMalloc(&c->val, size_t(std::max(total, 1)) * 4) != hipSuccess) {
