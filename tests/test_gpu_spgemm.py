@@ -272,6 +272,33 @@ def test_spgemm_long_rows_skewed_exact(ctx):
     check(got[0], want)
 
 
+def test_spgemm_long_rows_windowed_runs_exact(ctx):
+    """Long rows that the windowed LDS sort takes whole (every group of one
+    top-digit range stays under a window): rows of about 18000 products over
+    several windows, and runs of about 12 equal columns from a set of 512
+    spread over the whole column range, summed in FP64 in k order.  C is the
+    oracle's, values included."""
+    rng = np.random.default_rng(23)
+    n, mb = 1 << 20, 50000
+    common = np.sort(rng.choice(n, 512, replace=False))
+
+    def b_row(r, l):
+        cols = set(rng.choice(common, 4, replace=False).tolist())
+        while len(cols) < l:
+            cols.add(int(rng.integers(n)))
+        return np.array(sorted(cols))
+    brp, bcol, bval = _rows_of(rng, np.full(mb, 12), b_row)
+    hb = A.HostCsr(mb, n, brp, bcol, bval)
+    alens = np.concatenate([np.full(24, 1500), rng.integers(200, 400, 8)])
+    arp, acol, aval = _rows_of(rng, alens, lambda r, l: rng.choice(mb, l, replace=False))
+    ha = A.HostCsr(len(alens), mb, arp, acol, aval)
+    prods = long_row_products(ha, hb)
+    assert prods.max() > 3 * 2048 and prods.min() > 1024
+    c, _ = run(ctx, ha, hb)
+    want, _ = O.spgemm(ocsr(ha), ocsr(hb))
+    assert np.array_equal(c.val, want.val)
+
+
 @pytest.mark.skip(reason="the LDS-sorted long-row path (a workgroup per long row, column parts radix-sorted "
                   "in LDS, spgemm_long_rows=2) faulted on its first GPU run; neither reading nor a CPU "
                   "emulation of it showed a cause, so the path is not in the library")
