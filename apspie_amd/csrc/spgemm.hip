@@ -1,32 +1,32 @@
-// spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row-binned LDS
-// hash tables for rows up to 1024 products (spgemm_esc_min; the tables reach
-// 2048), expand-sort-compress beyond.
+// spgemm.hip -- C = A * B on gfx950: row-wise Gustavson with row bins:
+// register rows, LDS hash tables, LDS sorts up to 1024 products
+// (spgemm_esc_min; the sorts reach 2048), expand-sort-compress beyond.
 //
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
-//                edge-parallel over A with the frontier kernels.
+//                an int64 scan of every A entry's B row length.
 //   2. bins    rows by ub: <= 16 (four rows a wave) and <= 64 (a wave per
-//              row), both in registers; <= 128, <= 256, <= 512, <= 1024,
-//              <= 2048 (LDS tables of 512 .. 8192 slots, 128 .. 1024 threads
-//              per row); > 2048 (the long rows, joined by the bins above
-//              spgemm_esc_min, 1024 by default: expand-sort-compress moves a
-//              product faster than the largest tables).
-//   3. LDS rows in one pass: insert (column, a * b) with FP64 accumulation
-//      (products of FP32 inputs are exact in FP64; the sum is rounded to FP32
-//      once) and write the sorted row to a staging area at the prefix of the
-//      rows' ub, with its nnz.  Long rows: their products are written once
-//      in row order, sorted stably by column inside each row's range (a
-//      segmented radix sort that moves the FP64 values with the keys), and
-//      each run of equal columns is one entry (FP64 sum in k order); nnz =
-//      the runs.
+//              row), both in registers; <= 128, <= 256 (LDS hash tables of
+//              512 and 1024 slots); <= 512, <= 1024, <= 2048 (the row's
+//              products sorted in LDS, a workgroup a row); > 2048 (the long rows,
+//              joined by the bins above spgemm_esc_min, 1024 by default).
+//   3. LDS rows in one pass: the products (column, a * b) with FP64 sums
+//      (products of FP32 inputs are exact in FP64; a sum is rounded to FP32
+//      once), the sorted row written to a staging area at the prefix of the
+//      rows' ub, with its nnz.  A sorted row sums each run of equal columns
+//      in k order, as Gustavson's row accumulator does.  Long rows: their
+//      products are written once in row order, sorted stably by column
+//      inside each row's range (a radix pass on the top bits, then windows
+//      sorted in LDS), and each run of equal columns is one entry (FP64 sum
+//      in k order); nnz = the runs.
 //   4. row_ptr(C) = exclusive scan (decoupled look-back).
-//   5. the staged rows are copied into C; the long rows' runs are summed
+//   5. the staged rows are copied into C; the long rows' runs are copied
 //      into C.
 // Tuning keeps the older forms selectable (and they take over when int32
 // offsets would overflow): a counting pass before the product pass for the
 // LDS rows (spgemm_passes = 2), global hash tables of 4 * pow2ceil(ub) slots
 // for the long rows (spgemm_long_rows = 0).
-// The table uses an order-preserving home slot, home(c) = c * H / n, with
+// The hash table uses an order-preserving home slot, home(c) = c * H / n, with
 // linear probing and no wrap-around (2H slots for at most H / 2 keys): a key's
 // slot lies in the cluster (run of occupied slots) holding its home, so all
 // keys of an earlier cluster are smaller and the row comes out sorted after
@@ -1010,6 +1010,139 @@ __global__ __launch_bounds__(T) void k_long_lsort(int nrows, const int32_t *__re
     }
 }
 
+// Rows of at most W products by sorting instead of hashing: one workgroup a
+// row (taken from a counter) expands the row's products into LDS in k order
+// (column key, FP64 product; a column outside B gets the key ncol), sorts
+// them stably by column in LDS (7-bit LSD passes), and sums each run of equal
+// columns in FP64 in k order: nnzc[i] = the runs; NUMERIC also writes them
+// at crp[i] in column order.  The sums are Gustavson's, so the rows come out
+// identical to the oracle's.
+template <int T, int W, bool NUMERIC>
+__global__ __launch_bounds__(T) void k_rows_sort(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
+                                                 int bits, int32_t *nnzc, const int32_t *__restrict__ crp,
+                                                 int32_t *ccol, float *cval, unsigned *next) {
+    constexpr int NW = T / WAVE, R = W / T;
+    __shared__ int32_t s_k[2][W];
+    __shared__ double s_v[2][W];
+    __shared__ int32_t s_cnt[NW][128];
+    __shared__ int32_t s_tot[128];
+    __shared__ int32_t s_bs[T], s_pre[T];
+    __shared__ float s_a[T];
+    __shared__ int32_t s_tmp[NW + 1], s_wh[NW];
+    __shared__ int32_t s_row;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int ncol = B.n;
+    for (;;) {
+        if (tid == 0) s_row = int(atomicAdd(next, 1u));
+        __syncthreads();
+        const int ri = s_row;
+        __syncthreads();  // s_row is read before it is written again
+        if (ri >= nrows) break;
+        const int i = rows[ri];
+        // 1. the products in k order
+        int n = 0;
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        for (int jb = ab; jb < ae; jb += T) {
+            const int j = jb + tid;
+            int L = 0, bs = 0;
+            float a = 0.f;
+            if (j < ae) {
+                const int k = A.col[j];
+                if ((unsigned)k < (unsigned)B.m) {
+                    a = A.val ? A.val[j] : 1.0f;
+                    bs = B.rp[k];
+                    L = B.rp[k + 1] - bs;
+                }
+            }
+            int tot;
+            const int pre = block_excl_scan<T>(L, s_tmp, tot);
+            s_bs[tid] = bs;
+            s_pre[tid] = pre;
+            s_a[tid] = a;
+            const int ne = min(T, ae - jb);
+            __syncthreads();
+            constexpr int U = 4;  // products per thread in flight
+            int lo = 0;
+            for (int tb = tid; tb < tot; tb += U * T) {
+                int cq[U];
+                double pq[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = tb + u * T;
+                    cq[u] = -1;
+                    if (t < tot) {
+                        if (u == 0) {
+                            int hi = ne - 1;  // last entry whose prefix <= t
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (s_pre[mid] <= t) lo = mid;
+                                else hi = mid - 1;
+                            }
+                        } else {
+                            while (lo + 1 < ne && s_pre[lo + 1] <= t) ++lo;
+                        }
+                        const int q = s_bs[lo] + (t - s_pre[lo]);
+                        cq[u] = B.col[q];
+                        pq[u] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = tb + u * T;
+                    if (t < tot) {
+                        s_k[0][n + t] = (unsigned)cq[u] < (unsigned)ncol ? cq[u] : ncol;
+                        s_v[0][n + t] = pq[u];
+                    }
+                }
+            }
+            n += tot;
+            __syncthreads();  // the next chunk's scan rewrites s_bs / s_pre
+        }
+        // 2. the sort (n <= W)
+        int cur = 0;
+        for (int sh = 0; sh < bits; sh += 7, cur ^= 1)
+            lds_radix_pass<T, W>(s_k[cur], s_v[cur], s_k[cur ^ 1], s_v[cur ^ 1], n, sh, s_cnt, s_tot);
+        // 3. the runs: thread t takes the sorted items [t * R, t * R + R)
+        const int32_t *sk = s_k[cur];
+        const double *sv = s_v[cur];
+        uint32_t heads = 0;
+        int nh = 0;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int x = tid * R + q;
+            if (x < n && sk[x] < ncol && (x == 0 || sk[x] != sk[x - 1])) {
+                heads |= 1u << q;
+                ++nh;
+            }
+        }
+        const uint32_t incl = wave_incl_sum_dpp(uint32_t(nh));
+        if (lane == WAVE - 1) s_wh[wid] = int(incl);
+        __syncthreads();
+        int pos = int(incl) - nh, tot = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            if (w < wid) pos += s_wh[w];
+            tot += s_wh[w];
+        }
+        if (tid == 0) nnzc[i] = tot;
+        if (NUMERIC) {
+            const int o = crp[i];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (!((heads >> q) & 1u)) continue;
+                const int x = tid * R + q;
+                const int kk = sk[x];
+                double acc = sv[x];
+                for (int y = x + 1; y < n && sk[y] == kk; ++y) acc += sv[y];  // in k order
+                ccol[o + pos] = kk;
+                cval[o + pos] = float(acc);
+                ++pos;
+            }
+        }
+        __syncthreads();  // the next row refills the staging
+    }
+}
+
 // C(i,:) of the long rows from their staging (k_long_lsort): row r's runs
 // at poff[r] go to crp[rows[r]], one workgroup a row.
 __global__ __launch_bounds__(256) void k_long_copy(int nrows, const int32_t *__restrict__ rows,
@@ -1232,36 +1365,63 @@ struct Plan {
     int64_t gslots_total = 0;
 };
 
+// Workgroups of `kernel` resident on the device at once (its register and
+// LDS use, from the runtime's occupancy query), capped by `want`: the grid of
+// a kernel whose workgroups take rows from a counter.
+template <class K>
+unsigned resident_grid(const apspie_ctx *ctx, K kernel, int threads, int64_t want) {
+    int per = 0;
+    APSPIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->cu_count) * std::max(per, 1))));
+}
+
 template <bool NUMERIC>
 void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, int32_t *nnzc, const int32_t *crp,
                  int32_t *ccol, float *cval, int32_t *gkeys, double *gvals, const int64_t *goff,
                  const int32_t *gslots) {
     const int m = A.m;
-    auto grid = [&](int n, unsigned per_cu) {
-        return std::max(1u, std::min((unsigned)n, (unsigned)ctx->cu_count * per_cu));
+    auto L = [&](int k) { return P.lists + k * int64_t(m); };
+    // the register and hash-table bins stride over their rows; workgroups a
+    // CU as measured on the 1M power-law A * A (the hash bins are slower
+    // with all of their resident workgroups: 14 of 16 and 10 of 12)
+    auto grid = [&](int64_t n, unsigned per_cu) {
+        return unsigned(std::max<int64_t>(1, std::min<int64_t>(n, int64_t(ctx->cu_count) * per_cu)));
     };
     if (P.counts[0])
-        k_rows_tiny<NUMERIC><<<std::max(1u, std::min(ceil_div(P.counts[0], 16), (unsigned)ctx->cu_count * 16)), 256, 0,
-                               ctx->stream>>>(P.lists + 0 * int64_t(m), P.counts[0], A, B, nnzc, crp, ccol, cval);
+        k_rows_tiny<NUMERIC><<<grid(ceil_div(P.counts[0], 16), 16), 256, 0, ctx->stream>>>(L(0), P.counts[0], A, B,
+                                                                                          nnzc, crp, ccol, cval);
     if (P.counts[1])
-        k_rows_wave<NUMERIC><<<std::max(1u, std::min(ceil_div(P.counts[1], 256 / WAVE), (unsigned)ctx->cu_count * 16)),
-                               256, 0, ctx->stream>>>(P.lists + 1 * int64_t(m), P.counts[1], A, B, nnzc, crp, ccol,
-                                                      cval);
+        k_rows_wave<NUMERIC><<<grid(ceil_div(P.counts[1], 256 / WAVE), 16), 256, 0, ctx->stream>>>(
+            L(1), P.counts[1], A, B, nnzc, crp, ccol, cval);
     if (P.counts[2])
-        k_rows<128, 512, NUMERIC, false><<<grid(P.counts[2], 8), 128, 0, ctx->stream>>>(
-            P.lists + 2 * int64_t(m), P.counts[2], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+        k_rows<128, 512, NUMERIC, false><<<grid(P.counts[2], 14), 128, 0, ctx->stream>>>(
+            L(2), P.counts[2], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
     if (P.counts[3])
-        k_rows<128, 1024, NUMERIC, false><<<grid(P.counts[3], 8), 128, 0, ctx->stream>>>(
-            P.lists + 3 * int64_t(m), P.counts[3], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
-    if (P.counts[4])
-        k_rows<256, 2048, NUMERIC, false><<<grid(P.counts[4], 4), 256, 0, ctx->stream>>>(
-            P.lists + 4 * int64_t(m), P.counts[4], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
-    if (P.counts[5])
-        k_rows<512, 4096, NUMERIC, false><<<grid(P.counts[5], 2), 512, 0, ctx->stream>>>(
-            P.lists + 5 * int64_t(m), P.counts[5], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
-    if (P.counts[6])
-        k_rows<1024, 8192, NUMERIC, false><<<grid(P.counts[6], 1), 1024, 0, ctx->stream>>>(
-            P.lists + 6 * int64_t(m), P.counts[6], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+        k_rows<128, 1024, NUMERIC, false><<<grid(P.counts[3], 10), 128, 0, ctx->stream>>>(
+            L(3), P.counts[3], A, B, nnzc, crp, ccol, cval, nullptr, nullptr, nullptr, nullptr);
+    // rows of 257 to 2048 products: sorted in LDS, a workgroup a row taken
+    // from a counter (each bin its own)
+    unsigned *next = nullptr;
+    if (P.counts[4] || P.counts[5] || P.counts[6]) {
+        next = (unsigned *)ctx->slot(S_OUTER0 + 9, 64);
+        k_zero_i32<<<1, 64, 0, ctx->stream>>>((int32_t *)next, 16);
+    }
+    const int bits = esc::bits_for(int64_t(B.n) + 1);
+    if (P.counts[4]) {
+        auto k = k_rows_sort<128, 512, NUMERIC>;
+        k<<<resident_grid(ctx, k, 128, P.counts[4]), 128, 0, ctx->stream>>>(L(4), P.counts[4], A, B, bits, nnzc, crp,
+                                                                            ccol, cval, next + 4);
+    }
+    if (P.counts[5]) {
+        auto k = k_rows_sort<256, 1024, NUMERIC>;
+        k<<<resident_grid(ctx, k, 256, P.counts[5]), 256, 0, ctx->stream>>>(L(5), P.counts[5], A, B, bits, nnzc, crp,
+                                                                            ccol, cval, next + 5);
+    }
+    if (P.counts[6]) {
+        auto k = k_rows_sort<256, 2048, NUMERIC>;
+        k<<<resident_grid(ctx, k, 256, P.counts[6]), 256, 0, ctx->stream>>>(L(6), P.counts[6], A, B, bits, nnzc, crp,
+                                                                            ccol, cval, next + 6);
+    }
     if (P.counts[LONG])
         k_rows<1024, 1, NUMERIC, true><<<grid(P.counts[LONG], 2), 1024, 0, ctx->stream>>>(
             P.lists + LONG * int64_t(m), P.counts[LONG], A, B, nnzc, crp, ccol, cval, gkeys, gvals, goff, gslots);

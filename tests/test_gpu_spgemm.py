@@ -218,6 +218,62 @@ def test_spgemm_register_bins_exact(ctx):
     assert np.array_equal(c.val, want.val)
 
 
+@pytest.mark.parametrize("esc_min,passes", [(1024, 1), (2048, 1), (2048, 2)])
+def test_spgemm_sorted_bins_exact(ctx, esc_min, passes):
+    """Rows of 128 to 2048 products (LDS hash tables up to 256, then sorted
+    in LDS, a workgroup a row) at every bin boundary (128/129, 256/257,
+    512/513, 1024/1025, 2048), with
+    runs of equal columns (a narrow column range), columns outside B
+    (dropped), and A rows of several hundred entries most of whose B rows are
+    empty (several expansion chunks).  With spgemm_esc_min 2048 the rows of
+    1025 to 2048 products are sorted too; two passes count before they write.
+    Every run is summed in FP64 in k order, as the oracle does, so C is
+    identical to the oracle's, values included."""
+    rng = np.random.default_rng(29)
+    n, mb = 700, 6000
+    blen = rng.choice([0, 0, 0, 1, 3, 6, 9, 12], mb)
+    brp = np.zeros(mb + 1, np.int32)
+    brp[1:] = np.cumsum(blen)
+    bcol = np.concatenate([np.sort(rng.choice(n + 3, l, replace=False)) for l in blen]).astype(np.int32)
+    bval = rng.uniform(-2, 2, bcol.size).astype(np.float32)
+    hb = A.HostCsr(mb, n, brp, bcol, bval)  # columns n .. n+2 lie outside B
+    full = np.nonzero(blen > 0)[0]
+    empty = np.nonzero(blen == 0)[0]
+    rows = []
+    for target in [128, 129, 200, 256, 257, 400, 512, 513, 777, 1024, 1025, 1500, 2047, 2048] * 3:
+        ks, tot = [], 0
+        for k in rng.permutation(full):
+            if tot + blen[k] <= target:
+                ks.append(k)
+                tot += blen[k]
+            if tot == target:
+                break
+        assert tot == target
+        if rng.random() < 0.5:  # empty B rows: more A entries than a chunk
+            ks += list(rng.choice(empty, 300, replace=False))
+        rows.append(np.sort(np.array(ks, np.int32)))
+    arp = np.zeros(len(rows) + 1, np.int32)
+    arp[1:] = np.cumsum([r.size for r in rows])
+    acol = np.concatenate(rows).astype(np.int32)
+    ha = A.HostCsr(len(rows), mb, arp, acol, rng.uniform(-2, 2, acol.size).astype(np.float32))
+    ub = long_row_products(ha, hb)
+    assert ub.min() == 128 and ub.max() == 2048
+    keep = bcol < n
+    krp = np.zeros(mb + 1, np.int32)
+    krp[1:] = np.cumsum(np.bincount(np.repeat(np.arange(mb), blen)[keep], minlength=mb))
+    want, _ = O.spgemm(ocsr(ha), O.Csr(mb, n, krp, bcol[keep], bval[keep]))
+    assert (np.diff(want.row_ptr) < ub).all()  # every row has runs
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(spgemm_esc_min=esc_min, spgemm_passes=passes)
+        c, st = ctx.spgemm(ctx.upload(ha), ctx.upload(hb))
+    finally:
+        ctx.set_tuning(**saved)
+    assert st.edges == ub.sum()
+    assert np.array_equal(c.row_ptr, want.row_ptr) and np.array_equal(c.col_ind, want.col_ind)
+    assert np.array_equal(c.val, want.val)
+
+
 def skewed_long_rows():
     """A * B whose rows are all over 2048 products: rows over uniform B rows
     (up to 21000 products), rows whose B rows crowd a band of 300 low columns,
