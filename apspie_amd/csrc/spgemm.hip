@@ -14,11 +14,12 @@
 //      (products of FP32 inputs are exact in FP64; a sum is rounded to FP32
 //      once), the sorted row written to a staging area at the prefix of the
 //      rows' ub, with its nnz.  A sorted row sums each run of equal columns
-//      in k order, as Gustavson's row accumulator does.  Long rows: their
-//      products are written once in row order, sorted stably by column
-//      inside each row's range (a radix pass on the top bits, then windows
-//      sorted in LDS), and each run of equal columns is one entry (FP64 sum
-//      in k order); nnz = the runs.
+//      in k order, as Gustavson's row accumulator does.  Long rows: a count
+//      of each row's products by the top 8 bits of their column, then the
+//      products written once, each straight into its row's group of equal
+//      top bits in k order; each window of a row's groups is sorted by column
+//      in LDS and each run of equal columns is one entry (FP64 sum in k
+//      order); nnz = the runs.
 //   4. row_ptr(C) = exclusive scan (decoupled look-back).
 //   5. the staged rows are copied into C; the long rows' runs are copied
 //      into C.
@@ -229,6 +230,19 @@ __device__ __forceinline__ int block_excl_scan(int x, int *s_tmp, int &total) {
     __syncthreads();
     total = tot;
     return pre + incl - x;
+}
+
+// The lanes of the wave holding the same BITS-bit digit d as this lane (and
+// valid), from a ballot per bit.
+template <int BITS>
+__device__ __forceinline__ uint64_t digit_peers(bool valid, int d) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const uint64_t ones = __ballot((d >> b) & 1);
+        peers &= ((d >> b) & 1) ? ones : ~ones;
+    }
+    return peers;
 }
 
 template <bool GLOBAL>
@@ -771,7 +785,8 @@ __global__ __launch_bounds__(256) void k_rows_wave(const int32_t *__restrict__ r
 // dropped later as the hash tables skip it), value a * b in FP64.  A stable
 // sort by column inside each row's range (cvt::seg_radix_kv, the values
 // moving with the keys) leaves each output entry's products in increasing k,
-// so the FP64 run sums are the tables' and Gustavson's.
+// so the FP64 run sums are the tables' and Gustavson's.  (The full radix
+// sort's path; the windowed path expands through k_long_dscatter.)
 template <int T>
 __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ rows, int nrows, CsrD A, CsrD B,
                                                    const int32_t *__restrict__ poff, int32_t *key, double *pval) {
@@ -823,11 +838,236 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
     }
 }
 
+// A chunk of up to T * EPT entries of A(i,:) from jb (thread t takes the EPT
+// consecutive entries from jb + t * EPT, their loads issued together): each
+// entry's B row start, its products' prefix within the chunk and its A
+// value into LDS; returns the chunk's products.  Contains barriers; the
+// caller synchronises before reading the arrays.
+template <int T, int EPT>
+__device__ __forceinline__ int chunk_entries(const CsrD &A, const CsrD &B, int jb, int ae, int32_t *s_bs,
+                                             int32_t *s_pre, float *s_a, int32_t *s_tmp) {
+    const int tid = threadIdx.x;
+    int k[EPT], bs[EPT], L[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int j = jb + tid * EPT + e;
+        k[e] = j < ae ? A.col[j] : -1;
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        bs[e] = 0;
+        L[e] = 0;
+        if ((unsigned)k[e] < (unsigned)B.m) {
+            bs[e] = B.rp[k[e]];
+            L[e] = B.rp[k[e] + 1] - bs[e];
+        }
+    }
+    int sum = 0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) sum += L[e];
+    int tot;
+    int pre = block_excl_scan<T>(sum, s_tmp, tot);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+        const int j = jb + tid * EPT + e;
+        s_bs[tid * EPT + e] = bs[e];
+        s_pre[tid * EPT + e] = pre;
+        if (s_a) s_a[tid * EPT + e] = (j < ae && A.val) ? A.val[j] : 1.0f;
+        pre += L[e];
+    }
+    return tot;
+}
+
+// The long rows' expansion fused with the first radix pass on the top bits
+// of the column (digit = key >> shift, below 256 since key <= ncol <
+// 2^(shift + 8)): k_long_dhist counts each row's digits from B's columns
+// alone, k_long_dscatter expands the row again in k order and writes each
+// product (key, FP64 value) straight to its place among the row's digit
+// groups, stable.  A workgroup a row, rows taken from a counter in `order`
+// (largest first); both kernels make the same products in the same order.
+template <int T, int U, int EPT>
+__global__ __launch_bounds__(T) void k_long_dhist(int nrows, const int32_t *__restrict__ order,
+                                                  const int32_t *__restrict__ rows, CsrD A, CsrD B, int shift,
+                                                  int32_t *hist, unsigned *next) {
+    constexpr int NW = T / WAVE;
+    static_assert(T == 256, "one thread per digit");
+    __shared__ int32_t s_h[NW][256];
+    __shared__ int32_t s_bs[T * EPT], s_pre[T * EPT];
+    __shared__ int32_t s_tmp[NW + 1];
+    __shared__ int32_t s_row;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const int ncol = B.n;
+    for (;;) {
+        if (tid == 0) s_row = int(atomicAdd(next, 1u));
+#pragma unroll
+        for (int w = 0; w < NW; ++w) s_h[w][tid] = 0;
+        __syncthreads();
+        const int o = s_row;
+        __syncthreads();  // s_row is read before it is written again
+        if (o >= nrows) break;
+        const int r = order[o];
+        const int i = rows[r];
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        for (int jb = ab; jb < ae; jb += T * EPT) {
+            const int tot = chunk_entries<T, EPT>(A, B, jb, ae, s_bs, s_pre, nullptr, s_tmp);
+            const int ne = min(T * EPT, ae - jb);
+            __syncthreads();
+            int lo = 0;
+            for (int tb = 0; tb < tot; tb += U * T) {  // uniform over the workgroup
+                int cq[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = tb + u * T + tid;
+                    cq[u] = -1;
+                    if (t < tot) {
+                        int hi = ne - 1;  // last entry whose prefix <= t (t only grows)
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_pre[mid] <= t) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        cq[u] = B.col[s_bs[lo] + (t - s_pre[lo])];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {  // one add per digit a wave holds
+                    const bool valid = tb + u * T + tid < tot;
+                    const int d = valid ? ((unsigned)cq[u] < (unsigned)ncol ? cq[u] : ncol) >> shift : 0;
+                    const uint64_t peers = digit_peers<8>(valid, d);
+                    if (valid && __popcll(peers & ((1ull << lane) - 1)) == 0)
+                        atomicAdd(&s_h[wid][d], __popcll(peers));
+                }
+            }
+            __syncthreads();  // the next chunk's scan rewrites s_bs / s_pre
+        }
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) c += s_h[w][tid];
+        hist[int64_t(r) * 256 + tid] = c;
+        __syncthreads();  // the next row clears the counts
+    }
+}
+
+template <int T, int U, int EPT>
+__global__ __launch_bounds__(T) void k_long_dscatter(int nrows, const int32_t *__restrict__ order,
+                                                     const int32_t *__restrict__ rows, CsrD A, CsrD B,
+                                                     const int32_t *__restrict__ poff, int shift,
+                                                     const int32_t *__restrict__ hist, int32_t *key_out,
+                                                     double *val_out, unsigned *next) {
+    constexpr int NW = T / WAVE, C = T * U;  // C products a step; wave w holds [w * 64U, (w + 1) * 64U)
+    static_assert(T == 256, "one thread per digit");
+    __shared__ int32_t s_cnt[NW][256];
+    __shared__ int32_t s_cur[256];   // the digit's next place in the row
+    __shared__ int32_t s_goff[256];  // the digit's place less its start in the step's staging
+    __shared__ int32_t s_k[C];
+    __shared__ double s_v[C];
+    __shared__ int32_t s_bs[T * EPT], s_pre[T * EPT];
+    __shared__ float s_a[T * EPT];
+    __shared__ int32_t s_tmp[NW + 1];
+    __shared__ int32_t s_row;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+    const uint64_t lt = (1ull << lane) - 1;
+    const int ncol = B.n;
+    for (;;) {
+        if (tid == 0) s_row = int(atomicAdd(next, 1u));
+        __syncthreads();
+        const int o = s_row;
+        __syncthreads();  // s_row is read before it is written again
+        if (o >= nrows) break;
+        const int r = order[o];
+        const int i = rows[r];
+        {
+            const int h = hist[int64_t(r) * 256 + tid];
+            int tot;
+            const int ex = block_excl_scan<T>(h, s_tmp, tot);
+            s_cur[tid] = poff[r] + ex;
+        }
+        const int ab = A.rp[i], ae = A.rp[i + 1];
+        for (int jb = ab; jb < ae; jb += T * EPT) {
+            const int tot = chunk_entries<T, EPT>(A, B, jb, ae, s_bs, s_pre, s_a, s_tmp);
+            const int ne = min(T * EPT, ae - jb);
+            int lo = 0;
+            for (int tb = 0; tb < tot; tb += C) {  // uniform over the workgroup
+#pragma unroll
+                for (int w = 0; w < NW; ++w) s_cnt[w][tid] = 0;
+                __syncthreads();  // (also publishes s_bs / s_pre / s_a)
+                int k[U];
+                double v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int t = tb + wid * (WAVE * U) + u * WAVE + lane;
+                    k[u] = -1;
+                    v[u] = 0.0;
+                    if (t < tot) {
+                        int hi = ne - 1;  // last entry whose prefix <= t (t only grows)
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_pre[mid] <= t) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        const int q = s_bs[lo] + (t - s_pre[lo]);
+                        k[u] = B.col[q];
+                        v[u] = double(s_a[lo]) * double(B.val ? B.val[q] : 1.0f);
+                    }
+                }
+                uint64_t pe[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {  // one add per digit a wave holds
+                    const bool valid = tb + wid * (WAVE * U) + u * WAVE + lane < tot;
+                    if (valid) k[u] = (unsigned)k[u] < (unsigned)ncol ? k[u] : ncol;
+                    pe[u] = digit_peers<8>(valid, valid ? k[u] >> shift : 0);
+                    if (valid && __popcll(pe[u] & lt) == 0) atomicAdd(&s_cnt[wid][k[u] >> shift], __popcll(pe[u]));
+                }
+                __syncthreads();
+                {  // thread d: the digit's start in the staging, then each wave's
+                    int dt = 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) dt += s_cnt[w][tid];
+                    int t2;
+                    int run = block_excl_scan<T>(dt, s_tmp, t2);
+                    s_goff[tid] = s_cur[tid] - run;
+                    s_cur[tid] += dt;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const int c = s_cnt[w][tid];
+                        s_cnt[w][tid] = run;
+                        run += c;
+                    }
+                }
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool valid = k[u] >= 0;
+                    const int d = valid ? k[u] >> shift : 0;
+                    const uint64_t peers = pe[u];
+                    const int rank = __popcll(peers & lt);
+                    const int pos = valid ? s_cnt[wid][d] + rank : 0;
+                    if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
+                    if (valid) {
+                        s_k[pos] = k[u];
+                        s_v[pos] = v[u];
+                    }
+                }
+                __syncthreads();
+                const int cnt = min(C, tot - tb);
+                for (int q = tid; q < cnt; q += T) {
+                    const int kk = s_k[q];
+                    const int g = s_goff[kk >> shift] + q;
+                    key_out[g] = kk;
+                    val_out[g] = s_v[q];
+                }
+                __syncthreads();  // the next step clears the counts and refills the staging
+            }
+            __syncthreads();  // the next chunk's scan rewrites s_bs / s_pre / s_a
+        }
+    }
+}
+
 // One stable LSD radix pass in LDS over items [0, cnt) of a window: digit =
 // (key >> sh) & 127, from (ik, iv) to (ok, ov).  Wave w takes items [w *
-// PER, (w + 1) * PER) in rounds of 64; per-wave digit counts, a digit-major
-// scan over (digit, wave), then ranks inside a round from ballots over the
-// digit's 7 bits.  Contains barriers; every thread of the workgroup calls it.
+// PER, (w + 1) * PER) in rounds of 64; per-wave digit counts (one LDS add
+// per digit a round holds), a digit-major scan over (digit, wave), then
+// ranks inside a round from the same ballots over the digit's 7 bits.  Contains barriers; every thread of the workgroup calls it.
 template <int T, int W>
 __device__ __forceinline__ void lds_radix_pass(const int32_t *ik, const double *iv, int32_t *ok, double *ov, int cnt,
                                                int sh, int32_t (*s_cnt)[128], int32_t *s_tot) {
@@ -845,9 +1085,13 @@ __device__ __forceinline__ void lds_radix_pass(const int32_t *ik, const double *
         k[r] = t < cnt ? ik[t] : -1;
         v[r] = t < cnt ? iv[t] : 0.0;
     }
+    uint64_t pe[RR];
 #pragma unroll
-    for (int r = 0; r < RR; ++r)
-        if (k[r] >= 0) atomicAdd(&s_cnt[wid][(k[r] >> sh) & 127], 1);
+    for (int r = 0; r < RR; ++r) {  // one add per digit a wave holds
+        const bool valid = k[r] >= 0;
+        pe[r] = digit_peers<7>(valid, valid ? (k[r] >> sh) & 127 : 0);
+        if (valid && __popcll(pe[r] & lt) == 0) atomicAdd(&s_cnt[wid][(k[r] >> sh) & 127], __popcll(pe[r]));
+    }
     __syncthreads();
     // thread d < 128: digit d's start (a DPP scan over the two waves of
     // digits), then each wave's start inside it
@@ -873,12 +1117,7 @@ __device__ __forceinline__ void lds_radix_pass(const int32_t *ik, const double *
     for (int r = 0; r < RR; ++r) {
         const bool valid = k[r] >= 0;
         const int d = valid ? (k[r] >> sh) & 127 : 0;
-        uint64_t peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 7; ++b) {
-            const uint64_t ones = __ballot((d >> b) & 1);
-            peers &= ((d >> b) & 1) ? ones : ~ones;
-        }
+        const uint64_t peers = pe[r];
         const int rank = __popcll(peers & lt);
         const int pos = valid ? s_cnt[wid][d] + rank : 0;
         if (valid && rank == 0) s_cnt[wid][d] = pos + __popcll(peers);
@@ -1277,6 +1516,16 @@ struct LongRows {
 
 // Symbolic phase of the long rows: products, the sort, runs, nnz(C(i,:)).
 // gub[r] = the products of row r of the list (their sum < 2^31).
+// Workgroups of `kernel` resident on the device at once (its register and
+// LDS use, from the runtime's occupancy query), capped by `want`: the grid of
+// a kernel whose workgroups take rows from a counter.
+template <class K>
+unsigned resident_grid(const apspie_ctx *ctx, K kernel, int threads, int64_t want) {
+    int per = 0;
+    APSPIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->cu_count) * std::max(per, 1))));
+}
+
 LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const CsrD &A,
                        const CsrD &B, int32_t *nnzc, int32_t *ovf) {
     LongRows L;
@@ -1302,35 +1551,41 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
     double *val[2] = {(double *)S(4, n * 8), (double *)S(5, n * 8)};
     APSPIE_HIP(hipMemcpyAsync(L.poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
     APSPIE_HIP(hipMemcpyAsync(L.brow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-    k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
-                          ctx->stream>>>(rows, ng, A, B, L.poff, key[0], val[0]);
-    APSPIE_HIP(hipGetLastError());
-    // ovf (device): one radix pass on the top 8 bits, then each row's digit
-    // groups sorted in LDS (k_long_lsort; a group too large for it raises
-    // *ovf); null: the full radix sort
+    // ovf (device): the expansion fused with one radix pass on the top 8
+    // bits, then each row's digit groups sorted in LDS (k_long_lsort; a group
+    // too large for it raises *ovf); null: the expansion, then the full
+    // radix sort
     const int bits = esc::bits_for(int64_t(B.n) + 1);
-    const bool msb = ovf != nullptr && bits > 8;
-    const int c = cvt::seg_radix_kv(ctx, key, val, n, po, bits, L.htiles, msb ? bits - 8 : 0);
-    if (msb) {
+    if (ovf != nullptr && bits > 8) {
         // rows by decreasing product count (a row is one workgroup's work)
         std::vector<int32_t> &ord = L.hord;
         ord.resize(size_t(ng));
         for (int r = 0; r < ng; ++r) ord[r] = r;
         std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return gub[x] > gub[y]; });
         int32_t *dord = (int32_t *)S(7, int64_t(ng) * 4 + 256);
-        unsigned *next = (unsigned *)(dord + ((ng + 63) / 64) * 64);
+        unsigned *next = (unsigned *)(dord + ((ng + 63) / 64) * 64);  // three row counters
         APSPIE_HIP(hipMemcpyAsync(dord, ord.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
-        k_zero_i32<<<1, 64, 0, ctx->stream>>>((int32_t *)next, 1);
-        // the staging: the buffers the pass left free
+        k_zero_i32<<<1, 64, 0, ctx->stream>>>((int32_t *)next, 3);
+        int32_t *hist = (int32_t *)S(6, int64_t(ng) * 256 * 4);
+        auto kh = k_long_dhist<256, 8, 4>;
+        auto ks = k_long_dscatter<256, 4, 4>;
+        kh<<<resident_grid(ctx, kh, 256, ng), 256, 0, ctx->stream>>>(ng, dord, rows, A, B, bits - 8, hist, next + 1);
+        ks<<<resident_grid(ctx, ks, 256, ng), 256, 0, ctx->stream>>>(ng, dord, rows, A, B, L.poff, bits - 8, hist,
+                                                                      key[1], val[1], next + 2);
+        // the staging: the buffers the expansion left free
         L.fused = true;
-        L.scol = key[c ^ 1];
-        L.sval = (float *)val[c ^ 1];
+        L.scol = key[0];
+        L.sval = (float *)val[0];
         k_long_lsort<256, 2048><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 3)), 256, 0,
-                                 ctx->stream>>>(ng, dord, rows, L.poff, bits - 8, bits, B.n, key[c], val[c], L.scol,
+                                 ctx->stream>>>(ng, dord, rows, L.poff, bits - 8, bits, B.n, key[1], val[1], L.scol,
                                                 L.sval, nnzc, ovf, next);
         APSPIE_HIP(hipGetLastError());
         return L;
     }
+    k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
+                          ctx->stream>>>(rows, ng, A, B, L.poff, key[0], val[0]);
+    APSPIE_HIP(hipGetLastError());
+    const int c = cvt::seg_radix_kv(ctx, key, val, n, po, bits, L.htiles, 0);
     L.key = key[c];
     L.val = val[c];
     L.flag = key[c ^ 1];  // free after the last pass
@@ -1364,16 +1619,6 @@ struct Plan {
     std::vector<int32_t> grows;
     int64_t gslots_total = 0;
 };
-
-// Workgroups of `kernel` resident on the device at once (its register and
-// LDS use, from the runtime's occupancy query), capped by `want`: the grid of
-// a kernel whose workgroups take rows from a counter.
-template <class K>
-unsigned resident_grid(const apspie_ctx *ctx, K kernel, int threads, int64_t want) {
-    int per = 0;
-    APSPIE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0));
-    return unsigned(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->cu_count) * std::max(per, 1))));
-}
 
 template <bool NUMERIC>
 void launch_bins(apspie_ctx *ctx, const Plan &P, const CsrD &A, const CsrD &B, int32_t *nnzc, const int32_t *crp,
