@@ -93,6 +93,15 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t x) {
 __device__ __forceinline__ void after_values(uint32_t dep) { asm volatile("" ::"v"(dep) : "memory"); }
 
 // Sum across the wave; every lane receives the total.
+// Makes a wave's earlier LDS writes visible to its lanes' later LDS reads:
+// a wavefront-scope release/acquire fence pair around a code-motion barrier,
+// so the compiler keeps a read of another lane's slot after the write.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);

@@ -88,6 +88,9 @@ struct apspie_ctx {
     void *slot_ptr[apspie::S_NSLOTS] = {};
     size_t slot_bytes[apspie::S_NSLOTS] = {};
     void *pinned = nullptr;  // PINNED_BYTES of pinned host staging for counters
+    void *hslot_ptr[4] = {};  // growable pinned host staging (hslot)
+    void *hslot_dev[4] = {};  // the same memory as kernels address it
+    size_t hslot_bytes[4] = {};
     std::mutex mu;
 
     // Per-kernel-class event timing (off unless enabled).
@@ -120,6 +123,14 @@ struct apspie_ctx {
     // Device buffer of at least `bytes` bytes for slot `s`.  Growing drops
     // the old contents; callers initialise what they use.
     void *slot(int s, size_t bytes);
+    // Pinned host buffer of at least `bytes` bytes for host slot `s` (< 4),
+    // so copies from or into it run asynchronously on the stream instead of
+    // waiting for the stream; growing synchronises the stream first.
+    void *hslot(int s, size_t bytes);
+    // Device address of host slot `s` (after hslot(s, ...)): kernels write
+    // small results there directly, visible on the host once the stream is
+    // synchronised, with no copy engine in between.
+    void *hslot_device(int s) const { return hslot_dev[s]; }
 
     // Bracket one launch of kernel class `cls` (no-ops unless profiling).
     void prof_begin(int cls);

@@ -1,4 +1,5 @@
 // runtime.cpp -- context lifetime, scratch pool and memory helpers.
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -31,6 +32,32 @@ void *apspie_ctx::slot(int s, size_t bytes) {
     }
     slot_ptr[s] = p;
     slot_bytes[s] = want;
+    return p;
+}
+
+void *apspie_ctx::hslot(int s, size_t bytes) {
+    if (hslot_bytes[s] >= bytes && hslot_ptr[s]) return hslot_ptr[s];
+    APSPIE_HIP(hipStreamSynchronize(stream));  // copies in flight may use the old buffer
+    if (hslot_ptr[s]) {
+        APSPIE_HIP(hipHostFree(hslot_ptr[s]));
+        hslot_ptr[s] = hslot_dev[s] = nullptr;
+        hslot_bytes[s] = 0;
+    }
+    const size_t want = (std::max<size_t>(bytes, 1) + (1u << 16) - 1) & ~size_t((1u << 16) - 1);
+    void *p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(APSPIE_ENOMEM, "pinned host allocation of " + std::to_string(want) + " bytes failed");
+    }
+    void *d = nullptr;
+    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipHostFree(p);
+        throw Error(APSPIE_EHIP, "no device address for pinned host memory");
+    }
+    hslot_ptr[s] = p;
+    hslot_dev[s] = d;
+    hslot_bytes[s] = want;
     return p;
 }
 
@@ -122,6 +149,8 @@ int apspie_ctx_destroy(apspie_ctx *ctx) {
     for (int s = 0; s < S_NSLOTS; ++s)
         if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    for (auto p : ctx->hslot_ptr)
+        if (p) (void)hipHostFree(p);
     for (auto &o : ctx->prof_open) {
         (void)hipEventDestroy(o.second.first);
         (void)hipEventDestroy(o.second.second);

@@ -4,7 +4,7 @@
 //
 // Pipeline (one context stream):
 //   1. ub[i]   = sum over k in A(i,:) of nnz(B(k,:))  (products of row i),
-//                an int64 scan of every A entry's B row length.
+//                a wave per 64 rows summing its entries' B row lengths.
 //   2. bins    rows by ub: <= 16 (four rows a wave) and <= 64 (a wave per
 //              row), both in registers; <= 128, <= 256 (LDS hash tables of
 //              512 and 1024 slots); <= 512, <= 1024, <= 2048 (the row's
@@ -68,25 +68,78 @@ constexpr int NBINS = 8;
 constexpr int BIN_UB[NBINS - 1] = {16, 64, 128, 256, 512, 1024, 2048};
 constexpr int LONG = NBINS - 1;  // the bin of rows beyond the LDS tables
 
-// Product bounds ub[i] = sum over A(i,:) of nnz(B(k,:)): every entry's B
-// row length (0 for a column outside B), an exclusive scan in int64, and one
-// difference per row, so no atomics.
-__global__ __launch_bounds__(256) void k_entry_len(int64_t nnz, const int32_t *__restrict__ col,
-                                                   const int32_t *__restrict__ brp, int bm, int32_t *len) {
-    for (int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x; e < nnz; e += int64_t(gridDim.x) * 256) {
-        const int k = col[e];
-        len[e] = (unsigned)k < (unsigned)bm ? brp[k + 1] - brp[k] : 0;
+// Product bounds ub[i] = sum over A(i,:) of nnz(B(k,:)) (0 for a column
+// outside B).  A wave takes 64 consecutive rows: their entries are
+// contiguous, so its lanes stride over them, find each entry's row by a
+// binary search over the rows' pointers in LDS (an entry's row only grows)
+// and add the entry's B row length into the row's 64-bit LDS sum; four
+// entries a lane have their loads in flight together.  The waves
+// share no LDS, so they synchronise by wavefront-scope fences only.  *bad =
+// 1 for a row of 2^31 - 1 or more products.
+__global__ __launch_bounds__(256) void k_row_ub(int m, const int32_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                const int32_t *__restrict__ brp, int bm, int32_t *ub, int32_t *bad) {
+    constexpr int NW = 256 / WAVE;
+    __shared__ int32_t s_rp[NW][WAVE];
+    __shared__ unsigned long long s_acc[NW][WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    int32_t *srp = s_rp[wid];
+    unsigned long long *acc = s_acc[wid];
+    for (int64_t r0 = (int64_t(blockIdx.x) * NW + wid) * WAVE; r0 < m; r0 += int64_t(gridDim.x) * NW * WAVE) {
+        const int nr = int(std::min<int64_t>(WAVE, m - r0));
+        const int e0 = rp[r0], e1 = rp[r0 + nr];
+        wave_lds_sync();  // the previous rows' reads are done
+        srp[lane] = rp[r0 + std::min(lane, nr)];
+        acc[lane] = 0;
+        wave_lds_sync();
+        constexpr int U = 16;  // entries per lane in flight
+        int row = 0;          // the last row x with srp[x] <= e
+        for (int eb = e0 + lane; eb < e1; eb += U * WAVE) {
+            int k[U], b0[U], b1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = eb + u * WAVE;
+                k[u] = e < e1 ? col[e] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                b0[u] = b1[u] = 0;
+                if ((unsigned)k[u] < (unsigned)bm) {
+                    b0[u] = brp[k[u]];
+                    b1[u] = brp[k[u] + 1];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int e = eb + u * WAVE;
+                if (e >= e1 || b1[u] == b0[u]) continue;
+                int top = nr - 1;
+                while (row < top) {
+                    const int mid = (row + top + 1) >> 1;
+                    if (srp[mid] <= e) row = mid;
+                    else top = mid - 1;
+                }
+                atomicAdd(&acc[row], (unsigned long long)(b1[u] - b0[u]));
+            }
+        }
+        wave_lds_sync();
+        if (lane < nr) {
+            const unsigned long long u = acc[lane];
+            if (u >= (1ull << 31) - 1) *bad = 1;
+            ub[r0 + lane] = int32_t(std::min<unsigned long long>(u, (1ull << 31) - 2));
+        }
     }
 }
 
-// ub[i] from the scanned lengths; *bad = 1 for a row of 2^31 or more products
-__global__ __launch_bounds__(256) void k_row_ub(int m, const int32_t *__restrict__ rp,
-                                                const int64_t *__restrict__ psum, int32_t *ub, int32_t *bad) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < m; i += gridDim.x * 256) {
-        const int64_t u = psum[rp[i + 1]] - psum[rp[i]];
-        if (u >= (int64_t(1) << 31) - 1) *bad = 1;
-        ub[i] = int32_t(std::min<int64_t>(u, (int64_t(1) << 31) - 2));
-    }
+// dst[0, n) = src[0, n): small results written straight into pinned host
+// memory (its device address), read by the host after a stream
+// synchronisation.
+__global__ __launch_bounds__(256) void k_to_host(const int32_t *__restrict__ src, int n, int32_t *dst) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) dst[i] = src[i];
+}
+
+inline void to_host(apspie_ctx *ctx, const int32_t *src, int n, int hs, size_t byte_off) {
+    int32_t *dst = (int32_t *)((char *)ctx->hslot_device(hs) + byte_off);
+    k_to_host<<<std::max(1u, std::min(ceil_div(std::max(n, 1), 256), 1024u)), 256, 0, ctx->stream>>>(src, n, dst);
 }
 
 __global__ __launch_bounds__(256) void k_zero_i32(int32_t *p, int64_t n) {
@@ -838,6 +891,30 @@ __global__ __launch_bounds__(T) void k_long_expand(const int32_t *__restrict__ r
     }
 }
 
+// order[0, ng) = the long rows by decreasing bit length of their product
+// count (rows of one length in any order), one workgroup; also clears the
+// three row counters of the windowed path.
+__global__ __launch_bounds__(1024) void k_long_order(int ng, const int32_t *__restrict__ gub, int32_t *order,
+                                                     unsigned *next) {
+    __shared__ int32_t s_cnt[33];
+    const int tid = threadIdx.x;
+    if (tid < 33) s_cnt[tid] = 0;
+    if (tid < 3) next[tid] = 0;
+    __syncthreads();
+    for (int r = tid; r < ng; r += 1024) atomicAdd(&s_cnt[32 - __clz(gub[r])], 1);
+    __syncthreads();
+    if (tid == 0) {
+        int run = 0;
+        for (int b = 32; b >= 0; --b) {
+            const int c = s_cnt[b];
+            s_cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int r = tid; r < ng; r += 1024) order[atomicAdd(&s_cnt[32 - __clz(gub[r])], 1)] = r;
+}
+
 // A chunk of up to T * EPT entries of A(i,:) from jb (thread t takes the EPT
 // consecutive entries from jb + t * EPT, their loads issued together): each
 // entry's B row start, its products' prefix within the chunk and its A
@@ -1388,12 +1465,29 @@ __global__ __launch_bounds__(256) void k_long_copy(int nrows, const int32_t *__r
                                                    const int32_t *__restrict__ poff, const int32_t *__restrict__ crp,
                                                    const int32_t *__restrict__ scol, const float *__restrict__ sval,
                                                    int32_t *ccol, float *cval) {
+    constexpr int U = 4;  // loads in flight per thread
     for (int r = blockIdx.x; r < nrows; r += gridDim.x) {
         const int i = rows[r];
         const int c0 = crp[i], nz = crp[i + 1] - c0, s0 = poff[r];
-        for (int x = threadIdx.x; x < nz; x += 256) {
-            ccol[c0 + x] = scol[s0 + x];
-            cval[c0 + x] = sval[s0 + x];
+        for (int xb = threadIdx.x; xb < nz; xb += U * 256) {
+            int32_t c[U];
+            float v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int x = xb + u * 256;
+                if (x < nz) {
+                    c[u] = scol[s0 + x];
+                    v[u] = sval[s0 + x];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int x = xb + u * 256;
+                if (x < nz) {
+                    ccol[c0 + x] = c[u];
+                    cval[c0 + x] = v[u];
+                }
+            }
         }
     }
 }
@@ -1511,7 +1605,7 @@ struct LongRows {
     int32_t *scol = nullptr;
     float *sval = nullptr;
     // host sources of the uploaded tables, kept until the stream syncs
-    std::vector<int32_t> hpo, hbrow, htiles, hord;
+    std::vector<int32_t> hpo, hbrow, htiles;
 };
 
 // Symbolic phase of the long rows: products, the sort, runs, nnz(C(i,:)).
@@ -1526,46 +1620,32 @@ unsigned resident_grid(const apspie_ctx *ctx, K kernel, int threads, int64_t wan
     return unsigned(std::max<int64_t>(1, std::min<int64_t>(want, int64_t(ctx->cu_count) * std::max(per, 1))));
 }
 
-LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const CsrD &A,
-                       const CsrD &B, int32_t *nnzc, int32_t *ovf) {
+LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<int32_t> &gub, const int32_t *dgub,
+                       const CsrD &A, const CsrD &B, int32_t *nnzc, int32_t *ovf) {
     LongRows L;
     const int ng = int(gub.size());
-    std::vector<int32_t> &po = L.hpo;
-    po.assign(size_t(ng) + 1, 0);
-    for (int r = 0; r < ng; ++r) po[r + 1] = po[r] + gub[r];
-    const int64_t n = po[ng];
-    // brow[b] = the row holding product b * 2048
-    std::vector<int32_t> &brow = L.hbrow;
-    brow.resize(size_t(n >> 11) + 1);
-    for (size_t b = 0, r = 0; b < brow.size(); ++b) {
-        while (r + 1 < size_t(ng) && po[r + 1] <= (int64_t(b) << 11)) ++r;
-        brow[b] = int32_t(r);
-    }
+    int64_t n = 0;
+    for (int r = 0; r < ng; ++r) n += gub[r];
     L.nrows = ng;
     L.n = n;
     L.rows = rows;
     auto S = [&](int k, int64_t bytes) { return ctx->slot(S_OUTER0 + k, size_t(std::max<int64_t>(bytes, 16))); };
     L.poff = (int32_t *)S(0, (int64_t(ng) + 1) * 4);
-    L.brow = (int32_t *)S(1, int64_t(brow.size()) * 4);
     int32_t *key[2] = {(int32_t *)S(2, n * 4), (int32_t *)S(3, n * 4)};
     double *val[2] = {(double *)S(4, n * 8), (double *)S(5, n * 8)};
-    APSPIE_HIP(hipMemcpyAsync(L.poff, po.data(), (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
-    APSPIE_HIP(hipMemcpyAsync(L.brow, brow.data(), brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     // ovf (device): the expansion fused with one radix pass on the top 8
     // bits, then each row's digit groups sorted in LDS (k_long_lsort; a group
     // too large for it raises *ovf); null: the expansion, then the full
     // radix sort
     const int bits = esc::bits_for(int64_t(B.n) + 1);
     if (ovf != nullptr && bits > 8) {
-        // rows by decreasing product count (a row is one workgroup's work)
-        std::vector<int32_t> &ord = L.hord;
-        ord.resize(size_t(ng));
-        for (int r = 0; r < ng; ++r) ord[r] = r;
-        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return gub[x] > gub[y]; });
+        // the rows' product offsets and their order by decreasing product
+        // count (a row is one workgroup's work) made on the device, so
+        // nothing waits for a copy from the host
+        cvt::scan32(ctx, dgub, L.poff, ng);
         int32_t *dord = (int32_t *)S(7, int64_t(ng) * 4 + 256);
         unsigned *next = (unsigned *)(dord + ((ng + 63) / 64) * 64);  // three row counters
-        APSPIE_HIP(hipMemcpyAsync(dord, ord.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
-        k_zero_i32<<<1, 64, 0, ctx->stream>>>((int32_t *)next, 3);
+        k_long_order<<<1, 1024, 0, ctx->stream>>>(ng, dgub, dord, next);
         int32_t *hist = (int32_t *)S(6, int64_t(ng) * 256 * 4);
         auto kh = k_long_dhist<256, 8, 4>;
         auto ks = k_long_dscatter<256, 4, 4>;
@@ -1582,6 +1662,23 @@ LongRows long_symbolic(apspie_ctx *ctx, const int32_t *rows, const std::vector<i
         APSPIE_HIP(hipGetLastError());
         return L;
     }
+    // the row offsets and brow[b] = the row holding product b * 2048 from the
+    // host (pinned staging)
+    std::vector<int32_t> &po = L.hpo;
+    po.assign(size_t(ng) + 1, 0);
+    for (int r = 0; r < ng; ++r) po[r + 1] = po[r] + gub[r];
+    std::vector<int32_t> &brow = L.hbrow;
+    brow.resize(size_t(n >> 11) + 1);
+    for (size_t b = 0, r = 0; b < brow.size(); ++b) {
+        while (r + 1 < size_t(ng) && po[r + 1] <= (int64_t(b) << 11)) ++r;
+        brow[b] = int32_t(r);
+    }
+    L.brow = (int32_t *)S(1, int64_t(brow.size()) * 4);
+    int32_t *hs = (int32_t *)ctx->hslot(2, (size_t(ng) + 1 + brow.size()) * 4);
+    memcpy(hs, po.data(), (size_t(ng) + 1) * 4);
+    memcpy(hs + ng + 1, brow.data(), brow.size() * 4);
+    APSPIE_HIP(hipMemcpyAsync(L.poff, hs, (size_t(ng) + 1) * 4, hipMemcpyHostToDevice, ctx->stream));
+    APSPIE_HIP(hipMemcpyAsync(L.brow, hs + ng + 1, brow.size() * 4, hipMemcpyHostToDevice, ctx->stream));
     k_long_expand<1024><<<std::max(1u, std::min((unsigned)ng, unsigned(ctx->cu_count) * 2)), 1024, 0,
                           ctx->stream>>>(rows, ng, A, B, L.poff, key[0], val[0]);
     APSPIE_HIP(hipGetLastError());
@@ -1688,9 +1785,10 @@ int64_t rowptr_scan(apspie_ctx *ctx, const int32_t *counts, int n, int32_t *row_
         (int32_t *)st, int64_t(nb) * 2 + 4);
     k_scan<<<nb, 256, 0, ctx->stream>>>(counts, row_ptr, n, st, ctr, tot);
     APSPIE_HIP(hipGetLastError());
-    unsigned long long total = 0;
-    APSPIE_HIP(hipMemcpyAsync(&total, tot, 8, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long *htot = (unsigned long long *)((char *)ctx->hslot(0, 512) + 256);
+    to_host(ctx, (const int32_t *)tot, 2, 0, 256);
     APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    const unsigned long long total = *htot;
     if (total >= (1ull << 31) - 4096) {
         on_fail();
         throw Error(APSPIE_ELIMIT, "the result would hold " + std::to_string(total) +
@@ -1737,30 +1835,36 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
         int32_t *counts = (int32_t *)ctx->slot(S_AUX3, 256);
         unsigned long long *products = (unsigned long long *)(counts + 16);
         const unsigned g256 = std::max(1u, std::min(ceil_div(std::max(m, 1), 256), 4096u));
-        int32_t *bad = counts + 10;
+        int32_t *bad = counts + 10;  // counts[0, NBINS), bad, products: 128 bytes
         k_zero_i32<<<1, 256, 0, ctx->stream>>>(counts, 64);
-        {
-            int32_t *len = (int32_t *)ctx->slot(S_OUTER0 + 14, size_t(std::max<int64_t>(a->nnz, 1)) * 4);
-            int64_t *psum = (int64_t *)ctx->slot(S_OUTER0 + 15, (size_t(a->nnz) + 1) * 8);
-            k_entry_len<<<cvt::grid_for(ctx, a->nnz, 256), 256, 0, ctx->stream>>>(a->nnz, a->col_ind, b->row_ptr,
-                                                                                  b->m, len);
-            cvt::scan_wide(ctx, len, psum, a->nnz);
-            k_row_ub<<<g256, 256, 0, ctx->stream>>>(m, a->row_ptr, psum, ub, bad);
-        }
+        k_row_ub<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 256), (unsigned)ctx->cu_count * 16)), 256, 0,
+                   ctx->stream>>>(m, a->row_ptr, a->col_ind, b->row_ptr, b->m, ub, bad);
         // 2. bins
         k_bin<<<std::max(1u, std::min(ceil_div(std::max(m, 1), 4096), (unsigned)ctx->cu_count * 2)), 256, 0,
                 ctx->stream>>>(m, ub, nnzc, lists, counts, products);
         APSPIE_HIP(hipGetLastError());
+        // the counts, the bad flag and the product total in one copy into
+        // pinned memory; C's row pointers allocated while the kernels run
+        int32_t *hc = (int32_t *)ctx->hslot(0, 512);
+        to_host(ctx, counts, 32, 0, 0);
+        struct FreeOnThrow {  // C's arrays go if anything below throws
+            apspie_ctx *ctx;
+            apspie_csr_owned *c;
+            bool keep = false;
+            ~FreeOnThrow() {
+                if (!keep) apspie_csr_free(ctx, c);
+            }
+        } out_guard{ctx, c};
+        APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
+        c->m = m;
+        c->n = b->n;
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
         Plan P;
         P.lists = lists;
-        unsigned long long prod_total = 0;
         int hcounts[NBINS];
-        APSPIE_HIP(hipMemcpyAsync(hcounts, counts, sizeof(hcounts), hipMemcpyDeviceToHost, ctx->stream));
-        APSPIE_HIP(hipMemcpyAsync(&prod_total, products, 8, hipMemcpyDeviceToHost, ctx->stream));
-        int32_t hbad = 0;
-        APSPIE_HIP(hipMemcpyAsync(&hbad, bad, 4, hipMemcpyDeviceToHost, ctx->stream));
-        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-        APSPIE_CHECK(hbad == 0, APSPIE_ELIMIT, "a row of C has 2^31 or more products");
+        memcpy(hcounts, hc, sizeof(hcounts));
+        const unsigned long long prod_total = *(const unsigned long long *)(hc + 16);
+        APSPIE_CHECK(hc[10] == 0, APSPIE_ELIMIT, "a row of C has 2^31 or more products");
         for (int k = 0; k < NBINS; ++k) P.counts[k] = hcounts[k];
         // rows beyond lub products leave the LDS tables for the long rows'
         // path: the bins above spgemm_esc_min join the long list (each bin's
@@ -1789,9 +1893,10 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             gslots = (int32_t *)ctx->slot(S_AUX5, size_t(ng) * 4);
             k_gather<<<std::max(1u, std::min(ceil_div(ng, 256), 1024u)), 256, 0, ctx->stream>>>(
                 lists + LONG * int64_t(m), ng, ub, gslots);
-            gub.resize(ng);
-            APSPIE_HIP(hipMemcpyAsync(gub.data(), gslots, size_t(ng) * 4, hipMemcpyDeviceToHost, ctx->stream));
+            int32_t *hg = (int32_t *)ctx->hslot(1, size_t(ng) * 4);
+            to_host(ctx, gslots, ng, 1, 0);
             APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+            gub.assign(hg, hg + ng);
             for (int r = 0; r < ng; ++r) gtot += gub[r];
             // expand-sort-compress while the products index with int32
             esc_rows = ctx->tune.spgemm_long_rows == 1 && gtot < (int64_t(1) << 31) - 1;
@@ -1812,8 +1917,12 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             goff = (int64_t *)ctx->slot(S_AUX4, size_t(ng) * 8);
             gkeys = (int32_t *)ctx->slot(S_AUX6, size_t(tot) * 4);
             gvals = (double *)ctx->slot(S_AUX7, size_t(tot) * 8);
-            APSPIE_HIP(hipMemcpyAsync(goff, off.data(), size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
-            APSPIE_HIP(hipMemcpyAsync(gslots, sl.data(), size_t(ng) * 4, hipMemcpyHostToDevice, ctx->stream));
+            char *hs = (char *)ctx->hslot(3, size_t(ng) * 12);
+            memcpy(hs, off.data(), size_t(ng) * 8);
+            memcpy(hs + size_t(ng) * 8, sl.data(), size_t(ng) * 4);
+            APSPIE_HIP(hipMemcpyAsync(goff, hs, size_t(ng) * 8, hipMemcpyHostToDevice, ctx->stream));
+            APSPIE_HIP(hipMemcpyAsync(gslots, hs + size_t(ng) * 8, size_t(ng) * 4, hipMemcpyHostToDevice,
+                                      ctx->stream));
         }
         // 3. the rows of the LDS tables in one pass while their products fit
         // int32 offsets: each sorted row goes to a staging area at the prefix
@@ -1840,31 +1949,29 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             launch_bins<false>(ctx, P03, A, B, nnzc, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
         }
         LongRows lr;
-        int32_t *ovf = nullptr, hovf = 0;
+        int32_t *ovf = nullptr;
+        int32_t *hovf = (int32_t *)((char *)ctx->hslot(0, 512) + 264);
+        *hovf = 0;
         if (esc_rows) {
             ovf = (int32_t *)ctx->slot(S_OUTER0 + 8, 256);
             k_zero_i32<<<1, 256, 0, ctx->stream>>>(ovf, 1);
-            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc, ovf);
-            APSPIE_HIP(hipMemcpyAsync(&hovf, ovf, 4, hipMemcpyDeviceToHost, ctx->stream));
+            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, gslots, A, B, nnzc, ovf);
+            to_host(ctx, ovf, 1, 0, 264);
         } else if (P.counts[LONG]) {
             launch_bins<false>(ctx, P4, A, B, nnzc, nullptr, nullptr, nullptr, gkeys, gvals, goff, gslots);
         }
         // 4. row pointers (the scan's synchronisation also brings the long
         // rows' overflow flag; on overflow they are sorted again by the full
         // radix passes and the scan repeated)
-        APSPIE_HIP(hipMalloc(&c->row_ptr, (size_t(m) + 1) * 4));
-        c->m = m;
-        c->n = b->n;
         int32_t total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
-        if (hovf) {
-            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, A, B, nnzc, nullptr);
+        if (*hovf) {
+            lr = long_symbolic(ctx, lists + LONG * int64_t(m), gub, gslots, A, B, nnzc, nullptr);
             total = int32_t(rowptr_scan(ctx, nnzc, m, c->row_ptr, [&] { apspie_csr_free(ctx, c); }));
         }
         c->nnz = total;
         if (hipMalloc(&c->col_ind, size_t(std::max(total, 1)) * 4) != hipSuccess ||
             hipMalloc(&c->val, size_t(std::max(total, 1)) * 4) != hipSuccess) {
             (void)hipGetLastError();
-            apspie_csr_free(ctx, c);
             throw Error(APSPIE_ENOMEM, "output allocation failed");
         }
         // 5. numeric
@@ -1892,5 +1999,6 @@ extern "C" int apspie_spgemm(apspie_ctx *ctx, const apspie_csr *a, const apspie_
             stats->bytes_alg = 8.0 * (double(m) + 1) + 16.0 * double(a->nnz) + 8.0 * double(prod_total) +
                                8.0 * double(total);
         }
+        out_guard.keep = true;
     });
 }
