@@ -410,6 +410,7 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
     C->nnew[0][0].v = 1;
     C->acc.vertices[0].v = 1;
     C->acc.levels = 1;
+    C->acc.binned = 0;
     C->acc.edges = (unsigned long long)deg;
     Ctl c;
     c.nF = deg > 0 ? 1 : 0;
@@ -417,7 +418,9 @@ __global__ __launch_bounds__(WG) void k_seed(Pick pk, int seed, const int32_t *_
     c.nTiles = ntiles;
     c.pull = 0;
     if constexpr (has_direction<Pick>::value) c.pull = pk.direction(c, (unsigned long long)deg, C->ctl[1].pull);
-    if (c.pull == DIR_BINNED) C->acc.binned = 1;
+    // the source's round runs push: the host then launches none of the
+    // binned round's kernels for it
+    if (c.pull == DIR_BINNED) c.pull = DIR_PUSH;
     C->ctl[0] = c;
     C->log[0] = c;
 }
