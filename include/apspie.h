@@ -150,14 +150,15 @@ typedef struct {
     int32_t sssp_delta;       /* SSSP near/far step: a changed vertex waits while its distance is at or
                                  past a threshold that rises by this much when the frontier below it
                                  runs dry; 0 = off (plain frontier Bellman-Ford), 0..2^24 */
-    int32_t spgemm_long_rows; /* SpGEMM rows beyond the LDS tables (over 2048 products): 1 = expand-
+    int32_t spgemm_long_rows; /* SpGEMM rows beyond the LDS bins (over 2048 products): 1 = expand-
                                  sort-compress (default; global hash tables when their products reach
                                  2^31), 0 = global hash tables */
-    int32_t spgemm_passes;    /* SpGEMM rows of the LDS tables: 1 = one pass into a staging area
-                                 (default; two when their products reach 2^31), 2 = a counting pass,
-                                 then the product pass */
+    int32_t spgemm_passes;    /* SpGEMM rows of the LDS bins (hash tables up to 256 products, LDS
+                                 sorts up to 2048): 1 = one pass into a staging area (default; two
+                                 when their products reach 2^31), 2 = a counting pass, then the
+                                 product pass */
     int32_t spgemm_esc_min;   /* SpGEMM with spgemm_long_rows = 1: rows with more products than this go
-                                 to expand-sort-compress instead of the LDS tables; 256, 512, 1024
+                                 to expand-sort-compress instead of the LDS bins; 256, 512, 1024
                                  (default) or 2048 */
     int32_t bfs_group_cap;    /* BFS with a CSC: entries one round may append to a queue group region
                                  (16 regions); a round whose new frontier does not fit, and whose
