@@ -755,6 +755,7 @@ class ServerReport(C.Structure):
         ("max_ms", C.c_double),
         ("qps", C.c_double),
         ("trivial", C.c_int64),
+        ("joined", C.c_int64),
     ]
 
     def as_dict(self) -> dict:

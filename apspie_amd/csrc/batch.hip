@@ -26,6 +26,7 @@
 // union frontier is empty.
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -266,12 +267,14 @@ __global__ __launch_bounds__(256) void k_mss_init(int m, int k, const int32_t *s
     control_clear(C, gtid, stride, ~0u);
 }
 
+// sources srcs[0, k) in lanes lane0 .. lane0 + k - 1
 template <int K>
-__global__ void k_mss_seed(int k, const int32_t *srcs, uint32_t *dist, uint64_t *chg) {
+__global__ void k_mss_seed(int k, int lane0, const int32_t *srcs, uint32_t *dist, uint64_t *chg) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < k) {
-        dist[int64_t(srcs[s]) * K + s] = 0u;
-        atomicOr((unsigned long long *)&chg[srcs[s]], 1ull << s);
+        const int lane = lane0 + s;
+        dist[int64_t(srcs[s]) * K + lane] = 0u;
+        atomicOr((unsigned long long *)&chg[srcs[s]], 1ull << lane);
     }
 }
 
@@ -294,44 +297,53 @@ __global__ __launch_bounds__(256) void k_mss_transpose(int m, int k, const uint3
 // ---------------------------------------------------------------------------
 // driver: device-driven rounds captured in chunks into a hipGraph
 
-// Replays a captured chunk of CH rounds until the union frontier is empty.
-// The instantiated graph is cached in the context under `key`, which names
-// every address and size its kernel arguments hold (slot growth drops the
-// cache), so repeated calls on one graph skip capture and instantiation.
+// The captured chunk of batch_chunk rounds.  The instantiated graph is
+// cached in the context under `key`, which names every address and size its
+// kernel arguments hold (slot growth drops the cache), so repeated calls on
+// one graph skip capture and instantiation.
 template <class F>
-int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &base_key, int max_rounds, F &&enqueue_round) {
+hipGraphExec_t chunk_exec(apspie_ctx *ctx, const Bufs &b, const std::string &base_key, F &&enqueue_round) {
     const int CH = ctx->tune.batch_chunk;
     // the chunk length and the launch shape are part of what was captured
     const std::string key = base_key + "|ch" + std::to_string(CH) + "|eg" + std::to_string(b.expand_grid);
-    hipGraphExec_t exec = nullptr;
     auto it = ctx->graphs.find(key);
-    if (it != ctx->graphs.end()) {
-        exec = it->second;
-    } else {
-        if (ctx->graphs.size() >= 64) ctx->drop_graphs();  // bound the cache
-        hipGraph_t graph = nullptr;
-        APSPIE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < CH; ++i) enqueue_round();
-        APSPIE_HIP(hipStreamEndCapture(ctx->stream, &graph));
-        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        APSPIE_HIP(ie);
-        ctx->graphs[key] = exec;
-    }
+    if (it != ctx->graphs.end()) return it->second;
+    if (ctx->graphs.size() >= 64) ctx->drop_graphs();  // bound the cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    APSPIE_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < CH; ++i) enqueue_round();
+    APSPIE_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    APSPIE_HIP(ie);
+    ctx->graphs[key] = exec;
+    return exec;
+}
+
+// One replay of the chunk; false once the union frontier is empty (the
+// frontier written by the last compaction that ran: an exhausted traversal
+// stops advancing cur_level, see k_compact).
+bool chunk_step(apspie_ctx *ctx, const Bufs &b, hipGraphExec_t exec, int &rounds, int max_rounds) {
+    APSPIE_HIP(hipGraphLaunch(exec, ctx->stream));
+    rounds += ctx->tune.batch_chunk;
     Control *hc = (Control *)ctx->pinned;
+    // the descriptors and the round counter lead the control block
+    static_assert(offsetof(Control, cur_level) + sizeof(int) <= 4096, "fits the pinned staging");
+    APSPIE_HIP(hipMemcpyAsync(hc, b.C, offsetof(Control, cur_level) + sizeof(int), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    if (hc->ctl[(hc->cur_level - 1) & 1].nF == 0) return false;
+    APSPIE_CHECK(rounds <= max_rounds, APSPIE_ENOCONV, "batched traversal did not converge");
+    return true;
+}
+
+// Replays the chunk until the union frontier is empty.
+template <class F>
+int run_rounds(apspie_ctx *ctx, const Bufs &b, const std::string &base_key, int max_rounds, F &&enqueue_round) {
+    const hipGraphExec_t exec = chunk_exec(ctx, b, base_key, enqueue_round);
     int rounds = 0;
-    for (;;) {
-        APSPIE_HIP(hipGraphLaunch(exec, ctx->stream));
-        rounds += CH;
-        // the descriptors and the round counter lead the control block
-        static_assert(offsetof(Control, cur_level) + sizeof(int) <= 4096, "fits the pinned staging");
-        APSPIE_HIP(hipMemcpyAsync(hc, b.C, offsetof(Control, cur_level) + sizeof(int), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
-        // frontier written by the last compaction that ran (an exhausted
-        // traversal stops advancing cur_level, see k_compact)
-        if (hc->ctl[(hc->cur_level - 1) & 1].nF == 0) break;
-        APSPIE_CHECK(rounds <= max_rounds, APSPIE_ENOCONV, "batched traversal did not converge");
+    while (chunk_step(ctx, b, exec, rounds, max_rounds)) {
     }
     return rounds;
 }
@@ -381,36 +393,108 @@ void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k,
     if (stats) stats->bytes_alg = 0.0;
 }
 
+}  // namespace
+
+// Batched SSSP in K lanes that more sources may join between chunks of
+// rounds: sources seeded in lanes lane0 .. lane0 + k - 1 start from distance
+// 0 and are picked by the next compaction; the rounds relax live distances
+// until nothing changes, so a source's distances are its shortest paths
+// whichever round it joins in.
+struct SsspJoin {
+    apspie_ctx *ctx = nullptr;
+    const apspie_csr *g = nullptr;
+    int K = 0, lanes = 0, rounds = 0;
+    Bufs b;
+    uint32_t *dist = nullptr;
+    uint64_t *chg = nullptr, *fr = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
 template <int K>
-void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, float *out,
-                apspie_stats *stats) {
-    const Bufs b = get_bufs(ctx, g->m, g->nnz);
-    uint32_t *dist = (uint32_t *)ctx->slot(S_AUX0, size_t(g->m) * K * 4);
-    uint64_t *chg = (uint64_t *)ctx->slot(S_AUX1, size_t(g->m) * 8);
-    uint64_t *fr = (uint64_t *)ctx->slot(S_AUX2, size_t(g->m) * 8);
+void join_begin(SsspJoin &j, const int32_t *srcs, int k) {
+    apspie_ctx *ctx = j.ctx;
+    const apspie_csr *g = j.g;
+    j.b = get_bufs(ctx, g->m, g->nnz);
+    const Bufs &b = j.b;
+    j.dist = (uint32_t *)ctx->slot(S_AUX0, size_t(g->m) * K * 4);
+    j.chg = (uint64_t *)ctx->slot(S_AUX1, size_t(g->m) * 8);
+    j.fr = (uint64_t *)ctx->slot(S_AUX2, size_t(g->m) * 8);
     APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * K, 256), 8192u));
-    k_mss_init<K><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, dist, chg, b.nblk, b.status, b.C);
-    k_mss_seed<K><<<1, 64, 0, ctx->stream>>>(k, srcs, dist, chg);
-    const PickChanged pk{chg, fr, g->m};
-    k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
-                                                             b.C);
+    k_mss_init<K><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, j.dist, j.chg, b.nblk, b.status, b.C);
+    k_mss_seed<K><<<1, 64, 0, ctx->stream>>>(k, 0, srcs, j.dist, j.chg);
+    const PickChanged pk{j.chg, j.fr, g->m};
+    k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b), b.C);
     APSPIE_HIP(hipGetLastError());
-    const std::string key = graph_key("mssssp", g, b, {dist, chg, fr, b.status, b.tiles}, K);
-    run_rounds(ctx, b, key, g->m * 2 + 4, [&] {
+    const std::string key = graph_key("mssssp", g, b, {j.dist, j.chg, j.fr, b.status, b.tiles}, K);
+    uint32_t *dist = j.dist;
+    uint64_t *chg = j.chg, *fr = j.fr;
+    j.exec = chunk_exec(ctx, b, key, [&] {
         k_expand_ms_sssp<K><<<b.expand_grid, WG, 0, ctx->stream>>>(g->m, b.C, queue_of(b), g->col_ind, g->val,
                                                                     dist, fr, chg);
         k_compact<PickChanged><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
                                                                  b.C);
     });
-    k_mss_transpose<K><<<ceil_div(g->m, 64), 256, 0, ctx->stream>>>(g->m, k, dist, (uint32_t *)out);
+    j.lanes = k;
+}
+
+template <int K>
+void join_end(SsspJoin &j, float *out, apspie_stats *stats) {
+    apspie_ctx *ctx = j.ctx;
+    k_mss_transpose<K><<<ceil_div(j.g->m, 64), 256, 0, ctx->stream>>>(j.g->m, j.lanes, j.dist, (uint32_t *)out);
     APSPIE_HIP(hipGetLastError());
     APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    fill_stats(ctx, b, MODE_SSSP, g->m, stats);
+    fill_stats(ctx, j.b, MODE_SSSP, j.g->m, stats);
     if (stats) {
         stats->improved = 0;
         stats->bytes_alg = 0.0;
     }
+}
+
+}  // namespace
+
+SsspJoin *sssp_join_begin(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k, int K) {
+    APSPIE_CHECK(K == 16 || K == 64, APSPIE_EINVAL, "a joinable batch has 16 or 64 lanes");
+    APSPIE_CHECK(k >= 1 && k <= K, APSPIE_ELIMIT, "sources beyond the batch's lanes");
+    std::unique_ptr<SsspJoin> j(new SsspJoin);
+    j->ctx = ctx;
+    j->g = g;
+    j->K = K;
+    if (K == 16) join_begin<16>(*j, d_srcs, k);
+    else join_begin<64>(*j, d_srcs, k);
+    return j.release();
+}
+
+bool sssp_join_step(SsspJoin *j) { return chunk_step(j->ctx, j->b, j->exec, j->rounds, j->g->m * 2 + 4); }
+
+int sssp_join_lanes(const SsspJoin *j) { return j->lanes; }
+
+void sssp_join_add(SsspJoin *j, const int32_t *d_srcs, int k) {
+    APSPIE_CHECK(k >= 1 && j->lanes + k <= j->K, APSPIE_ELIMIT, "sources beyond the batch's lanes");
+    if (j->K == 16) k_mss_seed<16><<<1, 64, 0, j->ctx->stream>>>(k, j->lanes, d_srcs, j->dist, j->chg);
+    else k_mss_seed<64><<<1, 64, 0, j->ctx->stream>>>(k, j->lanes, d_srcs, j->dist, j->chg);
+    APSPIE_HIP(hipGetLastError());
+    j->lanes += k;
+}
+
+void sssp_join_end(SsspJoin *j, float *d_out, apspie_stats *stats) {
+    if (j->K == 16) join_end<16>(*j, d_out, stats);
+    else join_end<64>(*j, d_out, stats);
+}
+
+void sssp_join_free(SsspJoin *j) { delete j; }
+
+namespace {
+
+template <int K>
+void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, float *out,
+                apspie_stats *stats) {
+    std::unique_ptr<SsspJoin, void (*)(SsspJoin *)> j(sssp_join_begin(ctx, g, srcs, k, K), sssp_join_free);
+    while (sssp_join_step(j.get())) {
+    }
+    sssp_join_end(j.get(), out, stats);
 }
 
 void check_sources(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k) {

@@ -80,7 +80,7 @@ struct apspie_server {
     std::vector<std::thread> workers;
     std::vector<double> lat_ms;
     double first_arrive = 0, last_done = 0;
-    int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_reqs = 0, trivial = 0;
+    int64_t submitted = 0, completed = 0, rejected = 0, batches = 0, batched_reqs = 0, trivial = 0, joined = 0;
     std::string err;
 
     size_t footprint(int kind, int g) const {
@@ -91,7 +91,58 @@ struct apspie_server {
     }
     void worker(int lane);
     void answer_trivial(std::vector<Request *> &rs);
+    int run_joined(apspie_ctx *ctx, int gi, int K, std::vector<Request *> &batch, int32_t *d_src, float *d_out);
 };
+
+// A batched SSSP with continuous batching: after each chunk of rounds, while
+// the union frontier is not empty, SSSP requests of the same graph that
+// wait in the queue (oldest first, as the budget admits them) take the
+// batch's free lanes; their sources start at the next round, and their rows
+// come out with the batch's.  The lane's context is held for the whole batch
+// (new knobs apply from its next batch).
+int apspie_server::run_joined(apspie_ctx *ctx, int gi, int K, std::vector<Request *> &batch, int32_t *d_src,
+                              float *d_out) {
+    std::lock_guard<std::mutex> clk(ctx->mu);
+    trav::SsspJoin *j = nullptr;
+    std::vector<Request *> more;
+    std::vector<int32_t> add;
+    const int rc = guarded(ctx, [&] {
+        j = trav::sssp_join_begin(ctx, &graphs[size_t(gi)], d_src, int(batch.size()), K);
+        while (trav::sssp_join_step(j)) {
+            const int lanes = trav::sssp_join_lanes(j);
+            if (lanes >= K) continue;
+            more.clear();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                for (Request *r : pending) {
+                    if ((int)more.size() == K - lanes) break;
+                    if (r->trivial || r->kind != APSPIE_REQ_SSSP || r->graph != gi) continue;
+                    if (used + r->footprint > opts.budget_bytes) break;
+                    used += r->footprint;
+                    r->picked = true;
+                    more.push_back(r);
+                }
+                if (!more.empty()) {
+                    pending.erase(std::remove_if(pending.begin(), pending.end(),
+                                                 [](const Request *r) { return r->picked; }),
+                                  pending.end());
+                    joined += int64_t(more.size());
+                }
+            }
+            if (more.empty()) continue;
+            batch.insert(batch.end(), more.begin(), more.end());
+            add.resize(more.size());
+            for (size_t i = 0; i < more.size(); ++i) add[i] = more[i]->src;
+            APSPIE_HIP(hipMemcpyAsync(d_src + lanes, add.data(), add.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+            trav::sssp_join_add(j, d_src + lanes, int(more.size()));
+        }
+        apspie_stats st;
+        trav::sssp_join_end(j, d_out, &st);
+        APSPIE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (j) trav::sssp_join_free(j);
+    return rc;
+}
 
 void apspie_server::answer_trivial(std::vector<Request *> &rs) {
     for (Request *r : rs) {
@@ -209,7 +260,11 @@ void apspie_server::worker(int lane) {
         srcs.resize(k);
         for (int i = 0; i < k; ++i) srcs[i] = batch[i]->src;
         int rc = apspie_memcpy_h2d(ctx, d_src, srcs.data(), size_t(k) * 4);
-        const size_t need = size_t(k) * size_t(g.m) * 4;
+        // a batched SSSP keeps rows for every lane that sources may join
+        const int plain_upto = kind == APSPIE_REQ_BFS ? opts.plain_bfs_upto : opts.plain_sssp_upto;
+        const bool joinable = kind == APSPIE_REQ_SSSP && k > plain_upto;
+        const int K = joinable ? (k <= 16 && ctx->tune.sssp_batch_lanes != 64 ? 16 : 64) : k;
+        const size_t need = size_t(K) * size_t(g.m) * 4;
         if (rc == APSPIE_OK && need > out_cap) {
             if (d_out) apspie_free(ctx, d_out);
             d_out = nullptr;
@@ -218,19 +273,19 @@ void apspie_server::worker(int lane) {
             if (rc == APSPIE_OK) out_cap = need;
         }
         // a batch of very few sources runs faster as plain launches
-        const int plain_upto = kind == APSPIE_REQ_BFS ? opts.plain_bfs_upto : opts.plain_sssp_upto;
         if (rc == APSPIE_OK && k <= plain_upto) {
             for (int i = 0; i < k && rc == APSPIE_OK; ++i) {
                 void *row = (char *)d_out + size_t(i) * g.m * 4;
                 rc = kind == APSPIE_REQ_BFS ? apspie_bfs(ctx, &g, srcs[i], (int32_t *)row, nullptr)
                                             : apspie_sssp(ctx, &g, srcs[i], (float *)row, nullptr);
             }
+        } else if (rc == APSPIE_OK && joinable) {
+            rc = run_joined(ctx, gi, K, batch, (int32_t *)d_src, (float *)d_out);
         } else if (rc == APSPIE_OK) {
             apspie_stats st;
-            rc = kind == 0 ? apspie_bfs_batch(ctx, &g, (const int32_t *)d_src, k, (int32_t *)d_out, &st)
-                           : apspie_sssp_batch(ctx, &g, (const int32_t *)d_src, k, (float *)d_out, &st);
+            rc = apspie_bfs_batch(ctx, &g, (const int32_t *)d_src, k, (int32_t *)d_out, &st);
         }
-        for (int i = 0; i < k && rc == APSPIE_OK; ++i)
+        for (size_t i = 0; i < batch.size() && rc == APSPIE_OK; ++i)  // joined requests too
             if (batch[i]->out)
                 rc = apspie_memcpy_d2h(ctx, batch[i]->out, (const char *)d_out + size_t(i) * g.m * 4,
                                        size_t(g.m) * 4);
@@ -250,7 +305,7 @@ void apspie_server::worker(int lane) {
             }
             if (rc != APSPIE_OK) err = apspie_last_error(ctx);
             batches++;
-            batched_reqs += k;
+            batched_reqs += int64_t(batch.size());
             released++;
             inflight--;
         }
@@ -396,6 +451,7 @@ int apspie_server_stats(apspie_server *s, apspie_server_report *st) {
     st->rejected = s->rejected;
     st->batches = s->batches;
     st->trivial = s->trivial;
+    st->joined = s->joined;
     st->mean_batch = s->batches ? double(s->batched_reqs) / double(s->batches) : 0.0;
     if (!s->lat_ms.empty()) {
         std::vector<double> v = s->lat_ms;
@@ -417,7 +473,7 @@ int apspie_server_reset_stats(apspie_server *s) {
     if (!s) return APSPIE_EINVAL;
     std::lock_guard<std::mutex> lk(s->mu);
     s->lat_ms.clear();
-    s->submitted = s->completed = s->rejected = s->batches = s->batched_reqs = s->trivial = 0;
+    s->submitted = s->completed = s->rejected = s->batches = s->batched_reqs = s->trivial = s->joined = 0;
     s->first_arrive = s->last_done = 0;
     return APSPIE_OK;
 }

@@ -110,3 +110,40 @@ def test_lane_tuning_keeps_results(ctx, pool):
     assert rep["completed"] == len(reqs)
     for i, r in enumerate(reqs):
         assert np.array_equal(outs[i], solo(ctx, graphs, r)), i
+
+
+def oracle_sssp(h, src):
+    import oracle as O
+    d = O.sssp_dijkstra(O.Csr(h.m, h.n, h.row_ptr, h.col_ind, h.val), src)
+    d[d == -1.0] = np.float32(1.70141e38)
+    return d
+
+
+@pytest.mark.parametrize("lanes_knob", [0, 64])
+def test_sssp_requests_join_running_batches(ctx, lanes_knob):
+    """Continuous batching: with one lane and every SSSP batch run batched
+    (no plain launches), SSSP requests submitted while a batch of their graph
+    runs take its free lanes at the next chunk of rounds (16-lane batches by
+    default, 64 with sssp_batch_lanes = 64).  Some requests join, and every
+    row equals the oracle's Dijkstra distances, whatever round it joined in."""
+    import time
+    h = A.rmat(15, 16, 9115, undirected=False, weight_seed=9116)
+    g = ctx.upload(h)
+    live = np.nonzero(h.degrees() > 0)[0]
+    srcs = [int(x) for x in np.random.RandomState(3).choice(live, 40, replace=False)]
+    outs = [np.empty(h.m, np.float32) for _ in srcs]
+    with A.Server([g], lanes=1, plain_sssp_upto=-1) as srv:
+        t = ctx.tuning()
+        t.update(batch_chunk=1, sssp_batch_lanes=lanes_knob)
+        srv.set_tuning(t)
+        srv.reset_stats()
+        tickets = []
+        for s, o in zip(srcs, outs):
+            tickets.append(srv.submit(A.REQ_SSSP, 0, s, o))
+            time.sleep(0.0003)
+        for tk in tickets:
+            srv.wait(tk)
+        rep = srv.report()
+    assert rep["completed"] == len(srcs) and rep["joined"] > 0, rep
+    for s, o in zip(srcs, outs):
+        assert np.array_equal(o, oracle_sssp(h, s)), s
