@@ -44,6 +44,7 @@ struct PickMs {
     int32_t *depth;
     const int *lvl;
     int m, k;
+    const int32_t *off;  // per source: the level its seed is picked at (a source that joined late)
     __device__ bool pick(int v, uint32_t &) const {
         if (v >= m) return false;
         uint64_t *vis = st + int64_t(v) * 2 * W, *next = vis + W;
@@ -92,7 +93,7 @@ struct PickMs {
                 while (any) {  // wave-uniform
                     const int b = __ffsll((unsigned long long)any) - 1;
                     const int src = w * 64 + b;
-                    if (((nw >> b) & 1ull) && src < k) depth[int64_t(src) * m + v] = level;
+                    if (((nw >> b) & 1ull) && src < k) depth[int64_t(src) * m + v] = level - off[src];
                     any &= any - 1;
                 }
             }
@@ -138,10 +139,20 @@ __global__ __launch_bounds__(256) void k_ms_init(int m, int k, const int32_t *sr
     control_clear(C, gtid, stride, ~0u);
 }
 
+// sources srcs[0, k) in lanes lane0 .. lane0 + k - 1, whose seeds the
+// compaction of round `level` picks: their depth rows start at -1 and their
+// depths count from that round
 template <int W>
-__global__ void k_ms_seed(int k, const int32_t *srcs, uint64_t *st) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < k) atomicOr((unsigned long long *)&st[int64_t(srcs[s]) * 2 * W + W + s / 64], 1ull << (s % 64));
+__global__ __launch_bounds__(256) void k_ms_seed(int m, int k, int lane0, int level, const int32_t *srcs,
+                                                 uint64_t *st, int32_t *depth, int32_t *off, bool rows) {
+    const int64_t gtid = int64_t(blockIdx.x) * 256 + threadIdx.x, stride = int64_t(gridDim.x) * 256;
+    if (rows)
+        for (int64_t i = gtid; i < int64_t(m) * k; i += stride) depth[int64_t(lane0) * m + i] = -1;
+    if (gtid < k) {
+        const int s = lane0 + int(gtid);
+        off[s] = level;
+        atomicOr((unsigned long long *)&st[int64_t(srcs[gtid]) * 2 * W + W + s / 64], 1ull << (s % 64));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -365,55 +376,65 @@ std::string graph_key(const char *what, const apspie_csr *g, const Bufs &b, std:
 template <int W>
 void bfs_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, int32_t *depth,
                apspie_stats *stats) {
-    const Bufs b = get_bufs(ctx, g->m, g->nnz);
-    const size_t words = size_t(g->m) * W;
-    uint64_t *st = (uint64_t *)ctx->slot(S_AUX0, words * 2 * 8);
-    uint64_t *fr = (uint64_t *)ctx->slot(S_AUX2, words * 8);
-    APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * std::max(W, k), 256), 8192u));
-    k_ms_init<W><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, st, depth, b.nblk, b.status, b.C);
-    k_ms_seed<W><<<ceil_div(k, 256), 256, 0, ctx->stream>>>(k, srcs, st);
-    const PickMs<W> pk{st, fr, depth, &b.C->cur_level, g->m, k};
-    const OpMs<W> op{fr, st};
-    auto compact = [&] {
-        k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
-                                                               b.C);
-    };
-    // seeding round (round 0) runs eagerly, the rest from the captured graph
-    k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b),
-                                                           b.C);
-    APSPIE_HIP(hipGetLastError());
-    const std::string key = graph_key("msbfs", g, b, {st, fr, depth, b.status, b.tiles}, W * 1000 + k);
-    run_rounds(ctx, b, key, g->m + 2, [&] {
-        k_expand<OpMs<W>><<<b.expand_grid, WG, 0, ctx->stream>>>(op, -1, g->m, b.C, queue_of(b), g->col_ind);
-        compact();
-    });
-    APSPIE_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    fill_stats(ctx, b, MODE_BFS, g->m, stats);
-    if (stats) stats->bytes_alg = 0.0;
+    std::unique_ptr<BatchJoin, void (*)(BatchJoin *)> j(
+        batch_join_begin(ctx, g, APSPIE_REQ_BFS, srcs, k, 64 * W, depth), batch_join_free);
+    while (batch_join_step(j.get())) {
+    }
+    batch_join_end(j.get(), depth, stats);
 }
 
 }  // namespace
 
-// Batched SSSP in K lanes that more sources may join between chunks of
-// rounds: sources seeded in lanes lane0 .. lane0 + k - 1 start from distance
-// 0 and are picked by the next compaction; the rounds relax live distances
-// until nothing changes, so a source's distances are its shortest paths
-// whichever round it joins in.
-struct SsspJoin {
+// A batch in K lanes (BFS: 64, 128 or 256; SSSP: 16 or 64) that more
+// sources may join between chunks of rounds.  SSSP: a joined source starts
+// at distance 0 and is picked by the next compaction; the rounds relax live
+// distances until nothing changes, so its distances are its shortest paths
+// whichever round it joins in.  BFS: a joined source's seed is picked by the
+// next compaction, whose round is the source's depth offset, so its depths
+// count from there.
+struct BatchJoin {
     apspie_ctx *ctx = nullptr;
     const apspie_csr *g = nullptr;
-    int K = 0, lanes = 0, rounds = 0;
+    int kind = 0, K = 0, lanes = 0, rounds = 0, level = 0;
     Bufs b;
     uint32_t *dist = nullptr;
-    uint64_t *chg = nullptr, *fr = nullptr;
+    uint64_t *chg = nullptr, *fr = nullptr, *st = nullptr;
+    int32_t *depth = nullptr, *off = nullptr;
     hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
+template <int W>
+void join_begin_bfs(BatchJoin &j, const int32_t *srcs, int k) {
+    apspie_ctx *ctx = j.ctx;
+    const apspie_csr *g = j.g;
+    j.b = get_bufs(ctx, g->m, g->nnz);
+    const Bufs &b = j.b;
+    const size_t words = size_t(g->m) * W;
+    j.st = (uint64_t *)ctx->slot(S_AUX0, words * 2 * 8);
+    j.fr = (uint64_t *)ctx->slot(S_AUX2, words * 8);
+    j.off = (int32_t *)ctx->slot(S_AUX3, 256 * 4);
+    APSPIE_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(g->m) * std::max(W, k), 256), 8192u));
+    k_ms_init<W><<<gi, 256, 0, ctx->stream>>>(g->m, k, srcs, j.st, j.depth, b.nblk, b.status, b.C);
+    k_ms_seed<W><<<1, 256, 0, ctx->stream>>>(g->m, k, 0, 0, srcs, j.st, j.depth, j.off, false);
+    const PickMs<W> pk{j.st, j.fr, j.depth, &b.C->cur_level, g->m, 64 * W, j.off};
+    const OpMs<W> op{j.fr, j.st};
+    // seeding round (round 0) runs eagerly, the rest from the captured graph
+    k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 1, g->row_ptr, queue_of(b), b.C);
+    APSPIE_HIP(hipGetLastError());
+    const std::string key = graph_key("msbfs", g, b, {j.st, j.fr, j.depth, j.off, b.status, b.tiles}, W);
+    j.exec = chunk_exec(ctx, b, key, [&] {
+        k_expand<OpMs<W>><<<b.expand_grid, WG, 0, ctx->stream>>>(op, -1, g->m, b.C, queue_of(b), g->col_ind);
+        k_compact<PickMs<W>><<<b.nblk, WGC, 0, ctx->stream>>>(pk, -1, g->m, b.nblk, 0, g->row_ptr, queue_of(b),
+                                                               b.C);
+    });
+    j.lanes = k;
+}
+
 template <int K>
-void join_begin(SsspJoin &j, const int32_t *srcs, int k) {
+void join_begin(BatchJoin &j, const int32_t *srcs, int k) {
     apspie_ctx *ctx = j.ctx;
     const apspie_csr *g = j.g;
     j.b = get_bufs(ctx, g->m, g->nnz);
@@ -441,7 +462,7 @@ void join_begin(SsspJoin &j, const int32_t *srcs, int k) {
 }
 
 template <int K>
-void join_end(SsspJoin &j, float *out, apspie_stats *stats) {
+void join_end(BatchJoin &j, float *out, apspie_stats *stats) {
     apspie_ctx *ctx = j.ctx;
     k_mss_transpose<K><<<ceil_div(j.g->m, 64), 256, 0, ctx->stream>>>(j.g->m, j.lanes, j.dist, (uint32_t *)out);
     APSPIE_HIP(hipGetLastError());
@@ -455,46 +476,82 @@ void join_end(SsspJoin &j, float *out, apspie_stats *stats) {
 
 }  // namespace
 
-SsspJoin *sssp_join_begin(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k, int K) {
-    APSPIE_CHECK(K == 16 || K == 64, APSPIE_EINVAL, "a joinable batch has 16 or 64 lanes");
+BatchJoin *batch_join_begin(apspie_ctx *ctx, const apspie_csr *g, int kind, const int32_t *d_srcs, int k, int K,
+                            void *d_out) {
+    if (kind == APSPIE_REQ_BFS)
+        APSPIE_CHECK(K == 64 || K == 128 || K == 256, APSPIE_EINVAL, "a joinable BFS batch has 64, 128 or 256 lanes");
+    else
+        APSPIE_CHECK(K == 16 || K == 64, APSPIE_EINVAL, "a joinable SSSP batch has 16 or 64 lanes");
     APSPIE_CHECK(k >= 1 && k <= K, APSPIE_ELIMIT, "sources beyond the batch's lanes");
-    std::unique_ptr<SsspJoin> j(new SsspJoin);
+    std::unique_ptr<BatchJoin> j(new BatchJoin);
     j->ctx = ctx;
     j->g = g;
+    j->kind = kind;
     j->K = K;
-    if (K == 16) join_begin<16>(*j, d_srcs, k);
-    else join_begin<64>(*j, d_srcs, k);
+    if (kind == APSPIE_REQ_BFS) {
+        j->depth = (int32_t *)d_out;
+        if (K == 64) join_begin_bfs<1>(*j, d_srcs, k);
+        else if (K == 128) join_begin_bfs<2>(*j, d_srcs, k);
+        else join_begin_bfs<4>(*j, d_srcs, k);
+    } else if (K == 16) {
+        join_begin<16>(*j, d_srcs, k);
+    } else {
+        join_begin<64>(*j, d_srcs, k);
+    }
     return j.release();
 }
 
-bool sssp_join_step(SsspJoin *j) { return chunk_step(j->ctx, j->b, j->exec, j->rounds, j->g->m * 2 + 4); }
+bool batch_join_step(BatchJoin *j) {
+    const bool more = chunk_step(j->ctx, j->b, j->exec, j->rounds, j->kind == APSPIE_REQ_BFS ? j->g->m + 2 : j->g->m * 2 + 4);
+    j->level = ((const Control *)j->ctx->pinned)->cur_level;  // the round the next compaction runs
+    return more;
+}
 
-int sssp_join_lanes(const SsspJoin *j) { return j->lanes; }
+int batch_join_lanes(const BatchJoin *j) { return j->lanes; }
 
-void sssp_join_add(SsspJoin *j, const int32_t *d_srcs, int k) {
+void batch_join_add(BatchJoin *j, const int32_t *d_srcs, int k) {
     APSPIE_CHECK(k >= 1 && j->lanes + k <= j->K, APSPIE_ELIMIT, "sources beyond the batch's lanes");
-    if (j->K == 16) k_mss_seed<16><<<1, 64, 0, j->ctx->stream>>>(k, j->lanes, d_srcs, j->dist, j->chg);
-    else k_mss_seed<64><<<1, 64, 0, j->ctx->stream>>>(k, j->lanes, d_srcs, j->dist, j->chg);
+    hipStream_t st = j->ctx->stream;
+    const int m = j->g->m;
+    if (j->kind == APSPIE_REQ_BFS) {
+        const unsigned gi = std::max(1u, std::min(ceil_div(int64_t(m) * k, 256), 8192u));
+        if (j->K == 64) k_ms_seed<1><<<gi, 256, 0, st>>>(m, k, j->lanes, j->level, d_srcs, j->st, j->depth, j->off, true);
+        else if (j->K == 128)
+            k_ms_seed<2><<<gi, 256, 0, st>>>(m, k, j->lanes, j->level, d_srcs, j->st, j->depth, j->off, true);
+        else k_ms_seed<4><<<gi, 256, 0, st>>>(m, k, j->lanes, j->level, d_srcs, j->st, j->depth, j->off, true);
+    } else if (j->K == 16) {
+        k_mss_seed<16><<<1, 64, 0, st>>>(k, j->lanes, d_srcs, j->dist, j->chg);
+    } else {
+        k_mss_seed<64><<<1, 64, 0, st>>>(k, j->lanes, d_srcs, j->dist, j->chg);
+    }
     APSPIE_HIP(hipGetLastError());
     j->lanes += k;
 }
 
-void sssp_join_end(SsspJoin *j, float *d_out, apspie_stats *stats) {
-    if (j->K == 16) join_end<16>(*j, d_out, stats);
-    else join_end<64>(*j, d_out, stats);
+void batch_join_end(BatchJoin *j, void *d_out, apspie_stats *stats) {
+    if (j->kind == APSPIE_REQ_BFS) {
+        APSPIE_HIP(hipEventRecord(j->ctx->ev1, j->ctx->stream));
+        fill_stats(j->ctx, j->b, MODE_BFS, j->g->m, stats);
+        if (stats) stats->bytes_alg = 0.0;
+    } else if (j->K == 16) {
+        join_end<16>(*j, (float *)d_out, stats);
+    } else {
+        join_end<64>(*j, (float *)d_out, stats);
+    }
 }
 
-void sssp_join_free(SsspJoin *j) { delete j; }
+void batch_join_free(BatchJoin *j) { delete j; }
 
 namespace {
 
 template <int K>
 void sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *srcs, int k, float *out,
                 apspie_stats *stats) {
-    std::unique_ptr<SsspJoin, void (*)(SsspJoin *)> j(sssp_join_begin(ctx, g, srcs, k, K), sssp_join_free);
-    while (sssp_join_step(j.get())) {
+    std::unique_ptr<BatchJoin, void (*)(BatchJoin *)> j(
+        batch_join_begin(ctx, g, APSPIE_REQ_SSSP, srcs, k, K, out), batch_join_free);
+    while (batch_join_step(j.get())) {
     }
-    sssp_join_end(j.get(), out, stats);
+    batch_join_end(j.get(), out, stats);
 }
 
 void check_sources(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k) {

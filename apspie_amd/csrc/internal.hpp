@@ -163,20 +163,23 @@ int guarded(apspie_ctx *ctx, F &&f) {
 void validate_csr(const apspie_csr *g, bool need_val);
 
 namespace trav {
-// Batched SSSP (16 or 64 lanes) that more sources may join between chunks of
-// captured rounds, for the serving loop's continuous batching (batch.hip):
-// begin seeds k sources in lanes 0 .. k-1 (d_srcs on the device); step runs
-// one chunk and returns false once the union frontier is empty; add seeds
-// more sources in the next lanes, only after a step returned true (the
-// rounds go on and take them); end writes rows 0 .. lanes-1 (lanes x m
-// floats at d_out) and stats.  All throw Error; free releases the handle.
-struct SsspJoin;
-SsspJoin *sssp_join_begin(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_srcs, int k, int K);
-bool sssp_join_step(SsspJoin *j);
-int sssp_join_lanes(const SsspJoin *j);
-void sssp_join_add(SsspJoin *j, const int32_t *d_srcs, int k);
-void sssp_join_end(SsspJoin *j, float *d_out, apspie_stats *stats);
-void sssp_join_free(SsspJoin *j);
+// A batched BFS (64, 128 or 256 lanes) or SSSP (16 or 64 lanes) that more
+// sources may join between chunks of captured rounds, for the serving loop's
+// continuous batching (batch.hip): begin seeds k sources in lanes 0 .. k-1
+// (d_srcs on the device) with the output rows at d_out (K x m depths for BFS;
+// SSSP writes its rows at end); step runs one chunk and returns false once
+// the union frontier is empty; add seeds more sources in the next lanes,
+// only after a step returned true (the rounds go on and take them); end
+// writes rows 0 .. lanes-1 and stats.  All throw Error; free releases the
+// handle.
+struct BatchJoin;
+BatchJoin *batch_join_begin(apspie_ctx *ctx, const apspie_csr *g, int kind, const int32_t *d_srcs, int k, int K,
+                            void *d_out);
+bool batch_join_step(BatchJoin *j);
+int batch_join_lanes(const BatchJoin *j);
+void batch_join_add(BatchJoin *j, const int32_t *d_srcs, int k);
+void batch_join_end(BatchJoin *j, void *d_out, apspie_stats *stats);
+void batch_join_free(BatchJoin *j);
 }  // namespace trav
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }

@@ -91,32 +91,36 @@ struct apspie_server {
     }
     void worker(int lane);
     void answer_trivial(std::vector<Request *> &rs);
-    int run_joined(apspie_ctx *ctx, int gi, int K, std::vector<Request *> &batch, int32_t *d_src, float *d_out);
+    int run_joined(apspie_ctx *ctx, int kind, int gi, int K, std::vector<Request *> &batch, int32_t *d_src,
+                   void *d_out);
 };
 
-// A batched SSSP with continuous batching: after each chunk of rounds, while
-// the union frontier is not empty, SSSP requests of the same graph that
-// wait in the queue (oldest first, as the budget admits them) take the
-// batch's free lanes; their sources start at the next round, and their rows
-// come out with the batch's.  The lane's context is held for the whole batch
-// (new knobs apply from its next batch).
-int apspie_server::run_joined(apspie_ctx *ctx, int gi, int K, std::vector<Request *> &batch, int32_t *d_src,
-                              float *d_out) {
+// A batch with continuous batching: after each chunk of rounds, while the
+// union frontier is not empty, requests of the same kind and graph that wait
+// in the queue (oldest first, as the budget admits them) take the batch's
+// free lanes; their sources start at the next round (SSSP distances are the
+// same whatever round a source starts in; BFS depths count from the round
+// that picks the source), and their rows come out with the batch's.  The
+// lane's context is held for the whole batch (new knobs apply from its next
+// batch).
+int apspie_server::run_joined(apspie_ctx *ctx, int kind, int gi, int K, std::vector<Request *> &batch,
+                              int32_t *d_src, void *d_out) {
     std::lock_guard<std::mutex> clk(ctx->mu);
-    trav::SsspJoin *j = nullptr;
+    trav::BatchJoin *j = nullptr;
     std::vector<Request *> more;
     std::vector<int32_t> add;
     const int rc = guarded(ctx, [&] {
-        j = trav::sssp_join_begin(ctx, &graphs[size_t(gi)], d_src, int(batch.size()), K);
-        while (trav::sssp_join_step(j)) {
-            const int lanes = trav::sssp_join_lanes(j);
-            if (lanes >= K) continue;
+        j = trav::batch_join_begin(ctx, &graphs[size_t(gi)], kind, d_src, int(batch.size()), K, d_out);
+        const int limit = std::min(K, kind == APSPIE_REQ_BFS ? opts.max_bfs_batch : opts.max_sssp_batch);
+        while (trav::batch_join_step(j)) {
+            const int lanes = trav::batch_join_lanes(j);
+            if (lanes >= limit) continue;
             more.clear();
             {
                 std::lock_guard<std::mutex> lk(mu);
                 for (Request *r : pending) {
-                    if ((int)more.size() == K - lanes) break;
-                    if (r->trivial || r->kind != APSPIE_REQ_SSSP || r->graph != gi) continue;
+                    if ((int)more.size() == limit - lanes) break;
+                    if (r->trivial || r->kind != kind || r->graph != gi) continue;
                     if (used + r->footprint > opts.budget_bytes) break;
                     used += r->footprint;
                     r->picked = true;
@@ -134,13 +138,13 @@ int apspie_server::run_joined(apspie_ctx *ctx, int gi, int K, std::vector<Reques
             add.resize(more.size());
             for (size_t i = 0; i < more.size(); ++i) add[i] = more[i]->src;
             APSPIE_HIP(hipMemcpyAsync(d_src + lanes, add.data(), add.size() * 4, hipMemcpyHostToDevice, ctx->stream));
-            trav::sssp_join_add(j, d_src + lanes, int(more.size()));
+            trav::batch_join_add(j, d_src + lanes, int(more.size()));
         }
         apspie_stats st;
-        trav::sssp_join_end(j, d_out, &st);
+        trav::batch_join_end(j, d_out, &st);
         APSPIE_HIP(hipStreamSynchronize(ctx->stream));
     });
-    if (j) trav::sssp_join_free(j);
+    if (j) trav::batch_join_free(j);
     return rc;
 }
 
@@ -260,10 +264,13 @@ void apspie_server::worker(int lane) {
         srcs.resize(k);
         for (int i = 0; i < k; ++i) srcs[i] = batch[i]->src;
         int rc = apspie_memcpy_h2d(ctx, d_src, srcs.data(), size_t(k) * 4);
-        // a batched SSSP keeps rows for every lane that sources may join
+        // a batch keeps rows for every lane that sources may join: SSSP 16
+        // or 64 lanes (as apspie_sssp_batch picks), BFS 64, 128 or 256
         const int plain_upto = kind == APSPIE_REQ_BFS ? opts.plain_bfs_upto : opts.plain_sssp_upto;
-        const bool joinable = kind == APSPIE_REQ_SSSP && k > plain_upto;
-        const int K = joinable ? (k <= 16 && ctx->tune.sssp_batch_lanes != 64 ? 16 : 64) : k;
+        const bool joinable = k > plain_upto;
+        int K = k;
+        if (joinable && kind == APSPIE_REQ_SSSP) K = k <= 16 && ctx->tune.sssp_batch_lanes != 64 ? 16 : 64;
+        if (joinable && kind == APSPIE_REQ_BFS) K = k <= 64 ? 64 : k <= 128 ? 128 : 256;
         const size_t need = size_t(K) * size_t(g.m) * 4;
         if (rc == APSPIE_OK && need > out_cap) {
             if (d_out) apspie_free(ctx, d_out);
@@ -279,11 +286,8 @@ void apspie_server::worker(int lane) {
                 rc = kind == APSPIE_REQ_BFS ? apspie_bfs(ctx, &g, srcs[i], (int32_t *)row, nullptr)
                                             : apspie_sssp(ctx, &g, srcs[i], (float *)row, nullptr);
             }
-        } else if (rc == APSPIE_OK && joinable) {
-            rc = run_joined(ctx, gi, K, batch, (int32_t *)d_src, (float *)d_out);
         } else if (rc == APSPIE_OK) {
-            apspie_stats st;
-            rc = apspie_bfs_batch(ctx, &g, (const int32_t *)d_src, k, (int32_t *)d_out, &st);
+            rc = run_joined(ctx, kind, gi, K, batch, (int32_t *)d_src, d_out);
         }
         for (size_t i = 0; i < batch.size() && rc == APSPIE_OK; ++i)  // joined requests too
             if (batch[i]->out)

@@ -147,3 +147,33 @@ def test_sssp_requests_join_running_batches(ctx, lanes_knob):
     assert rep["completed"] == len(srcs) and rep["joined"] > 0, rep
     for s, o in zip(srcs, outs):
         assert np.array_equal(o, oracle_sssp(h, s)), s
+
+
+def test_bfs_requests_join_running_batches(ctx):
+    """Continuous batching for BFS: requests submitted while a BFS batch of
+    their graph runs take its free lanes at the next chunk of rounds (every
+    batch batched, one lane); each joined source's depths count from the round
+    that picks it, so every row equals the oracle's BFS."""
+    import time
+    import oracle as O
+    h = A.rmat(15, 16, 9215, undirected=False)
+    g = ctx.upload(h)
+    live = np.nonzero(h.degrees() > 0)[0]
+    srcs = [int(x) for x in np.random.RandomState(4).choice(live, 60, replace=False)]
+    outs = [np.empty(h.m, np.int32) for _ in srcs]
+    with A.Server([g], lanes=1, plain_bfs_upto=-1) as srv:
+        t = ctx.tuning()
+        t.update(batch_chunk=1)
+        srv.set_tuning(t)
+        srv.reset_stats()
+        tickets = []
+        for s, o in zip(srcs, outs):
+            tickets.append(srv.submit(A.REQ_BFS, 0, s, o))
+            time.sleep(0.0002)
+        for tk in tickets:
+            srv.wait(tk)
+        rep = srv.report()
+    assert rep["completed"] == len(srcs) and rep["joined"] > 0, rep
+    oc = O.Csr(h.m, h.n, h.row_ptr, h.col_ind, None)
+    for s, o in zip(srcs, outs):
+        assert np.array_equal(o, O.bfs(oc, s)[0]), s
