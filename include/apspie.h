@@ -435,10 +435,11 @@ int apspie_sssp_batch(apspie_ctx *ctx, const apspie_csr *g, const int32_t *d_src
  * No counterpart in the reference.  A pool of device-resident graphs
  * (borrowed: they must outlive the server); BFS / SSSP requests are queued and
  * served in batches by `lanes` worker threads, each with its own stream, so
- * batches overlap on the device.  Requests that arrive meanwhile join the
- * next batch a lane forms; an SSSP request also joins a running SSSP batch of
- * its graph that has a free lane, at the next chunk of rounds (continuous
- * batching: its distances are the same whatever round it starts in).  Admission is by device-memory footprint against
+ * batches overlap on the device.  Requests that arrive meanwhile join a
+ * running batch of their kind and graph that has a free lane, at its next
+ * chunk of rounds (continuous batching: distances are the same whatever round
+ * a source starts in, and a joined BFS source's depths count from the round
+ * that picks it), or else the next batch a lane forms.  Admission is by device-memory footprint against
  * budget_bytes.  A request whose source has no out-edges needs no traversal
  * and is answered on the host without a lane or device memory.  All entry
  * points are thread-safe. */
@@ -458,7 +459,7 @@ typedef struct {
     double p50_ms, p99_ms, mean_ms, max_ms;
     double qps;      /* completed / (last completion - first arrival) */
     int64_t trivial; /* answered on the host: the source has no out-edges */
-    int64_t joined;  /* SSSP requests that joined a batch already running (continuous batching) */
+    int64_t joined;  /* requests that joined a batch already running (continuous batching) */
 } apspie_server_report;
 
 #define APSPIE_REQ_BFS 0
