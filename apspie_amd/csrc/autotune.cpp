@@ -124,7 +124,7 @@ extern "C" int apspie_autotune(apspie_ctx *ctx, const apspie_csr *g, int32_t src
         if (g->m <= (1 << 23)) tune(&apspie_tuning::sssp_bin_ratio, {0, 2, 4}, APSPIE_TUNE_SSSP);
         tune(&apspie_tuning::sssp_delta, {0, 16, 32, 64, 128}, APSPIE_TUNE_SSSP);
         tune(&apspie_tuning::sssp_hint, {0, 1}, APSPIE_TUNE_SSSP);
-        tune(&apspie_tuning::expand_wg_per_cu, {4, 8, 16}, plain | batched);
+        tune(&apspie_tuning::expand_wg_per_cu, {4, 8, 16, 32}, plain | batched);
         tune(&apspie_tuning::round_chunk, {3, 4, 5, 6, 8, 12}, plain);
         tune(&apspie_tuning::batch_chunk, {2, 4, 8}, batched);
         if (kb <= 16) tune(&apspie_tuning::sssp_batch_lanes, {16, 64}, APSPIE_TUNE_SSSP_BATCH);

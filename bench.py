@@ -449,6 +449,12 @@ def bench_served(ctx, n):
         serving.run_served(srv, reqs[:500], open_loop=False)  # warm-up: contexts, buffers
         res["served_open_loop"] = serving.run_served(srv, reqs, open_loop=True)
         log(f"served open-loop: {res['served_open_loop']}")
+        # the same requests arriving at 15,000 a second (half the saturated
+        # rate): batches run while requests keep arriving, which join them
+        # between chunks of rounds (the report's `joined`)
+        fast = serving.make_stream(ms, n=n, rate=15000.0)
+        res["served_open_loop_15k"] = serving.run_served(srv, fast, open_loop=True)
+        log(f"served open-loop at 15K/s: {res['served_open_loop_15k']}")
         res["served_saturation"] = serving.run_served(srv, reqs, open_loop=False)
         log(f"served saturation: {res['served_saturation']}")
     # plain over the whole stream: every request launched, then with the
