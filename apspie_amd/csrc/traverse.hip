@@ -1004,7 +1004,7 @@ void run(apspie_ctx *ctx, const apspie_csr *g, int32_t src, int32_t *depth, floa
                     OpRelax{g->val, dist}, level, g->m, nb, b.C, queue_of(b), g->col_ind, g->val, nullptr, bin_out,
                     bin_tbl, bin_wcnt);
             ctx->prof_end();
-            if (level > 0) {  // the source's round is never binned
+            if (level > 1) {  // level 1 expands the source's round, which is never binned
                 ctx->prof_begin(APSPIE_KCLASS_APPLY);
                 k_bin_scan<<<nb, 256, 0, ctx->stream>>>(level, nb, bin_grid, b.C, bin_tbl, bin_tot);
                 k_bin_scatter<<<bin_grid, WG, 0, ctx->stream>>>(level, nb, b.C, bin_out, bin_tbl, bin_tot, bin_wcnt,
