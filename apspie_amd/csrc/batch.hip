@@ -75,6 +75,9 @@ struct PickMs {
     // stores, per source present, the level at the lanes holding it
     __device__ void on_balls(uint64_t ball, int vbase, int lane, int rounds) const {
         const int level = *(volatile const int *)lvl;
+        int offw[W];  // lane l: the offset of source w * 64 + l (read by readlane below)
+#pragma unroll
+        for (int w = 0; w < W; ++w) offw[w] = off[w * 64 + lane];
         for (int r = 0; r < rounds; ++r) {
             const uint64_t picked = __shfl(ball, r, WAVE);
             if (!picked) continue;
@@ -92,8 +95,10 @@ struct PickMs {
                 }
                 while (any) {  // wave-uniform
                     const int b = __ffsll((unsigned long long)any) - 1;
-                    const int src = w * 64 + b;
-                    if (((nw >> b) & 1ull) && src < k) depth[int64_t(src) * m + v] = level - off[src];
+                    const int bu = __builtin_amdgcn_readfirstlane(b);
+                    const int src = w * 64 + bu;
+                    const int d = level - __builtin_amdgcn_readlane(offw[w], bu);
+                    if (((nw >> b) & 1ull) && src < k) depth[int64_t(src) * m + v] = d;
                     any &= any - 1;
                 }
             }
