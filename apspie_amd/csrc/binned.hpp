@@ -91,16 +91,15 @@ __device__ __forceinline__ void bin_tiles(const Round &cur, int m, int nb, const
             const int s = wid * SEG + r * WAVE + lane;
             pos[r] = s < elim ? s_base[own[r]] + s : -1;
         }
-#pragma unroll
-        for (int r = 0; r < IPT; ++r) col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
-        float wt[IPT];
-        uint32_t hc[IPT];
+        float wt[IPT];  // loaded beside the columns: it needs only the edge's position
 #pragma unroll
         for (int r = 0; r < IPT; ++r) {
-            const bool ok = (unsigned)col[r] < (unsigned)m;
-            wt[r] = (val && ok) ? val[pos[r]] : 1.0f;
-            hc[r] = (HINT && ok) ? uint32_t(hint[col[r]]) : 255u;
+            col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
+            wt[r] = (val && pos[r] >= 0) ? val[pos[r]] : 1.0f;
         }
+        uint32_t hc[IPT];
+#pragma unroll
+        for (int r = 0; r < IPT; ++r) hc[r] = (HINT && (unsigned)col[r] < (unsigned)m) ? uint32_t(hint[col[r]]) : 255u;
         uint32_t cand[IPT];
         uint32_t keep = 0;
 #pragma unroll

@@ -139,6 +139,14 @@ template <class P, class = void>
 struct has_direction : std::false_type {};
 template <class P>
 struct has_direction<P, std::void_t<decltype(&P::direction)>> : std::true_type {};
+// An op with weight(p) has its edge weight loaded by the expansion beside the
+// column and stored into Pre::w after load(), which then fetches only the
+// target's data: the weight load no longer waits for the column.
+template <class P, class = void>
+struct has_weight : std::false_type {};
+template <class P>
+struct has_weight<P, std::void_t<decltype(&P::weight)>> : std::true_type {};
+
 template <class P, class = void>
 struct has_mid : std::false_type {};
 template <class P>
@@ -538,8 +546,12 @@ __device__ __forceinline__ void expand_tiles_in(const Op &op, int m, const Round
             const int s = wid * SEG + r * WAVE + lane;
             pos[r] = s < elim ? s_base[own[r]] + s : -1;
         }
+        float wt[IPT];  // an op's edge weight, loaded beside the column (it needs only the position)
 #pragma unroll
-        for (int r = 0; r < IPT; ++r) col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
+        for (int r = 0; r < IPT; ++r) {
+            col[r] = pos[r] >= 0 ? col_ind[pos[r]] : -1;
+            if constexpr (has_weight<Op>::value) wt[r] = pos[r] >= 0 ? op.weight(pos[r]) : 0.0f;
+        }
         constexpr int B = op_batch<Op>::value;
         static_assert(IPT % B == 0, "batch divides IPT");
 #pragma unroll
@@ -547,9 +559,11 @@ __device__ __forceinline__ void expand_tiles_in(const Op &op, int m, const Round
             typename Op::Pre pre[B];
 #pragma unroll
             for (int r = 0; r < B; ++r)
-                if ((unsigned)col[r0 + r] < (unsigned)m)
+                if ((unsigned)col[r0 + r] < (unsigned)m) {
                     pre[r] = op.load(col[r0 + r], pos[r0 + r], Op::kVal ? s_val[own[r0 + r]] : 0u,
                                      Op::kVert ? s_vert[own[r0 + r]] : 0);
+                    if constexpr (has_weight<Op>::value) pre[r].w = wt[r0 + r];
+                }
             if constexpr (has_mid<Op>::value) {
 #pragma unroll
                 for (int r = 0; r < B; ++r)

@@ -187,7 +187,8 @@ struct OpRelax {
     };
     const float *val;
     uint32_t *dist;
-    __device__ Pre load(int c, int p, uint32_t, int) const { return Pre{val ? val[p] : 1.0f, dist[c]}; }
+    __device__ float weight(int p) const { return val ? val[p] : 1.0f; }
+    __device__ Pre load(int c, int, uint32_t, int) const { return Pre{0.0f, dist[c]}; }  // w: weight()
     __device__ void apply(int c, int, uint32_t du, int, const Pre &pr) const {
         const uint32_t nb = __float_as_uint(__uint_as_float(du) + pr.w);
         if (nb < pr.d) __hip_atomic_fetch_min(&dist[c], nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -195,7 +196,7 @@ struct OpRelax {
 };
 
 // OpRelax behind the distance bound codes: the loads fetch the weight and the
-// target's code, the middle phase probes the distance of the targets whose
+// target's code (the weight beside the column), the middle phase probes the distance of the targets whose
 // bound the candidate is below (the others get 0, which no candidate is
 // below), and the apply issues the atomic as OpRelax does.
 struct OpRelaxHint {
@@ -207,7 +208,8 @@ struct OpRelaxHint {
     const float *val;
     uint32_t *dist;
     const uint8_t *hint;
-    __device__ Pre load(int c, int p, uint32_t, int) const { return Pre{val ? val[p] : 1.0f, hint[c], 0u}; }
+    __device__ float weight(int p) const { return val ? val[p] : 1.0f; }
+    __device__ Pre load(int c, int, uint32_t, int) const { return Pre{0.0f, hint[c], 0u}; }  // w: weight()
     __device__ void mid(int c, int, uint32_t du, int, Pre &pr) const {
         const uint32_t nb = __float_as_uint(__uint_as_float(du) + pr.w);
         pr.d = nb < hint_bound(pr.h) ? dist[c] : 0u;
