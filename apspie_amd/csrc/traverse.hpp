@@ -11,7 +11,7 @@ namespace trav {
 
 constexpr int WG = 256;           // threads per expansion workgroup (4 waves of 64)
 constexpr int WGC = 1024;         // threads per compaction workgroup (16 waves)
-constexpr int VPT = 8;           // vertices per thread in the compaction pass
+constexpr int VPT = 4;           // vertices per thread in the compaction pass
 constexpr int CHUNK = WGC * VPT;  // vertices per compaction workgroup
 constexpr int TILE = 2048;        // frontier edges per expansion tile
 constexpr int IPT = TILE / WG;    // edges per thread per tile
